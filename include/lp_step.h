@@ -1,0 +1,173 @@
+/*
+ * lp_step.h -- C ABI of the MI355X (gfx950) daily agent step of laser-polio.
+ *
+ * The library (liblpstep.so, built from laser-polio_amd/csrc) runs one or many
+ * simulated days of the laser-polio agent-based model on one GPU.  Every agent,
+ * node, schedule and results buffer is owned by the caller (device memory,
+ * e.g. torch tensors) and handed over as a plain pointer + element count; the
+ * library owns only scratch buffers, its HIP stream and its captured day graph.
+ *
+ * Reference interface replaced (InstituteforDiseaseModeling/laser-polio 0.2.31):
+ *   lp_step_vital         VitalDynamics_ABM.step      src/laser_polio/model.py:1689-1757
+ *                         (+ get_deaths               model.py:1767-1781)
+ *   lp_step_disease       DiseaseState_ABM.step       model.py:715-795
+ *                         (+ disease_state_step(_kernel) model.py:344-454)
+ *   lp_step_ri            RI_ABM.step + fast_ri       model.py:1920-1985, 1784-1855
+ *   lp_step_sia           SIA_ABM.step + fast_sia     model.py:2099-2156, 1994-2060
+ *   lp_step_transmission  Transmission_ABM.step       model.py:1281-1459
+ *                         (+ tx_step_prep model.py:932-1007, tx_infect_nb model.py:1010-1149)
+ *   lp_log                Transmission_ABM.log + count_SEIRP  model.py:1461-1483, 869-929
+ *   lp_run_days           SEIR_ABM.run day loop       model.py:246-282 (all of the above, fused)
+ * The Python host package (laser-polio_amd) keeps the reference's component
+ * classes, their names, run order, step()/log() methods and results arrays, and
+ * calls these entry points through ctypes.
+ *
+ * Conventions: every entry point returns 0 (LP_OK) or a negative LP_ERR_* code;
+ * lp_last_error() gives the message.  Calls on one context are not re-entrant
+ * and come from one host thread.  Work is queued on the context's stream;
+ * lp_sync() waits for it and reports device-side errors (capacity exhausted,
+ * scratch overflow).
+ */
+#ifndef LP_STEP_H
+#define LP_STEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LP_ABI_VERSION 1
+#define LP_TILE 1024      /* agents per tile; node segments are multiples of this */
+#define LP_CHUNK 256      /* agents per sampling chunk (one wavefront x 4 agents) */
+#define LP_MAX_STRAINS 4
+#define LP_KM_LEN 101     /* cumulative-deaths table length (ages 0..100 y) */
+
+enum {
+  LP_OK = 0,
+  LP_ERR_ARG = -1,       /* bad argument / unknown binding name / wrong dtype or size */
+  LP_ERR_HIP = -2,       /* HIP runtime error */
+  LP_ERR_CAPACITY = -3,  /* a node ran out of agent slots for births */
+  LP_ERR_STATE = -4,     /* call order violated (e.g. run before lp_prepare) */
+  LP_ERR_UNBOUND = -5,   /* a required buffer was not bound */
+  LP_ERR_OVERFLOW = -6   /* a scratch list overflowed (too many exposures in one day) */
+};
+
+enum lp_dtype { LP_I8 = 1, LP_U8 = 2, LP_I16 = 3, LP_U16 = 4, LP_I32 = 5, LP_U32 = 6,
+                LP_I64 = 7, LP_U64 = 8, LP_F32 = 9 };
+
+/* Stages (bit mask) -- one per reference component, plus the log tally. */
+enum {
+  LP_STAGE_VITAL = 1 << 0,
+  LP_STAGE_DISEASE = 1 << 1,
+  LP_STAGE_RI = 1 << 2,
+  LP_STAGE_SIA = 1 << 3,
+  LP_STAGE_TRANSMISSION = 1 << 4,
+  LP_STAGE_LOG = 1 << 5,
+  LP_STAGE_ALL = 0x3f
+};
+
+/* Per-day schedule flags (int32 array "day_flags", one entry per tick). */
+enum {
+  LP_DAY_VD_STEP = 1 << 0,   /* VitalDynamics step day: deaths + births + pop update */
+  LP_DAY_VD_COPY = 1 << 1,   /* VitalDynamics present, non-step day: pop[t] = pop[t-1] */
+  LP_DAY_DS = 1 << 2,        /* DiseaseState present */
+  LP_DAY_RI = 1 << 3,        /* RI step day (t % step == 0 and vx_prob_ri set) */
+  LP_DAY_SIA = 1 << 4,       /* one or more SIA events today */
+  LP_DAY_TX = 1 << 5,        /* Transmission present (step + log) */
+  LP_DAY_STOPCHK = 1 << 6    /* DiseaseState early-stop rule enabled */
+};
+
+typedef struct {
+  int64_t capacity;        /* agent slots over all sims; multiple of LP_TILE */
+  int32_t n_nodes;         /* nodes over all sims (node ids are global) */
+  int32_t n_sims;          /* independent simulations in the batch */
+  int32_t n_strains;       /* 1..LP_MAX_STRAINS */
+  int32_t nt;              /* ticks = dur + 1 */
+  int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
+  int32_t tiles_per_block; /* tiles per workgroup in the agent pass (0 = default) */
+  int32_t mix_mode;        /* 0 = per-node tree reduction, 1 = batched MFMA contraction */
+  int32_t reserved[7];
+} lp_config;
+
+/* Per-simulation parameters; array bound as "sim_params" (LP_U8, n_sims * sizeof). */
+typedef struct {
+  uint64_t seed;           /* Philox key */
+  int64_t slot_base;       /* first agent slot of the sim (Philox agent index origin) */
+  int64_t net_off;         /* offset of the sim's n x n transposed network in "network_t" */
+  int32_t node_begin;      /* first global node id of the sim */
+  int32_t n_nodes;
+  float strain_r0[LP_MAX_STRAINS];
+  uint64_t thr_paralysis;  /* ceil(p_paralysis * 2^32) */
+  uint64_t thr_zi;         /* ceil(node_seeding_zero_inflation * 2^32) */
+  float zi;                /* node_seeding_zero_inflation */
+  int32_t disp_r;          /* max(1, round(node_seeding_dispersion)) */
+  int32_t ri_strain;       /* strain given by RI OPV (1 Sabin, 2 nOPV2) */
+  int32_t ri_step;         /* step_size_RI_ABM */
+  int32_t newborn_ri_timer;/* ri_timer given to newborns; -1 when RI_ABM is absent */
+  int32_t last_seed_day;   /* last tick with a seed_schedule entry, -1 if none */
+} lp_sim_params;
+
+typedef struct lp_ctx lp_ctx;
+
+/* Library / struct layout checks for bindings. */
+int32_t lp_abi_version(void);
+int64_t lp_sizeof_sim_params(void);
+
+/* Create a context for the given sizes.  Allocates nothing on the device yet. */
+int lp_create(const lp_config* cfg, lp_ctx** out);
+
+/* Bind a caller-owned device buffer by name (agent columns, node tables, sim
+ * parameters, schedule, results).  The name table is listed in lp_bind_names(). */
+int lp_bind(lp_ctx* ctx, const char* name, void* device_ptr, int32_t dtype, int64_t n_elems);
+const char* lp_bind_names(void);
+
+/* Validate bindings, build the tile/span tables from the node segments and
+ * allocate scratch.  Must follow the last lp_bind and precede any step. */
+int lp_prepare(lp_ctx* ctx);
+
+/* Use this HIP stream (hipStream_t) instead of the context's own. */
+int lp_set_stream(lp_ctx* ctx, void* hip_stream);
+
+/* Fused days: runs ticks [t_begin, t_end) with every stage the schedule enables,
+ * in reference order (VitalDynamics, DiseaseState, RI, SIA, Transmission, log).
+ * Tick 0 only logs the initial state.  Days after a sim's early stop are no-ops.
+ * use_graph != 0 replays one captured hipGraph per day. */
+int lp_run_days(lp_ctx* ctx, int32_t t_begin, int32_t t_end, int32_t use_graph);
+
+/* Single stages on tick t (any OR of LP_STAGE_*), for callers that step
+ * components one at a time.  Same kernels as the fused path, masked. */
+int lp_step_stage(lp_ctx* ctx, int32_t t, int32_t stage_mask);
+int lp_step_vital(lp_ctx* ctx, int32_t t);
+int lp_step_disease(lp_ctx* ctx, int32_t t);
+int lp_step_ri(lp_ctx* ctx, int32_t t);
+int lp_step_sia(lp_ctx* ctx, int32_t t);
+int lp_step_transmission(lp_ctx* ctx, int32_t t);
+int lp_log(lp_ctx* ctx, int32_t t);
+
+/* Wait for queued work; returns LP_ERR_CAPACITY / LP_ERR_OVERFLOW if the device flagged one. */
+int lp_sync(lp_ctx* ctx);
+
+/* Per-sim status after lp_sync: out[s] = tick after which sim s stopped early, or -1. */
+int lp_stop_days(lp_ctx* ctx, int32_t* out, int32_t n_sims);
+
+/* Diagnostics of the last transmission step (device pointers of library scratch):
+ * "p" float[n_nodes*n_strains], "expected" float[n_nodes], "total" int64[n_nodes],
+ * "beta_mixed" float[n_nodes*n_strains].  Returns NULL for unknown names. */
+const void* lp_debug_buffer(lp_ctx* ctx, const char* name);
+
+/* Timing of the agent-pass kernel: with timing on, lp_run_days launches the day
+ * kernels directly (no graph) and brackets every agent-pass launch with HIP
+ * events on the context stream; lp_agent_pass_timing returns the number of
+ * bracketed launches and their summed milliseconds since the last call. */
+int lp_set_timing(lp_ctx* ctx, int32_t on);
+int lp_agent_pass_timing(lp_ctx* ctx, int64_t* launches, double* total_ms);
+
+const char* lp_last_error(lp_ctx* ctx);
+void lp_destroy(lp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LP_STEP_H */

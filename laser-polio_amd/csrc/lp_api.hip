@@ -1,0 +1,470 @@
+// Host side of the C ABI declared in include/lp_step.h: binding table,
+// validation, span tables, scratch, the per-day kernel sequence, hipGraph
+// capture/replay and error reporting.
+#include "lp_kernels.hip"
+#include <string>
+#include <vector>
+#include <cstring>
+#include <cstdio>
+#include <algorithm>
+
+using lp::Dev;
+
+namespace {
+
+enum Len { L_CAP, L_NODES, L_NODES1, L_SIMS, L_SIMPAR, L_KM, L_T, L_TS, L_E, L_EN2, L_TN, L_TNK, L_ANY };
+
+struct BindSpec {
+  const char* name;
+  int dtype;
+  Len len;
+  bool required;
+  size_t off;  // offset of the pointer field inside Dev
+};
+
+#define F(field) offsetof(Dev, field)
+const BindSpec kBinds[] = {
+  {"disease_state", LP_I8, L_CAP, true, F(state)},
+  {"strain", LP_I8, L_CAP, true, F(strain)},
+  {"node_id", LP_I16, L_CAP, true, F(node_id)},
+  {"exposure_timer", LP_U16, L_CAP, true, F(exp_timer)},
+  {"infection_timer", LP_U16, L_CAP, true, F(inf_timer)},
+  {"paralysis_timer", LP_U16, L_CAP, true, F(par_timer)},
+  {"potentially_paralyzed", LP_I8, L_CAP, true, F(pp)},
+  {"paralyzed", LP_I8, L_CAP, true, F(paralyzed)},
+  {"ipv_protected", LP_I8, L_CAP, true, F(ipv)},
+  {"chronically_missed", LP_U8, L_CAP, true, F(missed)},
+  {"acq_risk_multiplier", LP_F32, L_CAP, true, F(risk)},
+  {"daily_infectivity", LP_F32, L_CAP, true, F(infect)},
+  {"date_of_birth", LP_I32, L_CAP, true, F(dob)},
+  {"date_of_death", LP_I32, L_CAP, true, F(dod)},
+  {"ri_timer", LP_I16, L_CAP, true, F(ri_timer)},
+  {"seg_begin", LP_I64, L_NODES, true, F(seg_begin)},
+  {"live_end", LP_I64, L_NODES, true, F(live_end)},
+  {"seg_end", LP_I64, L_NODES, true, F(seg_end)},
+  {"node_sim", LP_I32, L_NODES, true, F(node_sim)},
+  {"r0_scalar", LP_F32, L_NODES, true, F(r0s)},
+  {"keep", LP_F32, L_NODES, true, F(keep)},
+  {"birth_rate_step", LP_F32, L_NODES, true, F(birth_rate_step)},
+  {"thr_ri", LP_U64, L_NODES, true, F(thr_ri)},
+  {"thr_ipv", LP_U64, L_NODES, true, F(thr_ipv)},
+  {"sim_params", LP_U8, L_SIMPAR, true, F(sims)},
+  {"network_t", LP_F32, L_ANY, true, F(network_t)},
+  {"km_cd", LP_I64, L_KM, true, F(km_cd)},
+  {"day_flags", LP_I32, L_T, true, F(day_flags)},
+  {"day_sia_begin", LP_I32, L_T, true, F(day_sia_begin)},
+  {"day_sia_count", LP_I32, L_T, true, F(day_sia_count)},
+  {"season", LP_F32, L_TS, true, F(season)},
+  {"sia_strain", LP_I32, L_E, false, F(sia_strain)},
+  {"sia_min_age", LP_I32, L_E, false, F(sia_min_age)},
+  {"sia_max_age", LP_I32, L_E, false, F(sia_max_age)},
+  {"sia_thr", LP_U64, L_EN2, false, F(sia_thr)},
+  {"S", LP_I32, L_TN, true, F(r_S)},
+  {"E", LP_I32, L_TN, true, F(r_E)},
+  {"I", LP_I32, L_TN, true, F(r_I)},
+  {"R", LP_I32, L_TN, true, F(r_R)},
+  {"E_by_strain", LP_I32, L_TNK, true, F(r_Ebs)},
+  {"I_by_strain", LP_I32, L_TNK, true, F(r_Ibs)},
+  {"potentially_paralyzed_count", LP_I32, L_TN, true, F(r_potp)},
+  {"paralyzed_count", LP_I32, L_TN, true, F(r_p)},
+  {"new_potentially_paralyzed", LP_I32, L_TN, true, F(r_newpp)},
+  {"new_paralyzed", LP_I32, L_TN, true, F(r_newp)},
+  {"pop", LP_I32, L_TN, true, F(r_pop)},
+  {"births", LP_I32, L_TN, true, F(r_births)},
+  {"deaths", LP_I32, L_TN, true, F(r_deaths)},
+  {"new_exposed", LP_I32, L_TN, true, F(r_newexp)},
+  {"new_exposed_by_strain", LP_I32, L_TNK, true, F(r_newexp_bs)},
+  {"ri_vaccinated", LP_I32, L_TN, true, F(r_riv)},
+  {"ri_protected", LP_I32, L_TN, true, F(r_rip)},
+  {"ri_new_exposed_by_strain", LP_I32, L_TNK, true, F(r_ri_newexp_bs)},
+  {"ipv_vaccinated", LP_I32, L_TN, true, F(r_ipv)},
+  {"sia_vaccinated", LP_I32, L_TN, true, F(r_siav)},
+  {"sia_protected", LP_I32, L_TN, true, F(r_siap)},
+  {"sia_new_exposed_by_strain", LP_I32, L_TNK, true, F(r_sia_newexp_bs)},
+};
+#undef F
+constexpr int kNumBinds = sizeof(kBinds) / sizeof(kBinds[0]);
+
+std::string g_names;
+
+}  // namespace
+
+struct lp_ctx {
+  lp_config cfg{};
+  Dev d{};
+  bool bound[kNumBinds] = {};
+  int64_t bound_len[kNumBinds] = {};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool prepared = false;
+  int32_t n_spans = 0;
+  std::vector<void*> allocs;
+  hipGraphExec_t graph = nullptr;
+  bool graph_ready = false;
+  std::string err;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  int64_t time_launches = 0;
+  double time_ms = 0.0;
+};
+
+namespace {
+
+int fail(lp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail(c, LP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int64_t expected_len(const lp_ctx* c, Len l) {
+  const lp_config& g = c->cfg;
+  switch (l) {
+    case L_CAP: return g.capacity;
+    case L_NODES: return g.n_nodes;
+    case L_NODES1: return g.n_nodes + 1;
+    case L_SIMS: return g.n_sims;
+    case L_SIMPAR: return (int64_t)g.n_sims * (int64_t)sizeof(lp_sim_params);
+    case L_KM: return (int64_t)g.n_sims * LP_KM_LEN;
+    case L_T: return g.nt;
+    case L_TS: return (int64_t)g.nt * g.n_sims;
+    case L_E: return g.n_sia_events;
+    case L_EN2: return (int64_t)g.n_sia_events * g.n_nodes * 2;
+    case L_TN: return (int64_t)g.nt * g.n_nodes;
+    case L_TNK: return (int64_t)g.nt * g.n_nodes * g.n_strains;
+    case L_ANY: return -1;
+  }
+  return -1;
+}
+
+template <typename T>
+int alloc(lp_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(c, hipMalloc(&q, n * sizeof(T) + 16));
+  HIPCHK(c, hipMemsetAsync(q, 0, n * sizeof(T) + 16, c->stream));
+  c->allocs.push_back(q);
+  *p = reinterpret_cast<T*>(q);
+  return LP_OK;
+}
+
+int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+// The kernel sequence of one tick.  mask: stages; full_count: recount paralysis
+// flags from scratch; tx_adjust: apply today's transmission exposures to the log.
+int enqueue_day(lp_ctx* c, int mask, int full_count, int tx_adjust, bool advance) {
+  const Dev& d = c->d;
+  const int N = c->cfg.n_nodes;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
+  const bool ev = c->timing && !c->graph_ready;
+  if (ev) {
+    if (c->ev_used + 2 > c->ev.size()) {
+      hipEvent_t a, b;
+      HIPCHK(c, hipEventCreate(&a));
+      HIPCHK(c, hipEventCreate(&b));
+      c->ev.push_back(a); c->ev.push_back(b);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[c->ev_used], s));
+  }
+  hipLaunchKernelGGL(lp::k_agents, dim3(c->n_spans), dim3(lp::WG), 0, s, d, mask, full_count);
+  if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
+  hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, full_count);
+  if (mask & LP_STAGE_TRANSMISSION) {
+    hipLaunchKernelGGL(lp::k_reset_sel, dim3(1), dim3(64), 0, s, d);
+    hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
+    for (int r = 1; r <= lp::R_GRID; ++r) {
+      if (r > 1) hipLaunchKernelGGL(lp::k_prefix, dim3(N), dim3(lp::WG), 0, s, d);
+      hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(1024), 0, s, d, r);
+      hipLaunchKernelGGL(lp::k_resolve, dim3(2048), dim3(lp::WG), 0, s, d, r);
+      hipLaunchKernelGGL(lp::k_mark, dim3(512), dim3(lp::WG), 0, s, d, r);
+    }
+    hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);
+    hipLaunchKernelGGL(lp::k_expose, dim3(512), dim3(lp::WG), 0, s, d);
+  }
+  hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust);
+  if (advance) hipLaunchKernelGGL(lp::k_advance, dim3(1), dim3(64), 0, s, d);
+  HIPCHK(c, hipGetLastError());
+  return LP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t lp_abi_version(void) { return LP_ABI_VERSION; }
+int64_t lp_sizeof_sim_params(void) { return (int64_t)sizeof(lp_sim_params); }
+
+const char* lp_bind_names(void) {
+  if (g_names.empty()) {
+    for (int i = 0; i < kNumBinds; ++i) { if (i) g_names += ","; g_names += kBinds[i].name; }
+  }
+  return g_names.c_str();
+}
+
+int lp_create(const lp_config* cfg, lp_ctx** out) {
+  if (!cfg || !out) return LP_ERR_ARG;
+  *out = nullptr;
+  if (cfg->capacity <= 0 || cfg->capacity % LP_TILE != 0 || cfg->n_nodes <= 0 || cfg->n_sims <= 0 ||
+      cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->n_sia_events < 0 ||
+      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 40)
+    return LP_ERR_ARG;
+  lp_ctx* c = new lp_ctx();
+  c->cfg = *cfg;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return LP_ERR_HIP; }
+  c->own_stream = true;
+  *out = c;
+  return LP_OK;
+}
+
+int lp_bind(lp_ctx* c, const char* name, void* ptr, int32_t dtype, int64_t n) {
+  if (!c || !name) return LP_ERR_ARG;
+  if (c->prepared) return fail(c, LP_ERR_STATE, "lp_bind after lp_prepare");
+  for (int i = 0; i < kNumBinds; ++i) {
+    if (std::strcmp(kBinds[i].name, name) != 0) continue;
+    if (dtype != kBinds[i].dtype) return fail(c, LP_ERR_ARG, std::string("dtype mismatch for ") + name);
+    const int64_t want = expected_len(c, kBinds[i].len);
+    if (want >= 0 && n != want)
+      return fail(c, LP_ERR_ARG, std::string("length mismatch for ") + name + ": got " + std::to_string(n) +
+                                     ", want " + std::to_string(want));
+    if (!ptr && n > 0) return fail(c, LP_ERR_ARG, std::string("null pointer for ") + name);
+    *reinterpret_cast<void**>(reinterpret_cast<char*>(&c->d) + kBinds[i].off) = ptr;
+    c->bound[i] = true;
+    c->bound_len[i] = n;
+    return LP_OK;
+  }
+  return fail(c, LP_ERR_ARG, std::string("unknown binding ") + name);
+}
+
+int lp_set_stream(lp_ctx* c, void* s) {
+  if (!c) return LP_ERR_ARG;
+  if (c->graph_ready) return fail(c, LP_ERR_STATE, "stream change after graph capture");
+  c->stream = reinterpret_cast<hipStream_t>(s);
+  return LP_OK;
+}
+
+int lp_prepare(lp_ctx* c) {
+  if (!c) return LP_ERR_ARG;
+  if (c->prepared) return fail(c, LP_ERR_STATE, "lp_prepare called twice");
+  const lp_config& g = c->cfg;
+  for (int i = 0; i < kNumBinds; ++i) {
+    const bool need = kBinds[i].required || (g.n_sia_events > 0 && kBinds[i].len != L_ANY);
+    if (need && !c->bound[i]) return fail(c, LP_ERR_UNBOUND, std::string("unbound buffer ") + kBinds[i].name);
+  }
+  const int N = g.n_nodes, S = g.n_sims;
+  std::vector<int64_t> sb(N), se(N), le(N);
+  std::vector<int32_t> nsim(N);
+  std::vector<lp_sim_params> sims(S);
+  HIPCHK(c, hipMemcpy(sb.data(), c->d.seg_begin, N * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(se.data(), c->d.seg_end, N * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(le.data(), c->d.live_end, N * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nsim.data(), c->d.node_sim, N * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(sims.data(), c->d.sims, S * sizeof(lp_sim_params), hipMemcpyDeviceToHost));
+  int64_t net_len = 0;
+  for (int i = 0; i < kNumBinds; ++i) if (std::strcmp(kBinds[i].name, "network_t") == 0) net_len = c->bound_len[i];
+  for (int n = 0; n < N; ++n) {
+    if (sb[n] % LP_TILE || se[n] % LP_TILE || sb[n] > se[n] || se[n] > g.capacity || le[n] < sb[n] || le[n] > se[n] ||
+        (n > 0 && sb[n] < se[n - 1]))
+      return fail(c, LP_ERR_ARG, "bad node segment " + std::to_string(n));
+    if (nsim[n] < 0 || nsim[n] >= S) return fail(c, LP_ERR_ARG, "bad node_sim " + std::to_string(n));
+  }
+  for (int s = 0; s < S; ++s) {
+    const lp_sim_params& p = sims[s];
+    if (p.n_nodes <= 0 || p.node_begin < 0 || p.node_begin + p.n_nodes > N)
+      return fail(c, LP_ERR_ARG, "bad sim node range " + std::to_string(s));
+    for (int n = p.node_begin; n < p.node_begin + p.n_nodes; ++n)
+      if (nsim[n] != s) return fail(c, LP_ERR_ARG, "node_sim disagrees with sim_params " + std::to_string(s));
+    if (p.net_off < 0 || p.net_off + (int64_t)p.n_nodes * p.n_nodes > net_len)
+      return fail(c, LP_ERR_ARG, "network_t too short for sim " + std::to_string(s));
+    if (p.slot_base < 0 || p.slot_base > sb[p.node_begin])
+      return fail(c, LP_ERR_ARG, "bad slot_base for sim " + std::to_string(s));
+    if (p.ri_strain < 0 || p.ri_strain >= g.n_strains || p.disp_r < 1 || p.ri_step < 1)
+      return fail(c, LP_ERR_ARG, "bad scalar parameters for sim " + std::to_string(s));
+  }
+  if (g.n_sia_events > 0) {
+    std::vector<int32_t> ss(g.n_sia_events);
+    HIPCHK(c, hipMemcpy(ss.data(), c->d.sia_strain, g.n_sia_events * 4, hipMemcpyDeviceToHost));
+    for (int e = 0; e < g.n_sia_events; ++e)
+      if (ss[e] < 0 || ss[e] >= g.n_strains) return fail(c, LP_ERR_ARG, "SIA vaccine strain out of range");
+  }
+  const int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 4;
+  std::vector<int4> spans;
+  std::vector<int32_t> nspan(N + 1, 0);
+  for (int n = 0; n < N; ++n) {
+    nspan[n] = (int32_t)spans.size();
+    const int64_t t0 = sb[n] / LP_TILE, t1 = se[n] / LP_TILE;
+    for (int64_t t = t0; t < t1; t += tpw) {
+      const int64_t cnt = std::min<int64_t>(tpw, t1 - t);
+      spans.push_back(make_int4(n, (int)t, (int)cnt, 0));
+    }
+  }
+  nspan[N] = (int32_t)spans.size();
+  c->n_spans = (int32_t)spans.size();
+  if (c->n_spans == 0) return fail(c, LP_ERR_ARG, "no agent slots");
+  Dev& d = c->d;
+  d.capacity = g.capacity; d.n_nodes = N; d.n_sims = S; d.K = g.n_strains; d.nt = g.nt;
+  int4* spans_d; int32_t* nspan_d;
+  int rc;
+  if ((rc = alloc(c, &spans_d, spans.size()))) return rc;
+  if ((rc = alloc(c, &nspan_d, N + 1))) return rc;
+  HIPCHK(c, hipMemcpyAsync(spans_d, spans.data(), spans.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nspan_d, nspan.data(), (N + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  d.spans = spans_d; d.node_span_begin = nspan_d;
+  const int64_t nch = g.capacity / LP_CHUNK + 1;
+  if ((rc = alloc(c, &d.day, 4))) return rc;
+  if ((rc = alloc(c, &d.stop_day, S))) return rc;
+  if ((rc = alloc(c, &d.err, 4))) return rc;
+  if ((rc = alloc(c, &d.partials, (size_t)c->n_spans * lp::NPART))) return rc;
+  if ((rc = alloc(c, &d.chunk_sum, nch))) return rc;
+  if ((rc = alloc(c, &d.chunk_prefix, nch))) return rc;
+  if ((rc = alloc(c, &d.fin_prefix, nch))) return rc;
+  if ((rc = alloc(c, &d.nb_begin, N))) return rc;
+  if ((rc = alloc(c, &d.births, N))) return rc;
+  if ((rc = alloc(c, &d.node_tally, (size_t)N * lp::NPART))) return rc;
+  if ((rc = alloc(c, &d.potp_run, N))) return rc;
+  if ((rc = alloc(c, &d.p_run, N))) return rc;
+  if ((rc = alloc(c, &d.p_dbg, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.expected_dbg, N))) return rc;
+  if ((rc = alloc(c, &d.total_dbg, N))) return rc;
+  if ((rc = alloc(c, &d.size, N))) return rc;
+  if ((rc = alloc(c, &d.take_all, N))) return rc;
+  if ((rc = alloc(c, &d.n_uniq, N))) return rc;
+  if ((rc = alloc(c, &d.m_round, N))) return rc;
+  if ((rc = alloc(c, &d.W_round, N))) return rc;
+  if ((rc = alloc(c, &d.thr_strain, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.newexp, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.probe_off, N + 1))) return rc;
+  if ((rc = alloc(c, &d.round_base, 16))) return rc;
+  if ((rc = alloc(c, &d.sel_count, 2))) return rc;
+  d.sel_cap = g.capacity / 4 + 65536;
+  if ((rc = alloc(c, &d.sel_list, d.sel_cap))) return rc;
+  std::vector<int32_t> stop(S, INT_MAX);
+  HIPCHK(c, hipMemcpyAsync(d.stop_day, stop.data(), S * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->prepared = true;
+  return LP_OK;
+}
+
+int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
+  if (!c) return LP_ERR_ARG;
+  if (!c->prepared) return fail(c, LP_ERR_STATE, "lp_run_days before lp_prepare");
+  if (t_begin < 0 || t_end > c->cfg.nt || t_begin > t_end) return fail(c, LP_ERR_ARG, "bad tick range");
+  if (t_begin == t_end) return LP_OK;
+  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t_begin);
+  int rc;
+  int32_t t = t_begin;
+  if (t == 0) {
+    if ((rc = enqueue_day(c, LP_STAGE_LOG, 1, 0, true))) return rc;
+    t = 1;
+  }
+  if (t >= t_end) return LP_OK;
+  if (use_graph && !c->timing) {
+    if (!c->graph_ready) {
+      hipGraph_t graph;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_day(c, LP_STAGE_ALL, 0, 1, true);
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      HIPCHK(c, hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0));
+      HIPCHK(c, hipGraphDestroy(graph));
+      c->graph_ready = true;
+    }
+    for (; t < t_end; ++t) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    return LP_OK;
+  }
+  for (; t < t_end; ++t)
+    if ((rc = enqueue_day(c, LP_STAGE_ALL, 0, 1, true))) return rc;
+  return LP_OK;
+}
+
+int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
+  if (!c) return LP_ERR_ARG;
+  if (!c->prepared) return fail(c, LP_ERR_STATE, "step before lp_prepare");
+  if (t < 0 || t >= c->cfg.nt) return fail(c, LP_ERR_ARG, "bad tick");
+  if (mask & ~LP_STAGE_ALL) return fail(c, LP_ERR_ARG, "bad stage mask");
+  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t);
+  const bool fused_tx_log = (mask & LP_STAGE_TRANSMISSION) && (mask & LP_STAGE_LOG);
+  const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
+  return enqueue_day(c, mask, log_only ? 1 : 0, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+}
+
+int lp_step_vital(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_VITAL); }
+int lp_step_disease(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_DISEASE); }
+int lp_step_ri(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_RI); }
+int lp_step_sia(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_SIA); }
+int lp_step_transmission(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_TRANSMISSION); }
+int lp_log(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_LOG); }
+
+int lp_sync(lp_ctx* c) {
+  if (!c) return LP_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->prepared) return LP_OK;
+  int32_t e = 0;
+  HIPCHK(c, hipMemcpy(&e, c->d.err, 4, hipMemcpyDeviceToHost));
+  if (e & lp::ERR_CAPACITY) return fail(c, LP_ERR_CAPACITY, "a node ran out of agent slots for births");
+  if (e & lp::ERR_OVERFLOW) return fail(c, LP_ERR_OVERFLOW, "selected-agent list overflow");
+  if (e & lp::ERR_SAMPLING) return fail(c, LP_ERR_STATE, "infection sampling found an inconsistent chunk prefix");
+  return LP_OK;
+}
+
+int lp_stop_days(lp_ctx* c, int32_t* out, int32_t n) {
+  if (!c || !out || n != c->cfg.n_sims) return LP_ERR_ARG;
+  if (!c->prepared) return fail(c, LP_ERR_STATE, "not prepared");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out, c->d.stop_day, n * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) if (out[i] == INT_MAX) out[i] = -1;
+  return LP_OK;
+}
+
+const void* lp_debug_buffer(lp_ctx* c, const char* name) {
+  if (!c || !name || !c->prepared) return nullptr;
+  if (!std::strcmp(name, "p")) return c->d.p_dbg;
+  if (!std::strcmp(name, "expected")) return c->d.expected_dbg;
+  if (!std::strcmp(name, "total")) return c->d.total_dbg;
+  if (!std::strcmp(name, "beta_mixed")) return c->d.beta_mixed;
+  if (!std::strcmp(name, "size")) return c->d.size;
+  return nullptr;
+}
+
+int lp_agent_pass_timing(lp_ctx* c, int64_t* launches, double* total_ms) {
+  if (!c || !launches || !total_ms) return LP_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double ms = 0.0;
+  for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+    float x = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&x, c->ev[i], c->ev[i + 1]));
+    ms += x;
+  }
+  *launches = (int64_t)(c->ev_used / 2);
+  *total_ms = ms;
+  c->ev_used = 0;
+  return LP_OK;
+}
+
+int lp_set_timing(lp_ctx* c, int32_t on) {
+  if (!c) return LP_ERR_ARG;
+  c->timing = on != 0;
+  c->ev_used = 0;
+  return LP_OK;
+}
+
+const char* lp_last_error(lp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void lp_destroy(lp_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,894 @@
+// Kernels of the laser-polio daily agent step on MI355X (gfx950, wave64).
+//
+// One simulated day is this kernel sequence (all stages fused, reference order
+// VitalDynamics -> DiseaseState -> RI -> SIA -> Transmission -> log):
+//   k_day_begin    per node: births (count + newborn slot range), per-day resets,
+//                  early-stop rule                     model.py:1711-1719, 788-795
+//   k_agents       ONE streaming pass over every live agent slot, node-segmented:
+//                  deaths, newborn init, E->I->R timers + paralysis, RI, SIA, and
+//                  the tally the transmission step and the log need; per-workgroup
+//                  partial sums (no global atomics) and per-chunk (256 agents)
+//                  susceptible weight sums      model.py:1767-1781, 393-454,
+//                                               1784-1855, 1994-2060, 980-1007, 902-929
+//   k_node_reduce  per node: reduce workgroup partials, exclusive prefix of the
+//                  chunk weights, results rows of VD/DS/RI/SIA
+//   k_mix_foi      per node: network mix (fixed-order tree), seasonality, r0 scalars,
+//                  rate -> probability, expected exposures, Poisson / zero-inflated NB
+//                  count draw (chunks in parallel across the wave)  model.py:1327-1407
+//   k_plan/k_resolve/k_mark (x rounds), k_prefix, k_finish
+//                  weighted sampling without replacement by rounds of with-
+//                  replacement probes + unique (model.py:1089-1122): every probe
+//                  searches the node's chunk prefix, then one chunk, wave-parallel
+//   k_expose       per selected agent: strain draw, S -> E       model.py:1125-1147
+//   k_day_end      per node: log rows + transmission incidence   model.py:1436-1441, 1461-1483
+//   k_advance      day counter += 1 (days are replayed as one hipGraph)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <limits.h>
+#include "lp_device.h"
+#include "../../include/lp_step.h"
+
+namespace lp {
+
+constexpr int WG = 256;
+constexpr int NPART = 32;           // fields of a tally partial
+constexpr int8_t SEL_MARK = 0x7F;   // state byte of an agent selected for exposure today
+constexpr uint64_t SEL_INVALID = ~0ull;
+constexpr int R_GRID = 2;           // sampling rounds run as grid-wide kernels; later rounds in k_finish
+constexpr int MAX_ROUNDS = 4096;    // bound on the finisher's rounds (each round claims >= 1 agent)
+
+// Tally fields.
+enum : int {
+  P_S = 0, P_SPOS, P_R, P_POTP, P_P, P_DEATHS, P_DPP, P_DP, P_NEWPP, P_NEWP,
+  P_RIV, P_RIP, P_IPV, P_SIAV, P_SIAP,
+  P_E0 = 15, P_I0 = 19, P_SIAPK0 = 23, P_SW = 27, P_IB0 = 28
+};
+
+enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4 };
+
+struct Dev {
+  // agent columns
+  int8_t* state; int8_t* strain; int16_t* node_id;
+  uint16_t* exp_timer; uint16_t* inf_timer; uint16_t* par_timer;
+  int8_t* pp; int8_t* paralyzed; int8_t* ipv; uint8_t* missed;
+  float* risk; float* infect; int32_t* dob; int32_t* dod; int16_t* ri_timer;
+  // node tables
+  const int64_t* seg_begin; int64_t* live_end; const int64_t* seg_end; const int32_t* node_sim;
+  const float* r0s; const float* keep; const float* birth_rate_step;
+  const uint64_t* thr_ri; const uint64_t* thr_ipv;
+  // sim tables
+  const lp_sim_params* sims; const float* network_t; const int64_t* km_cd;
+  // schedule
+  const int32_t* day_flags; const int32_t* day_sia_begin; const int32_t* day_sia_count;
+  const float* season;
+  const int32_t* sia_strain; const int32_t* sia_min_age; const int32_t* sia_max_age; const uint64_t* sia_thr;
+  // results
+  int32_t *r_S, *r_E, *r_I, *r_R, *r_Ebs, *r_Ibs, *r_potp, *r_p, *r_newpp, *r_newp, *r_pop,
+          *r_births, *r_deaths, *r_newexp, *r_newexp_bs, *r_riv, *r_rip, *r_ri_newexp_bs, *r_ipv,
+          *r_siav, *r_siap, *r_sia_newexp_bs;
+  // scratch
+  int32_t* day; int32_t* stop_day; int32_t* err;
+  const int4* spans; const int32_t* node_span_begin;
+  int64_t* partials; int64_t* chunk_sum; int64_t* chunk_prefix;
+  int64_t* nb_begin; int32_t* births;
+  int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
+  float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
+  int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
+  uint64_t* thr_strain; int32_t* newexp;
+  int64_t* probe_off; int64_t* round_base; uint64_t* sel_list; int64_t* sel_count; int64_t sel_cap;
+  int64_t* fin_prefix;
+  int64_t capacity; int32_t n_nodes, n_sims, K, nt;
+};
+
+__device__ __forceinline__ bool sim_stopped(const Dev& d, int sim, int t) { return d.stop_day[sim] < t; }
+
+__device__ __forceinline__ int32_t stage_day_flags(int32_t flags, int mask) {
+  int32_t f = 0;
+  if (mask & LP_STAGE_VITAL) f |= flags & (LP_DAY_VD_STEP | LP_DAY_VD_COPY);
+  if (mask & LP_STAGE_DISEASE) f |= flags & (LP_DAY_DS | LP_DAY_STOPCHK);
+  if (mask & LP_STAGE_RI) f |= flags & LP_DAY_RI;
+  if (mask & LP_STAGE_SIA) f |= flags & LP_DAY_SIA;
+  if (mask & (LP_STAGE_TRANSMISSION | LP_STAGE_LOG)) f |= flags & LP_DAY_TX;
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_set_day(Dev d, int32_t t) { if (threadIdx.x == 0 && blockIdx.x == 0) d.day[0] = t; }
+__global__ void k_advance(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0) d.day[0] += 1; }
+
+// Kaplan-Meier style age at death (days) for an agent of `age` days, from a
+// cumulative deaths table by year of age: year of death drawn in proportion to
+// deaths among years >= current year, day uniform within that year (> age).
+__device__ int32_t km_age_at_death(const int64_t* cd, uint64_t seed, uint64_t slot, int32_t t, int32_t age) {
+  const uint4 o = draw(seed, slot, t, ST_KM);
+  const int32_t max_year = LP_KM_LEN - 1;
+  int32_t ay = age / 365;
+  if (ay > max_year) ay = max_year;
+  const int64_t lo = ay == 0 ? 0 : cd[ay - 1];
+  const int64_t hi = cd[max_year];
+  int32_t yod = ay;
+  if (hi > lo) {
+    const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+    const int64_t v = lo + (int64_t)mulhi64(u, (uint64_t)(hi - lo));
+    int32_t a = 0, b = LP_KM_LEN;          // first k with cd[k] > v
+    while (a < b) { const int32_t m = (a + b) >> 1; if (cd[m] > v) b = m; else a = m + 1; }
+    yod = a;
+  }
+  int32_t lo_d = yod * 365;
+  if (lo_d < age + 1) lo_d = age + 1;
+  const int32_t hi_d = (yod + 1) * 365;
+  if (hi_d <= lo_d) return lo_d;
+  return lo_d + (int32_t)(((uint64_t)o.z * (uint64_t)(hi_d - lo_d)) >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Per node: births, per-day resets, early-stop rule.
+__global__ void k_day_begin(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n_nodes) return;
+  const int sim = d.node_sim[n];
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) d.newexp[n * LP_MAX_STRAINS + k] = 0;
+  d.n_uniq[n] = 0; d.size[n] = 0; d.take_all[n] = 0; d.m_round[n] = 0;
+  d.births[n] = 0;
+  d.nb_begin[n] = d.live_end[n];
+  if (sim_stopped(d, sim, t)) return;
+  if ((flags & LP_DAY_VD_STEP) && t >= 1) {
+    const int32_t pop_prev = d.r_pop[(size_t)(t - 1) * d.n_nodes + n];
+    const float e = d.birth_rate_step[n] * (float)pop_prev;
+    int32_t b = 0;
+    if (e > 0.0f) {
+      const int32_t bi = (int32_t)e;
+      const float frac = e - (float)bi;
+      const uint4 o = philox(0u, (uint32_t)(n - sp.node_begin), (uint32_t)t, ST_BIRTH, sp.seed);
+      b = bi + ((uint64_t)o.x < thr_from_prob(frac) ? 1 : 0);
+    }
+    const int64_t room = d.seg_end[n] - d.live_end[n];
+    if (b > room) { b = (int32_t)room; atomicOr(d.err, ERR_CAPACITY); }
+    d.births[n] = b;
+    d.live_end[n] += b;
+  }
+  if ((flags & LP_DAY_STOPCHK) && t >= 1 && n == sp.node_begin && d.stop_day[sim] == INT_MAX) {
+    int64_t any = 0;
+    const size_t row = (size_t)(t - 1) * d.n_nodes;
+    for (int j = sp.node_begin; j < sp.node_begin + sp.n_nodes; ++j) any += d.r_E[row + j] + d.r_I[row + j];
+    if (any == 0 && !(sp.last_seed_day > t)) d.stop_day[sim] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The agent pass.  One workgroup per span (<= tiles_per_block tiles of one node);
+// each thread owns 4 consecutive slots of a 1024-slot tile, each wave one
+// 256-slot sampling chunk.
+struct Tally {
+  int32_t c[27];     // counts: fields 0..26
+  int64_t sw, ib[LP_MAX_STRAINS];
+};
+
+__global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) {
+  const int32_t t = d.day[0];
+  const int4 span = d.spans[blockIdx.x];
+  const int node = span.x;
+  const int sim = d.node_sim[node];
+  if (sim_stopped(d, sim, t)) return;
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const bool do_deaths = flags & LP_DAY_VD_STEP;
+  const bool do_ds = flags & LP_DAY_DS;
+  const bool do_ri = flags & LP_DAY_RI;
+  const bool do_sia = flags & LP_DAY_SIA;
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const int64_t live_hi = d.live_end[node];
+  const int64_t nb_lo = d.nb_begin[node];
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int K = d.K;
+  const int32_t sia_b = do_sia ? d.day_sia_begin[t] : 0;
+  const int32_t sia_n = do_sia ? d.day_sia_count[t] : 0;
+  const uint64_t thr_ri = do_ri ? d.thr_ri[node] : 0, thr_ipv = do_ri ? d.thr_ipv[node] : 0;
+
+  Tally tl;
+  for (int q = 0; q < 27; ++q) tl.c[q] = 0;
+  tl.sw = 0;
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) tl.ib[k] = 0;
+
+  for (int ti = 0; ti < span.z; ++ti) {
+    const int64_t base = ((int64_t)span.y + ti) * LP_TILE;
+    if (base >= live_hi) break;
+    const int64_t i0 = base + (int64_t)tid * 4;
+    uint32_t st4 = (i0 < live_hi) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+    const uint32_t st4_in = st4;
+    int64_t chunk_w = 0;
+    float4 rk = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool risk_loaded = false;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = i0 + a;
+      int8_t s = (int8_t)((st4 >> (8 * a)) & 0xffu);
+      if (i >= live_hi) continue;
+      const uint64_t slot = (uint64_t)(i - sbase);
+      if (i >= nb_lo) {                                   // newborn (VD births, model.py:1719-1732)
+        d.dob[i] = t;
+        d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
+        d.node_id[i] = (int16_t)node;
+        if (sp.newborn_ri_timer >= 0) d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
+        s = 0;
+      } else if (do_deaths && s >= 0) {                   // get_deaths (model.py:1775-1777)
+        if (d.dod[i] <= t) {
+          tl.c[P_DEATHS] += 1;
+          if (d.pp[i] == 1) tl.c[P_DPP] += 1;
+          if (d.paralyzed[i] == 1) tl.c[P_DP] += 1;
+          s = -1;
+        }
+      }
+      if (do_ds && (s == 1 || s == 2)) {                  // disease_state_step_kernel (model.py:419-452)
+        if (s == 1) {
+          const uint16_t et = d.exp_timer[i];
+          if (et == 0) s = 2; else d.exp_timer[i] = (uint16_t)(et - 1);
+        }
+        if (s == 2) {
+          const uint16_t it = d.inf_timer[i];
+          if (it == 0) s = 3; else d.inf_timer[i] = (uint16_t)(it - 1);
+          if (d.strain[i] == 0) {
+            const uint16_t pt = d.par_timer[i];
+            if (pt == 0 && d.pp[i] == -1) {
+              if (d.ipv[i] == 0) {
+                d.pp[i] = 1;
+                tl.c[P_NEWPP] += 1;
+                const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
+                if ((uint64_t)o.x < sp.thr_paralysis) { d.paralyzed[i] = 1; tl.c[P_NEWP] += 1; }
+              } else {
+                d.pp[i] = 0;
+              }
+            }
+            if (pt > 0) d.par_timer[i] = (uint16_t)(pt - 1);
+          }
+        }
+      }
+      if ((do_ri || do_sia) && s >= 0 && d.missed[i] != 1) {
+        if (do_ri) {                                      // fast_ri (model.py:1826-1854)
+          const int32_t step = sp.ri_step;
+          const int16_t timer = (int16_t)(d.ri_timer[i] - step);
+          d.ri_timer[i] = timer;
+          bool eligible = false;
+          if (t == step) eligible = timer <= 0 && timer >= -step;
+          else if (t > step) eligible = timer <= 0 && timer > -step;
+          if (eligible) {
+            const uint4 o = draw(seed, slot, t, ST_RI);
+            if ((uint64_t)o.x < thr_ri) {
+              tl.c[P_RIV] += 1;
+              if (s == 0) { s = 1; d.strain[i] = (int8_t)sp.ri_strain; tl.c[P_RIP] += 1; }
+            }
+            if ((uint64_t)o.y < thr_ipv) { tl.c[P_IPV] += 1; d.ipv[i] = 1; }
+          }
+        }
+        if (do_sia) {                                     // fast_sia (model.py:2033-2059), per event
+          const int32_t age = t - d.dob[i];
+          for (int e = 0; e < sia_n; ++e) {
+            const int ev = sia_b + e;
+            if (!(d.sia_min_age[ev] <= age && age <= d.sia_max_age[ev])) continue;
+            const uint64_t t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
+            if (t1 == 0) continue;
+            const uint4 o = draw(seed, slot, t, ST_SIA | ((uint32_t)e << 8));
+            if ((uint64_t)o.x < t1) {
+              const bool last = (e == sia_n - 1);
+              if (last) tl.c[P_SIAV] += 1;
+              if (s == 0 && (uint64_t)o.x < d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1]) {
+                s = 1;
+                const int ks = d.sia_strain[ev];
+                d.strain[i] = (int8_t)ks;
+#pragma unroll
+                for (int q = 0; q < LP_MAX_STRAINS; ++q) tl.c[P_SIAPK0 + q] += (q == ks);
+                if (last) tl.c[P_SIAP] += 1;
+              }
+            }
+          }
+        }
+      }
+      if (do_tally && s >= 0) {                           // tx_step_prep / count_SEIRP tally
+        if (s == 0) {
+          if (!risk_loaded) { rk = *reinterpret_cast<const float4*>(d.risk + i0); risk_loaded = true; }
+          const float rv = a == 0 ? rk.x : a == 1 ? rk.y : a == 2 ? rk.z : rk.w;
+          const int64_t w = weight_q24(rv);
+          tl.c[P_S] += 1;
+          tl.c[P_SPOS] += (w > 0);
+          tl.sw += w;
+          chunk_w += w;
+        } else if (s == 1) {
+          const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
+#pragma unroll
+          for (int q = 0; q < LP_MAX_STRAINS; ++q) tl.c[P_E0 + q] += (q == k);
+        } else if (s == 2) {
+          const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
+          const int64_t wv = weight_q24(d.infect[i] * sp.strain_r0[k]);
+#pragma unroll
+          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+            tl.c[P_I0 + q] += (q == k);
+            tl.ib[q] += (q == k) ? wv : 0;
+          }
+        } else if (s == 3) {
+          tl.c[P_R] += 1;
+        }
+        if (full_count) {
+          if (d.pp[i] == 1) tl.c[P_POTP] += 1;
+          if (d.paralyzed[i] == 1) tl.c[P_P] += 1;
+        }
+      }
+      st4 = (st4 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+    }
+    if (st4 != st4_in && i0 < live_hi) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = st4;
+    if (do_tally) {
+      const int64_t cw = wave_sum_i64(chunk_w);
+      if (lane == 0) d.chunk_sum[(base >> 8) + wave] = cw;
+    }
+  }
+  (void)K;
+
+  // Workgroup reduction of the tally -> one partial per span.
+  __shared__ int64_t red[WG / 64][NPART];
+  for (int q = 0; q < 27; ++q) {
+    const int32_t v = wave_sum_i32(tl.c[q]);
+    if (lane == 0) red[wave][q] = v;
+  }
+  {
+    const int64_t v = wave_sum_i64(tl.sw);
+    if (lane == 0) red[wave][P_SW] = v;
+  }
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+    const int64_t v = wave_sum_i64(tl.ib[k]);
+    if (lane == 0) red[wave][P_IB0 + k] = v;
+  }
+  __syncthreads();
+  if (tid < NPART) {
+    int64_t acc = 0;
+    for (int w = 0; w < WG / 64; ++w) acc += red[w][tid];
+    d.partials[(size_t)blockIdx.x * NPART + tid] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup exclusive scan of a node's chunk weights; returns the total.
+// `atomic_io` reads chunk sums / writes the prefix with agent-scope atomics.
+template <bool ATOMIC_IO>
+__device__ int64_t node_chunk_scan(const int64_t* cs, int64_t* prefix, int64_t c0, int64_t c1) {
+  __shared__ int64_t sh[WG];
+  const int tid = threadIdx.x;
+  const int64_t nch = c1 - c0;
+  const int64_t per = (nch + WG - 1) / WG;
+  const int64_t a = c0 + (int64_t)tid * per;
+  const int64_t b = (a + per < c1) ? a + per : c1;
+  int64_t s = 0;
+  for (int64_t c = a; c < b; ++c) {
+    const int64_t v = ATOMIC_IO ? __hip_atomic_load(cs + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cs[c];
+    s += v;
+  }
+  sh[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < WG; o <<= 1) {
+    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = sh[tid] - s;  // exclusive
+  for (int64_t c = a; c < b; ++c) {
+    const int64_t v = ATOMIC_IO ? __hip_atomic_load(cs + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cs[c];
+    if (ATOMIC_IO) __hip_atomic_store(prefix + c, run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else prefix[c] = run;
+    run += v;
+  }
+  const int64_t total = sh[WG - 1];
+  __syncthreads();
+  return total;
+}
+
+// Per node: reduce partials, chunk prefix, results rows of VD / DS / RI / SIA.
+__global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_count) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x;
+  const int sim = d.node_sim[n];
+  if (sim_stopped(d, sim, t)) return;
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  __shared__ int64_t tally[NPART];
+  const int tid = threadIdx.x;
+  if (tid < NPART) {
+    int64_t acc = 0;
+    for (int s = d.node_span_begin[n]; s < d.node_span_begin[n + 1]; ++s) acc += d.partials[(size_t)s * NPART + tid];
+    tally[tid] = acc;
+    d.node_tally[(size_t)n * NPART + tid] = acc;
+  }
+  __syncthreads();
+  if (flags & LP_DAY_TX) {
+    const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
+    const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
+    const int64_t W = node_chunk_scan<false>(d.chunk_sum, d.chunk_prefix, c0, c1);
+    if (tid == 0) d.W_round[n] = W;
+  }
+  if (tid != 0) return;
+  const size_t N = d.n_nodes, row = (size_t)t * N, K4 = d.K;
+  if (flags & LP_DAY_VD_STEP) {
+    d.r_births[row + n] = d.births[n];
+    d.r_deaths[row + n] = (int32_t)tally[P_DEATHS];
+    d.r_pop[row + n] = d.r_pop[row - N + n] + d.births[n] - (int32_t)tally[P_DEATHS];
+  } else if (flags & LP_DAY_VD_COPY) {
+    d.r_pop[row + n] = d.r_pop[row - N + n];
+  }
+  if (flags & LP_DAY_DS) {
+    d.r_newpp[row + n] = (int32_t)tally[P_NEWPP];
+    d.r_newp[row + n] = (int32_t)tally[P_NEWP];
+  }
+  if (flags & LP_DAY_RI) {
+    const int32_t rip = (int32_t)tally[P_RIP];
+    d.r_riv[row + n] = (int32_t)tally[P_RIV];
+    d.r_rip[row + n] = rip;
+    d.r_ipv[row + n] = (int32_t)tally[P_IPV];
+    d.r_newexp[row + n] += rip;
+    d.r_newexp_bs[(row + n) * K4 + sp.ri_strain] += rip;
+    d.r_ri_newexp_bs[(row + n) * K4 + sp.ri_strain] = rip;
+  }
+  if (flags & LP_DAY_SIA) {
+    d.r_siav[row + n] = (int32_t)tally[P_SIAV];
+    d.r_siap[row + n] = (int32_t)tally[P_SIAP];
+    int32_t tot = 0;
+    for (int k = 0; k < d.K; ++k) {
+      const int32_t v = (int32_t)tally[P_SIAPK0 + k];
+      d.r_newexp_bs[(row + n) * K4 + k] += v;
+      d.r_sia_newexp_bs[(row + n) * K4 + k] += v;
+      tot += v;
+    }
+    d.r_newexp[row + n] += tot;
+  }
+  if (full_count) {
+    d.potp_run[n] = (int32_t)tally[P_POTP];
+    d.p_run[n] = (int32_t)tally[P_P];
+  } else {
+    d.potp_run[n] += (int32_t)(tally[P_NEWPP] - tally[P_DPP]);
+    d.p_run[n] += (int32_t)(tally[P_NEWP] - tally[P_DP]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per node: network mix, FOI, count draw.  Workgroup per node.
+__global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int i = blockIdx.x;
+  const int sim = d.node_sim[i];
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
+    if (tid == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
+    return;
+  }
+  const int K = d.K;
+  const int ns = sp.n_nodes, li = i - sp.node_begin;
+  const float* NT = d.network_t + sp.net_off + (size_t)li * ns;
+  __shared__ float part[LP_MAX_STRAINS][WG];
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+    if (k >= K) break;
+    float acc = 0.0f;
+    for (int j = tid; j < ns; j += WG) {
+      const float bj = (float)d.node_tally[(size_t)(sp.node_begin + j) * NPART + P_IB0 + k] * 5.9604644775390625e-08f;
+      acc = __builtin_fmaf(NT[j], bj, acc);
+    }
+    part[k][tid] = acc;
+  }
+  __syncthreads();
+  for (int w = WG / 2; w >= 1; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int k = 0; k < LP_MAX_STRAINS; ++k) if (k < K) part[k][tid] = part[k][tid] + part[k][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid >= 64) return;
+  // Wave 0: node-level map, then the count draw with chunks spread over lanes.
+  const int64_t* tl = d.node_tally + (size_t)i * NPART;
+  const int32_t pop_t = d.r_pop[(size_t)t * d.n_nodes + i];
+  const float popc = (float)(pop_t > 1 ? pop_t : 1);
+  const float season = d.season[(size_t)t * d.n_sims + sim];
+  float pk[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
+  float ptot = 0.0f;
+  bool local_zero = true;
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+    if (k >= K) break;
+    const float bi = (float)tl[P_IB0 + k] * 5.9604644775390625e-08f;
+    const float bm = __builtin_fmaf(bi, d.keep[i], part[k][0]);
+    const float b2 = (bm * season) * d.r0s[i];
+    const float rate = b2 / popc;
+    float p = -expm1f_lp(-rate);
+    if (!(p > 0.0f)) p = 0.0f;
+    pk[k] = p;
+    ptot = ptot + p;
+    if (tl[P_IB0 + k] != 0) local_zero = false;
+    if (lane == 0) { d.p_dbg[(size_t)i * K + k] = p; d.beta_mixed[(size_t)i * K + k] = bm; }
+  }
+  const float expo = (float)tl[P_SW] * 5.9604644775390625e-08f;
+  const float ex = expo * ptot;
+  // mode 0: none, 1: Poisson(ex), 2: NB
+  int mode = 0;
+  if (ex > 0.0f) {
+    if (!local_zero) mode = 1;
+    else if (sp.zi < 1.0f) {
+      const uint4 o = philox(0xFFFFFFFFu, (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
+      if ((uint64_t)o.x >= sp.thr_zi) mode = 2;
+    }
+  }
+  int64_t mine = 0;
+  if (mode == 1) {
+    const float nf = floorf(ex / 16.0f);
+    const int64_t n_full = (int64_t)nf;
+    const float last = ex - 16.0f * nf;
+    const int64_t nch = n_full + (last > 0.0f ? 1 : 0);
+    for (int64_t j = lane; j < nch; j += 64) {
+      const float lc = j < n_full ? 16.0f : last;
+      const uint4 o = philox((uint32_t)j, (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
+      const float u = u24(o.x);
+      float p = expf_lp(-lc), F = p;
+      int k = 0;
+      while (u >= F) {
+        k += 1;
+        p = (p * lc) / (float)k;
+        const float Fn = F + p;
+        if (Fn == F) break;
+        F = Fn;
+      }
+      mine += k;
+    }
+  } else if (mode == 2) {
+    const float dm = ex / (1.0f - sp.zi);
+    const float q = dm / (float)sp.disp_r;
+    const float lq = log1pf_lp(q);
+    if (lq > 0.0f) {
+      const float omp = q / (1.0f + q);
+      int32_t rc_max = (int32_t)(16.0f / lq);
+      if (rc_max < 1) rc_max = 1;
+      const int32_t n_full = sp.disp_r / rc_max, rest = sp.disp_r % rc_max;
+      const int32_t nch = n_full + (rest > 0 ? 1 : 0);
+      for (int32_t j = lane; j < nch; j += 64) {
+        const int32_t rc = j < n_full ? rc_max : rest;
+        const uint4 o = philox((uint32_t)(j + 1), (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
+        const float u = u24(o.x);
+        float P = expf_lp(-((float)rc * lq)), F = P;
+        int64_t k = 0;
+        while (u >= F) {
+          P = (P * ((float)(k + rc) * omp)) / (float)(k + 1);
+          k += 1;
+          const float Fn = F + P;
+          if (Fn == F) break;
+          F = Fn;
+        }
+        mine += k;
+      }
+    }
+  }
+  const int64_t total = wave_sum_i64(mine);
+  if (lane != 0) return;
+  d.expected_dbg[i] = ex;
+  d.total_dbg[i] = total;
+  const int64_t scnt = tl[P_S];
+  const int32_t sz = (int32_t)(total < scnt ? total : scnt);
+  const int32_t ta = (sz > 0 && (int64_t)sz >= tl[P_SPOS]) ? 1 : 0;
+  d.size[i] = sz;
+  d.take_all[i] = ta;
+  d.m_round[i] = ta ? 0 : sz;
+  float cum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+    if (k >= K) break;
+    cum = cum + (ptot > 0.0f ? pk[k] / ptot : 0.0f);
+    d.thr_strain[(size_t)i * LP_MAX_STRAINS + k] = thr_from_prob(cum);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sampling round r >= 2: refresh the node's chunk prefix from the decremented
+// chunk sums, and the number of probes still needed.
+__global__ void __launch_bounds__(WG) k_prefix(Dev d) {
+  const int n = blockIdx.x;
+  const int32_t need = d.size[n] - d.n_uniq[n];
+  if (d.take_all[n] || need <= 0 || d.size[n] == 0) { if (threadIdx.x == 0) d.m_round[n] = 0; return; }
+  const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
+  const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
+  const int64_t W = node_chunk_scan<false>(d.chunk_sum, d.chunk_prefix, c0, c1);
+  if (threadIdx.x == 0) { d.W_round[n] = W; d.m_round[n] = W > 0 ? need : 0; }
+}
+
+// Single workgroup: exclusive scan of probes over nodes, reserve list space.
+__global__ void __launch_bounds__(1024) k_plan(Dev d, int round) {
+  __shared__ int64_t sh[1024];
+  const int tid = threadIdx.x;
+  const int N = d.n_nodes;
+  const int per = (N + 1023) / 1024;
+  const int a = tid * per, b = (a + per < N) ? a + per : N;
+  int64_t s = 0;
+  for (int n = a; n < b; ++n) s += d.m_round[n];
+  sh[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = sh[tid] - s;
+  for (int n = a; n < b; ++n) { d.probe_off[n] = run; run += d.m_round[n]; }
+  if (tid == 1023) {
+    int64_t M = sh[1023];
+    const int64_t base = d.sel_count[0];
+    if (base + M > d.sel_cap) { atomicOr(d.err, ERR_OVERFLOW); M = 0; }
+    d.probe_off[N] = M;
+    d.round_base[round] = base;
+    d.sel_count[0] = base + M;
+  }
+}
+
+// One wavefront per probe: target T in [0, W), find the chunk with prefix <= T by a
+// 64-ary search, then the agent inside the chunk by a wave prefix scan.  Writes the
+// hit (slot | node << 40) to the selected list; nothing is mutated here.
+__device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t p) {
+  int a = 0, b = N;   // largest n in [0, N) with off[n] <= p < off[n+1]
+  while (b - a > 1) { const int m = (a + b) >> 1; if (off[m] <= p) a = m; else b = m; }
+  return a;
+}
+
+template <bool ATOMIC_IO>
+__device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
+  int64_t lo = d.seg_begin[node] / LP_CHUNK;
+  int64_t hi = (d.live_end[node] + LP_CHUNK - 1) / LP_CHUNK;   // search [lo, hi)
+  while (hi - lo > 1) {
+    const int64_t span = hi - lo;
+    const int64_t step = (span + 63) / 64;
+    const int64_t c = lo + (int64_t)lane * step;
+    int64_t v = INT64_MAX;
+    if (c < hi) v = ATOMIC_IO ? __hip_atomic_load(prefix + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : prefix[c];
+    const uint64_t ok = __ballot(v <= T);        // lanes are monotone: lane 0 always true
+    const int L = 63 - __clzll(ok);
+    lo = lo + (int64_t)L * step;
+    const int64_t nhi = lo + step;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const int64_t c = lo;
+  const int64_t cp = ATOMIC_IO ? __hip_atomic_load(prefix + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : prefix[c];
+  const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
+  uint32_t st4;
+  if (ATOMIC_IO) st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else st4 = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
+  const float4 rk = *reinterpret_cast<const float4*>(d.risk + i0);
+  int64_t w[4];
+  w[0] = ((st4 & 0xffu) == 0) ? weight_q24(rk.x) : 0;
+  w[1] = (((st4 >> 8) & 0xffu) == 0) ? weight_q24(rk.y) : 0;
+  w[2] = (((st4 >> 16) & 0xffu) == 0) ? weight_q24(rk.z) : 0;
+  w[3] = (((st4 >> 24) & 0xffu) == 0) ? weight_q24(rk.w) : 0;
+  const int64_t lsum = w[0] + w[1] + w[2] + w[3];
+  const int64_t incl = wave_incl_scan_i64(lsum, lane);
+  int64_t cum = cp + (incl - lsum);
+  int hit = -1;
+  for (int a = 0; a < 4; ++a) { cum += w[a]; if (hit < 0 && cum > T) hit = a; }
+  const uint64_t hb = __ballot(hit >= 0);
+  if (hb == 0) return -1;
+  const int L = __ffsll((unsigned long long)hb) - 1;
+  const int ha = __shfl(hit, L, 64);
+  return c * LP_CHUNK + (int64_t)L * 4 + ha;
+}
+
+__global__ void __launch_bounds__(WG) k_resolve(Dev d, int round) {
+  const int32_t t = d.day[0];
+  const int N = d.n_nodes;
+  const int64_t M = d.probe_off[N];
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (WG / 64);
+  const int64_t base = d.round_base[round];
+  for (int64_t p = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); p < M; p += nw) {
+    const int node = (int)find_node(d.probe_off, N, p);
+    const int64_t j = p - d.probe_off[node];
+    const lp_sim_params& sp = d.sims[d.node_sim[node]];
+    const uint4 o = philox((uint32_t)j, (uint32_t)(node - sp.node_begin), (uint32_t)t,
+                           ST_PROBE | ((uint32_t)round << 8), sp.seed);
+    const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+    const int64_t W = d.W_round[node];
+    const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
+    const int64_t slot = W > 0 ? resolve_probe<false>(d, node, T, d.chunk_prefix, lane) : -1;
+    if (lane == 0) {
+      if (slot < 0) { d.sel_list[base + p] = SEL_INVALID; atomicOr(d.err, ERR_SAMPLING); }
+      else d.sel_list[base + p] = (uint64_t)slot | ((uint64_t)node << 40);
+    }
+  }
+}
+
+// Claim a hit: state byte 0 -> SEL_MARK with a word CAS; duplicates become invalid.
+__device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(state) + (slot >> 2);
+  const int sh = 8 * (int)(slot & 3);
+  uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    if (((old >> sh) & 0xffu) != 0) return false;
+    const uint32_t nw = (old & ~(0xffu << sh)) | ((uint32_t)(uint8_t)SEL_MARK << sh);
+    const uint32_t prev = atomicCAS(w, old, nw);
+    if (prev == old) return true;
+    old = prev;
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_mark(Dev d, int round) {
+  const int64_t M = d.probe_off[d.n_nodes];
+  const int64_t base = d.round_base[round];
+  for (int64_t p = (int64_t)blockIdx.x * WG + threadIdx.x; p < M; p += (int64_t)gridDim.x * WG) {
+    const uint64_t e = d.sel_list[base + p];
+    if (e == SEL_INVALID) continue;
+    const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+    const int node = (int)(e >> 40);
+    if (slot >= d.capacity || node >= d.n_nodes) { d.sel_list[base + p] = SEL_INVALID; continue; }
+    if (claim(d.state, slot)) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)),
+                (unsigned long long)(-weight_q24(d.risk[slot])));
+      atomicAdd(d.n_uniq + node, 1);
+    } else {
+      d.sel_list[base + p] = SEL_INVALID;
+    }
+  }
+}
+
+// Per node: take-all nodes (size >= #positive-weight susceptibles) select every
+// such agent; nodes still short after the grid rounds finish their rounds here.
+__global__ void __launch_bounds__(WG) k_finish(Dev d) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int64_t sh_base;
+  __shared__ int32_t sh_cnt;
+  __shared__ int64_t sh_W;
+  if (d.size[n] == 0) return;
+  if (d.take_all[n]) {
+    const int64_t lo = d.seg_begin[n], hi = d.live_end[n];
+    for (int64_t b0 = lo; b0 < hi; b0 += WG) {
+      const int64_t i = b0 + tid;
+      bool sel = false;
+      if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) sel = true;
+      const uint64_t bal = __ballot(sel);
+      if (tid == 0) sh_cnt = 0;
+      __syncthreads();
+      int woff = 0;
+      if (lane == 0) woff = atomicAdd(&sh_cnt, __popcll(bal));
+      woff = __shfl(woff, 0, 64);
+      __syncthreads();
+      if (tid == 0) {
+        const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)sh_cnt);
+        if (b + sh_cnt > d.sel_cap) {
+          atomicOr(d.err, ERR_OVERFLOW);
+          for (int64_t q = b; q < d.sel_cap; ++q) d.sel_list[q] = SEL_INVALID;
+          sh_base = -1;
+        } else sh_base = b;
+      }
+      __syncthreads();
+      if (sel && sh_base >= 0) {
+        const int pos = woff + __popcll(bal & ((1ull << lane) - 1));
+        d.state[i] = SEL_MARK;
+        d.sel_list[sh_base + pos] = (uint64_t)i | ((uint64_t)n << 40);
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  const lp_sim_params& sp = d.sims[d.node_sim[n]];
+  const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
+  const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
+  for (int round = R_GRID + 1;; ++round) {
+    if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); return; }
+    const int32_t need = d.size[n] - __hip_atomic_load(d.n_uniq + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (need <= 0) return;
+    const int64_t W = node_chunk_scan<true>(d.chunk_sum, d.fin_prefix, c0, c1);
+    if (W <= 0) return;
+    if (tid == 0) {
+      const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
+      if (b + need > d.sel_cap) {
+        atomicOr(d.err, ERR_OVERFLOW);
+        for (int64_t q = b; q < d.sel_cap; ++q) d.sel_list[q] = SEL_INVALID;
+        sh_base = -1;
+      } else sh_base = b;
+      sh_W = W;
+    }
+    __syncthreads();
+    const int64_t base = sh_base;
+    if (base < 0) return;
+    for (int j = wave; j < need; j += WG / 64) {
+      const uint4 o = philox((uint32_t)j, (uint32_t)(n - sp.node_begin), (uint32_t)t,
+                             ST_PROBE | ((uint32_t)round << 8), sp.seed);
+      const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+      const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
+      const int64_t slot = resolve_probe<true>(d, n, T, d.fin_prefix, lane);
+      if (lane == 0) {
+        const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
+        if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
+        __hip_atomic_store(d.sel_list + base + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    for (int j = tid; j < need; j += WG) {
+      const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (e == SEL_INVALID) continue;
+      const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+      if (claim(d.state, slot)) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)),
+                  (unsigned long long)(-weight_q24(d.risk[slot])));
+        atomicAdd(d.n_uniq + n, 1);
+      } else {
+        __hip_atomic_store(d.sel_list + base + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// Every valid selected slot: strain by cumulative strain probability, S -> E.
+__global__ void __launch_bounds__(WG) k_expose(Dev d) {
+  const int32_t t = d.day[0];
+  const int64_t M = d.sel_count[0] < d.sel_cap ? d.sel_count[0] : d.sel_cap;
+  for (int64_t p = (int64_t)blockIdx.x * WG + threadIdx.x; p < M; p += (int64_t)gridDim.x * WG) {
+    const uint64_t e = d.sel_list[p];
+    if (e == SEL_INVALID) continue;
+    const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+    const int node = (int)(e >> 40);
+    if (slot >= d.capacity || node >= d.n_nodes || d.state[slot] != SEL_MARK) continue;
+    const lp_sim_params& sp = d.sims[d.node_sim[node]];
+    const uint4 o = draw(sp.seed, (uint64_t)(slot - sp.slot_base), t, ST_STRAIN);
+    int k = 0;
+    for (int s = 0; s < d.K; ++s) {
+      if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
+    }
+    d.state[slot] = 1;
+    d.strain[slot] = (int8_t)k;
+    atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+  }
+}
+
+// Per node: transmission incidence and the log rows.
+__global__ void k_day_end(Dev d, int mask, int tx_adjust) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n_nodes) return;
+  const int sim = d.node_sim[n];
+  if (sim_stopped(d, sim, t)) return;
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if (!(flags & LP_DAY_TX)) return;
+  const size_t N = d.n_nodes, row = (size_t)t * N + n;
+  const int K = d.K;
+  const int64_t* tl = d.node_tally + (size_t)n * NPART;
+  int32_t ne[LP_MAX_STRAINS] = {0, 0, 0, 0};
+  int32_t ne_tot = 0;
+  if (tx_adjust) {
+    for (int k = 0; k < K; ++k) { ne[k] = d.newexp[n * LP_MAX_STRAINS + k]; ne_tot += ne[k]; }
+  }
+  if ((mask & LP_STAGE_TRANSMISSION) && tx_adjust) {
+    d.r_newexp[row] += ne_tot;
+    for (int k = 0; k < K; ++k) d.r_newexp_bs[row * K + k] += ne[k];
+  }
+  if (mask & LP_STAGE_LOG) {
+    int32_t e = 0, in = 0;
+    for (int k = 0; k < K; ++k) {
+      const int32_t ek = (int32_t)tl[P_E0 + k] + ne[k];
+      const int32_t ik = (int32_t)tl[P_I0 + k];
+      d.r_Ebs[row * K + k] = ek;
+      d.r_Ibs[row * K + k] = ik;
+      e += ek; in += ik;
+    }
+    d.r_S[row] = (int32_t)tl[P_S] - ne_tot;
+    d.r_E[row] = e;
+    d.r_I[row] = in;
+    d.r_R[row] += (int32_t)tl[P_R];
+    d.r_potp[row] = d.potp_run[n];
+    d.r_p[row] = d.p_run[n];
+  }
+}
+
+__global__ void k_reset_sel(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0) d.sel_count[0] = 0; }
+
+}  // namespace lp

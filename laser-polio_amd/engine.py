@@ -1,0 +1,263 @@
+"""ctypes binding of liblpstep.so (include/lp_step.h) and its device buffers.
+
+The engine owns one torch tensor per bound buffer (torch is used only as the
+device allocator and for host<->device copies) and one library context.  It
+raises loudly when the HIP library or a GPU is missing: there is no CPU path.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblpstep.so")
+
+LP_I8, LP_U8, LP_I16, LP_U16, LP_I32, LP_U32, LP_I64, LP_U64, LP_F32 = range(1, 10)
+_DTYPE_CODE = {
+    np.dtype(np.int8): LP_I8,
+    np.dtype(np.uint8): LP_U8,
+    np.dtype(np.int16): LP_I16,
+    np.dtype(np.uint16): LP_U16,
+    np.dtype(np.int32): LP_I32,
+    np.dtype(np.uint32): LP_U32,
+    np.dtype(np.int64): LP_I64,
+    np.dtype(np.uint64): LP_U64,
+    np.dtype(np.float32): LP_F32,
+}
+# torch has no general uint16/uint32/uint64 kernels: keep those buffers as signed views.
+_TORCH_VIEW = {np.dtype(np.uint16): np.int16, np.dtype(np.uint32): np.int32, np.dtype(np.uint64): np.int64}
+
+STAGE_VITAL, STAGE_DISEASE, STAGE_RI, STAGE_SIA, STAGE_TRANSMISSION, STAGE_LOG = (1, 2, 4, 8, 16, 32)
+STAGE_ALL = 0x3F
+DAY_VD_STEP, DAY_VD_COPY, DAY_DS, DAY_RI, DAY_SIA, DAY_TX, DAY_STOPCHK = (1, 2, 4, 8, 16, 32, 64)
+TILE = 1024
+CHUNK = 256
+MAX_STRAINS = 4
+KM_LEN = 101
+
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "node capacity exhausted", -4: "call order", -5: "unbound buffer", -6: "overflow"}
+
+
+class EngineUnavailable(RuntimeError):
+    pass
+
+
+class LPError(RuntimeError):
+    pass
+
+
+class LPConfig(ctypes.Structure):
+    _fields_ = [
+        ("capacity", ctypes.c_int64),
+        ("n_nodes", ctypes.c_int32),
+        ("n_sims", ctypes.c_int32),
+        ("n_strains", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("n_sia_events", ctypes.c_int32),
+        ("tiles_per_block", ctypes.c_int32),
+        ("mix_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+SIM_PARAMS_DTYPE = np.dtype(
+    [
+        ("seed", np.uint64),
+        ("slot_base", np.int64),
+        ("net_off", np.int64),
+        ("node_begin", np.int32),
+        ("n_nodes", np.int32),
+        ("strain_r0", np.float32, (MAX_STRAINS,)),
+        ("thr_paralysis", np.uint64),
+        ("thr_zi", np.uint64),
+        ("zi", np.float32),
+        ("disp_r", np.int32),
+        ("ri_strain", np.int32),
+        ("ri_step", np.int32),
+        ("newborn_ri_timer", np.int32),
+        ("last_seed_day", np.int32),
+    ],
+    align=True,
+)
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liblpstep.so once and declare its C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineUnavailable(f"{path} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    lib.lp_abi_version.restype = i32
+    lib.lp_sizeof_sim_params.restype = i64
+    lib.lp_create.argtypes = [ctypes.POINTER(LPConfig), ctypes.POINTER(vp)]
+    lib.lp_bind.argtypes = [vp, ctypes.c_char_p, vp, i32, i64]
+    lib.lp_bind_names.restype = ctypes.c_char_p
+    lib.lp_prepare.argtypes = [vp]
+    lib.lp_set_stream.argtypes = [vp, vp]
+    lib.lp_run_days.argtypes = [vp, i32, i32, i32]
+    lib.lp_step_stage.argtypes = [vp, i32, i32]
+    for f in ("lp_step_vital", "lp_step_disease", "lp_step_ri", "lp_step_sia", "lp_step_transmission", "lp_log"):
+        getattr(lib, f).argtypes = [vp, i32]
+    lib.lp_sync.argtypes = [vp]
+    lib.lp_stop_days.argtypes = [vp, ctypes.POINTER(i32), i32]
+    lib.lp_debug_buffer.argtypes = [vp, ctypes.c_char_p]
+    lib.lp_debug_buffer.restype = vp
+    lib.lp_set_timing.argtypes = [vp, i32]
+    lib.lp_agent_pass_timing.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
+    lib.lp_last_error.argtypes = [vp]
+    lib.lp_last_error.restype = ctypes.c_char_p
+    lib.lp_destroy.argtypes = [vp]
+    lib.lp_destroy.restype = None
+    if lib.lp_abi_version() != 1:
+        raise EngineUnavailable("liblpstep.so ABI version mismatch")
+    if lib.lp_sizeof_sim_params() != SIM_PARAMS_DTYPE.itemsize:
+        raise EngineUnavailable("lp_sim_params layout mismatch between include/lp_step.h and engine.py")
+    _lib = lib
+    return lib
+
+
+def _require_gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise EngineUnavailable("no GPU visible: the laser-polio step runs only on the HIP path (MI355X, gfx950)")
+    return torch
+
+
+def thr_from_prob64(p):
+    """ceil(p * 2^32) clamped to [0, 2^32], from float64 probabilities (host-side thresholds)."""
+    p = np.asarray(p, dtype=np.float64)
+    out = np.ceil(np.clip(p, 0.0, 1.0) * 4294967296.0)
+    out = np.where(p > 0, out, 0.0)
+    return out.astype(np.uint64)
+
+
+class Engine:
+    """One library context plus the device copies of every bound buffer."""
+
+    def __init__(self, config, arrays, timing=False):
+        self.torch = _require_gpu()
+        self.lib = load_library()
+        self.device = self.torch.device("cuda", self.torch.cuda.current_device())
+        self.arrays = arrays
+        self.tensors = {}
+        self.cfg = config
+        ctx = ctypes.c_void_p()
+        rc = self.lib.lp_create(ctypes.byref(config), ctypes.byref(ctx))
+        if rc != 0:
+            raise LPError(f"lp_create failed ({ERRORS.get(rc, rc)}): check sizes {config.capacity=} {config.n_nodes=}")
+        self.ctx = ctx
+        for name, arr in arrays.items():
+            self._upload_new(name, arr)
+        self.torch.cuda.synchronize()
+        for name, arr in arrays.items():
+            t = self.tensors[name]
+            n = arr.size if name != "sim_params" else arr.nbytes
+            code = LP_U8 if name == "sim_params" else _DTYPE_CODE[arr.dtype]
+            self._check(self.lib.lp_bind(self.ctx, name.encode(), ctypes.c_void_p(t.data_ptr()), code, int(n)))
+        self._check(self.lib.lp_prepare(self.ctx))
+        if timing:
+            self._check(self.lib.lp_set_timing(self.ctx, 1))
+
+    # -- buffers -----------------------------------------------------------------
+    def _upload_new(self, name, arr):
+        torch = self.torch
+        a = np.ascontiguousarray(arr)
+        if name == "sim_params":
+            a = a.view(np.uint8)
+        elif a.dtype in _TORCH_VIEW:
+            a = a.view(_TORCH_VIEW[a.dtype])
+        self.tensors[name] = torch.from_numpy(a.reshape(-1)).to(self.device)
+
+    def push(self, name, arr):
+        """Host -> device copy of one bound buffer (same size as at bind time)."""
+        a = np.ascontiguousarray(arr)
+        if name == "sim_params":
+            a = a.view(np.uint8)
+        elif a.dtype in _TORCH_VIEW:
+            a = a.view(_TORCH_VIEW[a.dtype])
+        self.sync()
+        self.tensors[name].copy_(self.torch.from_numpy(a.reshape(-1)))
+        self.torch.cuda.synchronize()
+
+    def pull(self, name, out):
+        """Device -> host copy into `out` (numpy, same size and dtype as bound)."""
+        self.sync()
+        src = self.tensors[name].cpu().numpy()
+        out.reshape(-1).view(src.dtype)[...] = src
+
+    # -- stepping ------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.lp_last_error(self.ctx).decode() if self.ctx else ""
+            raise LPError(f"liblpstep error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+    def run_days(self, t_begin, t_end, use_graph=True):
+        self._check(self.lib.lp_run_days(self.ctx, int(t_begin), int(t_end), 1 if use_graph else 0))
+
+    def step(self, t, mask):
+        self._check(self.lib.lp_step_stage(self.ctx, int(t), int(mask)))
+
+    def sync(self):
+        self._check(self.lib.lp_sync(self.ctx))
+
+    def stop_days(self):
+        out = (ctypes.c_int32 * self.cfg.n_sims)()
+        self._check(self.lib.lp_stop_days(self.ctx, out, self.cfg.n_sims))
+        return np.array(list(out), dtype=np.int32)
+
+    def agent_pass_timing(self):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        self._check(self.lib.lp_agent_pass_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def debug(self, name, count, dtype):
+        """Copy a library diagnostics buffer (see lp_debug_buffer) to the host."""
+        self.sync()
+        ptr = self.lib.lp_debug_buffer(self.ctx, name.encode())
+        if not ptr:
+            raise LPError(f"unknown debug buffer {name}")
+        out = np.empty(count, dtype=dtype)
+        self._memcpy_d2h(out, ptr)
+        return out
+
+    def _memcpy_d2h(self, out, dev_ptr):
+        hiprt = _hip_runtime()
+        rc = hiprt.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(dev_ptr), ctypes.c_size_t(out.nbytes), 2)
+        if rc != 0:
+            raise LPError(f"hipMemcpy failed ({rc})")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.lp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
+
+
+_hiprt = None
+
+
+def _hip_runtime():
+    global _hiprt
+    if _hiprt is None:
+        for name in ("libamdhip64.so", "libamdhip64.so.7", "libamdhip64.so.6"):
+            try:
+                _hiprt = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hiprt is None:
+            raise EngineUnavailable("libamdhip64 not found")
+        _hiprt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _hiprt
