@@ -1,0 +1,210 @@
+"""Benchmark: agent-days/s of the laser-polio daily step on one MI355X.
+
+Workload (the national-scale configuration): 774 nodes, populations
+LogNormal(ln 2.5e5, 0.6) -> about 2.3e8 agents, gravity network, RI + SIA
+campaigns, 2555 simulated days.  A "step" is one simulated day of the whole
+population.  The host initialises the population (not timed), uploads it once,
+runs ``--warmup`` days untimed, then times exactly ``--steps`` days bracketed by
+torch.cuda.synchronize().  value = sum over timed days of live agents / seconds.
+
+Also reported on the same JSON line:
+  roofline      the agent-pass kernel (k_agents): algorithmic bytes per day
+                (formula in DESIGN.md) over its HIP-event-timed duration, against
+                MI355X's 8 TB/s HBM peak
+  cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
+                threads) on the same initial population for the first --cpu-days days
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "agent-days/s, 200M-agent Nigeria run; replicate-sims/hr batched vs serial"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def agent_pass_bytes(res, flags, t0, t1):
+    """Algorithmic bytes of the agent pass over ticks [t0, t1).
+
+    per day: N_live*(1 state + 4 risk) + N_EI*17 (E/I timers r/w, strain, infectivity,
+    paralysis fields) + VD step day: N_live*4 (date_of_death) + RI day: N_live*5
+    (ri_timer r/w + chronically_missed) + SIA day: N_live*5 (date_of_birth + missed).
+    N_live = live agents that day (results.pop), N_EI = E+I at the start of the day.
+    """
+    from laser_polio_amd import engine as em
+
+    pop = res["pop"].astype(np.int64).sum(axis=1)
+    ei = (res["E"].astype(np.int64) + res["I"].astype(np.int64)).sum(axis=1)
+    total = 0
+    for t in range(t0, t1):
+        n_live, n_ei, f = int(pop[t]), int(ei[t - 1]), int(flags[t])
+        b = 5 * n_live + 17 * n_ei
+        if f & em.DAY_VD_STEP:
+            b += 4 * n_live
+        if f & em.DAY_RI:
+            b += 5 * n_live
+        if f & em.DAY_SIA:
+            b += 5 * n_live
+        total += b
+    return total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2550)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=float, default=1.0, help="population scale (1.0 = ~2.3e8 agents)")
+    ap.add_argument("--cpu-days", type=int, default=8, help="days of the CPU oracle baseline sample (0 = skip)")
+    ap.add_argument("--graph", type=int, default=0, help="1: replay one hipGraph per day (no per-kernel events)")
+    ap.add_argument("--tiles-per-block", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="instead of the benchmark: copy this many GiB device-to-device 5 times (a plain copy kernel "
+                         "of known size, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+
+        n = int(args.copy_calibration * (1 << 30))
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU node sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    lp = __graft_entry__.load_package()
+    from laser_polio_amd import configs
+
+    nt_needed = 1 + args.warmup + args.steps
+    t_init = time.perf_counter()
+    pars = configs.nigeria_national_pars(scale=args.scale, dur=max(2555, nt_needed - 1))
+    if args.tiles_per_block:
+        pars.tiles_per_block = args.tiles_per_block
+    sim = configs.build(pars)
+    n0 = int(pars.init_pop.sum())
+    log(f"[bench] init: {len(pars.init_pop)} nodes, {n0} agents, capacity {sim.people.capacity}, "
+        f"{time.perf_counter() - t_init:.1f}s")
+
+    oracle_sim = None
+    if args.cpu_days > 0:
+        from oracle.oracle import OracleSim
+
+        oracle_sim = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop)
+
+    sim._timing = not args.graph
+    t_up = time.perf_counter()
+    eng = sim._ensure_engine()
+    torch.cuda.synchronize()
+    log(f"[bench] upload {time.perf_counter() - t_up:.1f}s")
+    day_flags = eng.arrays["day_flags"]
+
+    eng.run_days(0, 1 + args.warmup, use_graph=bool(args.graph))
+    eng.sync()
+    eng.agent_pass_timing()  # discard warmup events
+    torch.cuda.synchronize()
+    t_begin, t_end = 1 + args.warmup, 1 + args.warmup + args.steps
+    t0 = time.perf_counter()
+    eng.run_days(t_begin, t_end, use_graph=bool(args.graph))
+    eng.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    launches, kernel_ms = (0, 0.0) if args.graph else eng.agent_pass_timing()
+    sim._mark_device_newer()
+    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "new_exposed")}
+    agent_days = int(res["pop"][t_begin:t_end].astype(np.int64).sum())
+    value = agent_days / elapsed
+    log(f"[bench] {args.steps} days in {elapsed:.3f}s -> {value:.4e} agent-days/s; "
+        f"exposures {int(res['new_exposed'][t_begin:t_end].sum())}")
+
+    roofline = None
+    if launches:
+        bytes_alg = agent_pass_bytes(res, day_flags, t_begin, t_end)
+        achieved = bytes_alg / (kernel_ms / 1e3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "kernel": "k_agents",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": launches,
+            "avg_launch_us": round(kernel_ms * 1e3 / launches, 2),
+            "bytes_per_launch": int(bytes_alg // launches),
+        }
+        log(f"[bench] k_agents: {launches} launches, avg {kernel_ms * 1e3 / launches:.1f} us, {achieved:.0f} GB/s")
+
+    cpu = None
+    if oracle_sim is not None:
+        from oracle import oracle as orc
+
+        days = min(args.cpu_days, sim.nt - 1)
+        oracle_sim.day(0)
+        tc = time.perf_counter()
+        for t in range(1, 1 + days):
+            oracle_sim.day(t)
+        cpu_s = time.perf_counter() - tc
+        cpu_ad = int(oracle_sim.res["pop"][1 : 1 + days].astype(np.int64).sum())
+        threads = int(orc.lib().orc_max_threads())
+        cpu = {
+            "value": round(cpu_ad / cpu_s, 1),
+            "unit": "agent-days/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"days 1-{days} of the same {n0}-agent population ({cpu_s:.1f}s, OpenMP {threads} threads, "
+                      f"host nproc {os.cpu_count()})",
+        }
+        log(f"[bench] cpu oracle: {cpu['value']:.4e} agent-days/s over {days} days ({cpu_s:.1f}s)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "agent-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"national: 774 nodes, {n0} agents (population scale {args.scale}), RI + SIA every 180 d, "
+                        f"days {t_begin}-{t_end - 1} of {sim.nt - 1} timed",
+            "nodes": int(len(pars.init_pop)),
+            "agents": n0,
+            "capacity": int(sim.people.capacity),
+            "graph": bool(args.graph),
+            "arithmetic": "int8 state, uint16 timers, f32 weights -> int64 fixed-point sums, f32 node map",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

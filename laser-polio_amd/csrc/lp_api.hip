@@ -98,9 +98,12 @@ struct lp_ctx {
   bool own_stream = false;
   bool prepared = false;
   int32_t n_spans = 0;
+  int32_t tpw = 4;
+  int32_t agent_grid = 1;
   std::vector<void*> allocs;
-  hipGraphExec_t graph = nullptr;
-  bool graph_ready = false;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};   // plain day, full day
+  bool graph_ready[2] = {false, false};
+  std::vector<int32_t> day_flags_h;
   std::string err;
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -155,14 +158,40 @@ int alloc(lp_ctx* c, T** p, size_t n) {
 
 int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
-// The kernel sequence of one tick.  mask: stages; full_count: recount paralysis
-// flags from scratch; tx_adjust: apply today's transmission exposures to the log.
-int enqueue_day(lp_ctx* c, int mask, int full_count, int tx_adjust, bool advance) {
+
+// Host-side mirror of the device day classification (k_agents::is_plain_day).
+int32_t host_stage_flags(int32_t flags, int mask) {
+  int32_t f = 0;
+  if (mask & LP_STAGE_VITAL) f |= flags & (LP_DAY_VD_STEP | LP_DAY_VD_COPY);
+  if (mask & LP_STAGE_DISEASE) f |= flags & (LP_DAY_DS | LP_DAY_STOPCHK);
+  if (mask & LP_STAGE_RI) f |= flags & LP_DAY_RI;
+  if (mask & LP_STAGE_SIA) f |= flags & LP_DAY_SIA;
+  if (mask & (LP_STAGE_TRANSMISSION | LP_STAGE_LOG)) f |= flags & LP_DAY_TX;
+  return f;
+}
+
+int day_kind(const lp_ctx* c, int32_t t, int mask, int full_count) {
+  const int32_t f = host_stage_flags(c->day_flags_h[t], mask);
+  return (!(f & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) && !full_count) ? lp::KIND_PLAIN : lp::KIND_FULL;
+}
+
+template <int U>
+void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask, int full_count) {
+  if (kind == lp::KIND_PLAIN)
+    hipLaunchKernelGGL((lp::k_agents<U, lp::KIND_PLAIN>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask, full_count);
+  else
+    hipLaunchKernelGGL((lp::k_agents<U, lp::KIND_FULL>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask, full_count);
+}
+
+// The kernel sequence of one tick.  kind: agent-pass variant; mask: stages;
+// full_count: recount paralysis flags from scratch; tx_adjust: apply today's
+// transmission exposures to the log; advance: move the day counter at the end.
+int enqueue_day(lp_ctx* c, int kind, int mask, int full_count, int tx_adjust, bool advance) {
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
-  const bool ev = c->timing && !c->graph_ready;
+  const bool ev = c->timing;
   if (ev) {
     if (c->ev_used + 2 > c->ev.size()) {
       hipEvent_t a, b;
@@ -172,23 +201,26 @@ int enqueue_day(lp_ctx* c, int mask, int full_count, int tx_adjust, bool advance
     }
     HIPCHK(c, hipEventRecord(c->ev[c->ev_used], s));
   }
-  hipLaunchKernelGGL(lp::k_agents, dim3(c->n_spans), dim3(lp::WG), 0, s, d, mask, full_count);
+  switch (c->tpw) {
+    case 1: launch_agents_u<1>(c, s, kind, mask, full_count); break;
+    case 2: launch_agents_u<2>(c, s, kind, mask, full_count); break;
+    case 8: launch_agents_u<8>(c, s, kind, mask, full_count); break;
+    default: launch_agents_u<4>(c, s, kind, mask, full_count); break;
+  }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, full_count);
   if (mask & LP_STAGE_TRANSMISSION) {
-    hipLaunchKernelGGL(lp::k_reset_sel, dim3(1), dim3(64), 0, s, d);
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
     for (int r = 1; r <= lp::R_GRID; ++r) {
       if (r > 1) hipLaunchKernelGGL(lp::k_prefix, dim3(N), dim3(lp::WG), 0, s, d);
-      hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(1024), 0, s, d, r);
+      hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, r);
       hipLaunchKernelGGL(lp::k_resolve, dim3(2048), dim3(lp::WG), 0, s, d, r);
       hipLaunchKernelGGL(lp::k_mark, dim3(512), dim3(lp::WG), 0, s, d, r);
     }
     hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);
     hipLaunchKernelGGL(lp::k_expose, dim3(512), dim3(lp::WG), 0, s, d);
   }
-  hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust);
-  if (advance) hipLaunchKernelGGL(lp::k_advance, dim3(1), dim3(64), 0, s, d);
+  hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust, advance ? 1 : 0);
   HIPCHK(c, hipGetLastError());
   return LP_OK;
 }
@@ -243,7 +275,7 @@ int lp_bind(lp_ctx* c, const char* name, void* ptr, int32_t dtype, int64_t n) {
 
 int lp_set_stream(lp_ctx* c, void* s) {
   if (!c) return LP_ERR_ARG;
-  if (c->graph_ready) return fail(c, LP_ERR_STATE, "stream change after graph capture");
+  if (c->graph_ready[0] || c->graph_ready[1]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
   c->stream = reinterpret_cast<hipStream_t>(s);
   return LP_OK;
 }
@@ -292,7 +324,9 @@ int lp_prepare(lp_ctx* c) {
     for (int e = 0; e < g.n_sia_events; ++e)
       if (ss[e] < 0 || ss[e] >= g.n_strains) return fail(c, LP_ERR_ARG, "SIA vaccine strain out of range");
   }
-  const int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 4;
+  int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 2;
+  if (tpw != 1 && tpw != 2 && tpw != 4 && tpw != 8) return fail(c, LP_ERR_ARG, "tiles_per_block must be 1, 2, 4 or 8");
+  c->tpw = tpw;
   std::vector<int4> spans;
   std::vector<int32_t> nspan(N + 1, 0);
   for (int n = 0; n < N; ++n) {
@@ -300,7 +334,7 @@ int lp_prepare(lp_ctx* c) {
     const int64_t t0 = sb[n] / LP_TILE, t1 = se[n] / LP_TILE;
     for (int64_t t = t0; t < t1; t += tpw) {
       const int64_t cnt = std::min<int64_t>(tpw, t1 - t);
-      spans.push_back(make_int4(n, (int)t, (int)cnt, 0));
+      spans.push_back(make_int4(n, (int)t, (int)cnt, nsim[n]));
     }
   }
   nspan[N] = (int32_t)spans.size();
@@ -308,6 +342,13 @@ int lp_prepare(lp_ctx* c) {
   if (c->n_spans == 0) return fail(c, LP_ERR_ARG, "no agent slots");
   Dev& d = c->d;
   d.capacity = g.capacity; d.n_nodes = N; d.n_sims = S; d.K = g.n_strains; d.nt = g.nt;
+  d.n_spans = c->n_spans; d.tpw = tpw;
+  {
+    int dev = 0, cus = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    c->agent_grid = std::max(1, std::min(c->n_spans, cus * 8));
+  }
   int4* spans_d; int32_t* nspan_d;
   int rc;
   if ((rc = alloc(c, &spans_d, spans.size()))) return rc;
@@ -321,8 +362,8 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.err, 4))) return rc;
   if ((rc = alloc(c, &d.partials, (size_t)c->n_spans * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.chunk_sum, nch))) return rc;
-  if ((rc = alloc(c, &d.chunk_prefix, nch))) return rc;
-  if ((rc = alloc(c, &d.fin_prefix, nch))) return rc;
+  if ((rc = alloc(c, &d.fin_prefix, c->n_spans))) return rc;
+  if ((rc = alloc(c, &d.span_prefix, c->n_spans))) return rc;
   if ((rc = alloc(c, &d.nb_begin, N))) return rc;
   if ((rc = alloc(c, &d.births, N))) return rc;
   if ((rc = alloc(c, &d.node_tally, (size_t)N * lp::NPART))) return rc;
@@ -342,8 +383,11 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.probe_off, N + 1))) return rc;
   if ((rc = alloc(c, &d.round_base, 16))) return rc;
   if ((rc = alloc(c, &d.sel_count, 2))) return rc;
+  if ((rc = alloc(c, &d.ticket, 4))) return rc;
   d.sel_cap = g.capacity / 4 + 65536;
   if ((rc = alloc(c, &d.sel_list, d.sel_cap))) return rc;
+  c->day_flags_h.resize(g.nt);
+  HIPCHK(c, hipMemcpy(c->day_flags_h.data(), c->d.day_flags, g.nt * 4, hipMemcpyDeviceToHost));
   std::vector<int32_t> stop(S, INT_MAX);
   HIPCHK(c, hipMemcpyAsync(d.stop_day, stop.data(), S * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -360,27 +404,28 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   int rc;
   int32_t t = t_begin;
   if (t == 0) {
-    if ((rc = enqueue_day(c, LP_STAGE_LOG, 1, 0, true))) return rc;
+    if ((rc = enqueue_day(c, lp::KIND_FULL, LP_STAGE_LOG, 1, 0, true))) return rc;
     t = 1;
   }
-  if (t >= t_end) return LP_OK;
-  if (use_graph && !c->timing) {
-    if (!c->graph_ready) {
-      hipGraph_t graph;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_day(c, LP_STAGE_ALL, 0, 1, true);
-      hipError_t e = hipStreamEndCapture(c->stream, &graph);
-      if (rc) return rc;
-      if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      HIPCHK(c, hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0));
-      HIPCHK(c, hipGraphDestroy(graph));
-      c->graph_ready = true;
+  for (; t < t_end; ++t) {
+    const int kind = day_kind(c, t, LP_STAGE_ALL, 0);
+    if (use_graph && !c->timing) {
+      if (!c->graph_ready[kind]) {
+        hipGraph_t graph;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_day(c, kind, LP_STAGE_ALL, 0, 1, true);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        HIPCHK(c, hipGraphInstantiate(&c->graph[kind], graph, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphDestroy(graph));
+        c->graph_ready[kind] = true;
+      }
+      HIPCHK(c, hipGraphLaunch(c->graph[kind], c->stream));
+    } else {
+      if ((rc = enqueue_day(c, kind, LP_STAGE_ALL, 0, 1, true))) return rc;
     }
-    for (; t < t_end; ++t) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-    return LP_OK;
   }
-  for (; t < t_end; ++t)
-    if ((rc = enqueue_day(c, LP_STAGE_ALL, 0, 1, true))) return rc;
   return LP_OK;
 }
 
@@ -392,7 +437,9 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t);
   const bool fused_tx_log = (mask & LP_STAGE_TRANSMISSION) && (mask & LP_STAGE_LOG);
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
-  return enqueue_day(c, mask, log_only ? 1 : 0, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+  const int full_count = log_only ? 1 : 0;
+  return enqueue_day(c, day_kind(c, t, mask, full_count), mask, full_count,
+                     fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
 }
 
 int lp_step_vital(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_VITAL); }
@@ -460,7 +507,7 @@ const char* lp_last_error(lp_ctx* c) { return c ? c->err.c_str() : "null context
 void lp_destroy(lp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  for (int k = 0; k < 2; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -6,25 +6,31 @@
 //                  early-stop rule                     model.py:1711-1719, 788-795
 //   k_agents       ONE streaming pass over every live agent slot, node-segmented:
 //                  deaths, newborn init, E->I->R timers + paralysis, RI, SIA, and
-//                  the tally the transmission step and the log need; per-workgroup
-//                  partial sums (no global atomics) and per-chunk (256 agents)
+//                  the tally the transmission step and the log need; one partial
+//                  per span (no global atomics) and per-chunk (256 agents)
 //                  susceptible weight sums      model.py:1767-1781, 393-454,
 //                                               1784-1855, 1994-2060, 980-1007, 902-929
-//   k_node_reduce  per node: reduce workgroup partials, exclusive prefix of the
-//                  chunk weights, results rows of VD/DS/RI/SIA
+//                  (a PLAIN variant for days without births/deaths/RI/SIA)
+//   k_node_reduce  per node: reduce span partials, exclusive prefix of the span
+//                  weights, results rows of VD/DS/RI/SIA
 //   k_mix_foi      per node: network mix (fixed-order tree), seasonality, r0 scalars,
 //                  rate -> probability, expected exposures, Poisson / zero-inflated NB
 //                  count draw (chunks in parallel across the wave)  model.py:1327-1407
-//   k_plan/k_resolve/k_mark (x rounds), k_prefix, k_finish
+//   k_plan/k_resolve/k_mark (round 1), k_prefix/k_plan/k_resolve/k_mark (round 2),
+//   k_finish
 //                  weighted sampling without replacement by rounds of with-
 //                  replacement probes + unique (model.py:1089-1122): every probe
-//                  searches the node's chunk prefix, then one chunk, wave-parallel
+//                  searches its node's span prefix, then a span's chunks, then one
+//                  chunk, one wavefront per probe; k_finish takes the remaining
+//                  rounds and the nodes where every susceptible is exposed
 //   k_expose       per selected agent: strain draw, S -> E       model.py:1125-1147
-//   k_day_end      per node: log rows + transmission incidence   model.py:1436-1441, 1461-1483
-//   k_advance      day counter += 1 (days are replayed as one hipGraph)
+//   k_day_end      per node: log rows + transmission incidence; its last
+//                  workgroup advances the day counter      model.py:1436-1441, 1461-1483
+// A day is replayed as one hipGraph (one graph per agent-pass variant).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <limits.h>
+#include <type_traits>
 #include "lp_device.h"
 #include "../../include/lp_step.h"
 
@@ -69,15 +75,16 @@ struct Dev {
   // scratch
   int32_t* day; int32_t* stop_day; int32_t* err;
   const int4* spans; const int32_t* node_span_begin;
-  int64_t* partials; int64_t* chunk_sum; int64_t* chunk_prefix;
+  int64_t* partials; int64_t* chunk_sum;
   int64_t* nb_begin; int32_t* births;
   int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
   int64_t* probe_off; int64_t* round_base; uint64_t* sel_list; int64_t* sel_count; int64_t sel_cap;
-  int64_t* fin_prefix;
-  int64_t capacity; int32_t n_nodes, n_sims, K, nt;
+  int64_t* fin_prefix; int64_t* span_prefix;
+  uint32_t* ticket;   // [4] last-block-done counters
+  int64_t capacity; int32_t n_nodes, n_sims, K, nt, n_spans, tpw;
 };
 
 __device__ __forceinline__ bool sim_stopped(const Dev& d, int sim, int t) { return d.stop_day[sim] < t; }
@@ -94,7 +101,6 @@ __device__ __forceinline__ int32_t stage_day_flags(int32_t flags, int mask) {
 
 // ---------------------------------------------------------------------------
 __global__ void k_set_day(Dev d, int32_t t) { if (threadIdx.x == 0 && blockIdx.x == 0) d.day[0] = t; }
-__global__ void k_advance(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0) d.day[0] += 1; }
 
 // Kaplan-Meier style age at death (days) for an agent of `age` days, from a
 // cumulative deaths table by year of age: year of death drawn in proportion to
@@ -126,6 +132,7 @@ __device__ int32_t km_age_at_death(const int64_t* cd, uint64_t seed, uint64_t sl
 __global__ void k_day_begin(Dev d, int mask) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) d.sel_count[0] = 0;
   if (n >= d.n_nodes) return;
   const int sim = d.node_sim[n];
   const lp_sim_params& sp = d.sims[sim];
@@ -159,233 +166,400 @@ __global__ void k_day_begin(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-// The agent pass.  One workgroup per span (<= tiles_per_block tiles of one node);
-// each thread owns 4 consecutive slots of a 1024-slot tile, each wave one
-// 256-slot sampling chunk.
-struct Tally {
-  int32_t c[27];     // counts: fields 0..26
-  int64_t sw, ib[LP_MAX_STRAINS];
-};
+// The agent pass.  Persistent workgroups walk the span table (a span = up to U
+// tiles of 1024 slots of one node).  Thread tid owns slots [tile*1024 + 4*tid, +4)
+// of each tile, so each wave covers one 256-slot sampling chunk per tile.
+//   phase A  the day's streaming columns of all U tiles (state, risk; dod on VD
+//            step days; ri_timer + chronically_missed on RI days; dob on SIA days)
+//   phase B  births and deaths                          model.py:1719-1732, 1775-1777
+//   phase C  one vector load per E/I column for every quad holding an E or I agent
+//   phase D  quads without E/I on plain days: tally only (fast path); otherwise
+//            disease progression, RI, SIA and the tally from registers
+// Common tallies stay in registers, rare ones (E/I, deaths, paralysis,
+// vaccination) go through LDS atomics; one partial per span, no global atomics.
+enum : int { L_E0 = 0, L_I0 = 4, L_SIAPK0 = 8, L_POTP = 12, L_P, L_DEATHS, L_DPP, L_DP, L_NEWPP, L_NEWP,
+             L_RIV, L_RIP, L_IPV, L_SIAV, L_SIAP, L_N };
 
+__device__ __forceinline__ int8_t byte_of(uint32_t w, int a) { return (int8_t)((w >> (8 * a)) & 0xffu); }
+__device__ __forceinline__ int32_t lane_of(const int4& v, int a) { return a == 0 ? v.x : a == 1 ? v.y : a == 2 ? v.z : v.w; }
+__device__ __forceinline__ float lane_of(const float4& v, int a) { return a == 0 ? v.x : a == 1 ? v.y : a == 2 ? v.z : v.w; }
+
+// Calls f(std::integral_constant<int, u>) for u = 0..U-1 with u a compile-time constant.
+template <int U, int u = 0, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (u < U) {
+    f(std::integral_constant<int, u>{});
+    static_for<U, u + 1>(f);
+  }
+}
+
+// Day kinds of the agent pass: PLAIN days have no births, deaths, RI or SIA
+// (about 80% of days) and compile to a small kernel; FULL handles everything.
+enum : int { KIND_PLAIN = 0, KIND_FULL = 1 };
+
+__device__ __forceinline__ bool is_plain_day(int32_t flags, int full_count) {
+  return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) && !full_count;
+}
+
+template <int U, int KIND>
 __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) {
-  const int32_t t = d.day[0];
+  // independent loads first: span descriptor and day; then node/sim/day tables
   const int4 span = d.spans[blockIdx.x];
-  const int node = span.x;
-  const int sim = d.node_sim[node];
-  if (sim_stopped(d, sim, t)) return;
-  const lp_sim_params& sp = d.sims[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  const bool do_deaths = flags & LP_DAY_VD_STEP;
-  const bool do_ds = flags & LP_DAY_DS;
-  const bool do_ri = flags & LP_DAY_RI;
-  const bool do_sia = flags & LP_DAY_SIA;
-  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const int32_t t = d.day[0];
+  const int node = span.x, sim = span.w;
   const int64_t live_hi = d.live_end[node];
-  const int64_t nb_lo = d.nb_begin[node];
-  const uint64_t seed = sp.seed;
-  const int64_t sbase = sp.slot_base;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int K = d.K;
+  const int32_t stop_day = d.stop_day[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const bool plain = is_plain_day(flags, full_count);
+  if ((KIND == KIND_PLAIN) != plain || stop_day < t) return;
+  const bool do_deaths = KIND == KIND_FULL && (flags & LP_DAY_VD_STEP);
+  const bool do_ds = flags & LP_DAY_DS;
+  const bool do_ri = KIND == KIND_FULL && (flags & LP_DAY_RI);
+  const bool do_sia = KIND == KIND_FULL && (flags & LP_DAY_SIA);
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const bool need_ei = do_ds || do_tally;
+  const bool plain_day = KIND == KIND_PLAIN;
   const int32_t sia_b = do_sia ? d.day_sia_begin[t] : 0;
   const int32_t sia_n = do_sia ? d.day_sia_count[t] : 0;
-  const uint64_t thr_ri = do_ri ? d.thr_ri[node] : 0, thr_ipv = do_ri ? d.thr_ipv[node] : 0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  Tally tl;
-  for (int q = 0; q < 27; ++q) tl.c[q] = 0;
-  tl.sw = 0;
-  for (int k = 0; k < LP_MAX_STRAINS; ++k) tl.ib[k] = 0;
+  __shared__ int32_t lc[L_N];
+  __shared__ unsigned long long lib[LP_MAX_STRAINS];
+  __shared__ int64_t red[WG / 64][4];
 
-  for (int ti = 0; ti < span.z; ++ti) {
-    const int64_t base = ((int64_t)span.y + ti) * LP_TILE;
-    if (base >= live_hi) break;
-    const int64_t i0 = base + (int64_t)tid * 4;
-    uint32_t st4 = (i0 < live_hi) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
-    const uint32_t st4_in = st4;
-    int64_t chunk_w = 0;
-    float4 rk = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool risk_loaded = false;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int64_t i = i0 + a;
-      int8_t s = (int8_t)((st4 >> (8 * a)) & 0xffu);
-      if (i >= live_hi) continue;
-      const uint64_t slot = (uint64_t)(i - sbase);
-      if (i >= nb_lo) {                                   // newborn (VD births, model.py:1719-1732)
-        d.dob[i] = t;
-        d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
-        d.node_id[i] = (int16_t)node;
-        if (sp.newborn_ri_timer >= 0) d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
-        s = 0;
-      } else if (do_deaths && s >= 0) {                   // get_deaths (model.py:1775-1777)
-        if (d.dod[i] <= t) {
-          tl.c[P_DEATHS] += 1;
-          if (d.pp[i] == 1) tl.c[P_DPP] += 1;
-          if (d.paralyzed[i] == 1) tl.c[P_DP] += 1;
-          s = -1;
-        }
-      }
-      if (do_ds && (s == 1 || s == 2)) {                  // disease_state_step_kernel (model.py:419-452)
-        if (s == 1) {
-          const uint16_t et = d.exp_timer[i];
-          if (et == 0) s = 2; else d.exp_timer[i] = (uint16_t)(et - 1);
-        }
-        if (s == 2) {
-          const uint16_t it = d.inf_timer[i];
-          if (it == 0) s = 3; else d.inf_timer[i] = (uint16_t)(it - 1);
-          if (d.strain[i] == 0) {
-            const uint16_t pt = d.par_timer[i];
-            if (pt == 0 && d.pp[i] == -1) {
-              if (d.ipv[i] == 0) {
-                d.pp[i] = 1;
-                tl.c[P_NEWPP] += 1;
-                const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
-                if ((uint64_t)o.x < sp.thr_paralysis) { d.paralyzed[i] = 1; tl.c[P_NEWP] += 1; }
-              } else {
-                d.pp[i] = 0;
-              }
-            }
-            if (pt > 0) d.par_timer[i] = (uint16_t)(pt - 1);
-          }
-        }
-      }
-      if ((do_ri || do_sia) && s >= 0 && d.missed[i] != 1) {
-        if (do_ri) {                                      // fast_ri (model.py:1826-1854)
-          const int32_t step = sp.ri_step;
-          const int16_t timer = (int16_t)(d.ri_timer[i] - step);
-          d.ri_timer[i] = timer;
-          bool eligible = false;
-          if (t == step) eligible = timer <= 0 && timer >= -step;
-          else if (t > step) eligible = timer <= 0 && timer > -step;
-          if (eligible) {
-            const uint4 o = draw(seed, slot, t, ST_RI);
-            if ((uint64_t)o.x < thr_ri) {
-              tl.c[P_RIV] += 1;
-              if (s == 0) { s = 1; d.strain[i] = (int8_t)sp.ri_strain; tl.c[P_RIP] += 1; }
-            }
-            if ((uint64_t)o.y < thr_ipv) { tl.c[P_IPV] += 1; d.ipv[i] = 1; }
-          }
-        }
-        if (do_sia) {                                     // fast_sia (model.py:2033-2059), per event
-          const int32_t age = t - d.dob[i];
-          for (int e = 0; e < sia_n; ++e) {
-            const int ev = sia_b + e;
-            if (!(d.sia_min_age[ev] <= age && age <= d.sia_max_age[ev])) continue;
-            const uint64_t t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
-            if (t1 == 0) continue;
-            const uint4 o = draw(seed, slot, t, ST_SIA | ((uint32_t)e << 8));
-            if ((uint64_t)o.x < t1) {
-              const bool last = (e == sia_n - 1);
-              if (last) tl.c[P_SIAV] += 1;
-              if (s == 0 && (uint64_t)o.x < d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1]) {
-                s = 1;
-                const int ks = d.sia_strain[ev];
-                d.strain[i] = (int8_t)ks;
-#pragma unroll
-                for (int q = 0; q < LP_MAX_STRAINS; ++q) tl.c[P_SIAPK0 + q] += (q == ks);
-                if (last) tl.c[P_SIAP] += 1;
-              }
-            }
-          }
-        }
-      }
-      if (do_tally && s >= 0) {                           // tx_step_prep / count_SEIRP tally
-        if (s == 0) {
-          if (!risk_loaded) { rk = *reinterpret_cast<const float4*>(d.risk + i0); risk_loaded = true; }
-          const float rv = a == 0 ? rk.x : a == 1 ? rk.y : a == 2 ? rk.z : rk.w;
-          const int64_t w = weight_q24(rv);
-          tl.c[P_S] += 1;
-          tl.c[P_SPOS] += (w > 0);
-          tl.sw += w;
-          chunk_w += w;
-        } else if (s == 1) {
-          const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
-#pragma unroll
-          for (int q = 0; q < LP_MAX_STRAINS; ++q) tl.c[P_E0 + q] += (q == k);
-        } else if (s == 2) {
-          const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
-          const int64_t wv = weight_q24(d.infect[i] * sp.strain_r0[k]);
-#pragma unroll
-          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
-            tl.c[P_I0 + q] += (q == k);
-            tl.ib[q] += (q == k) ? wv : 0;
-          }
-        } else if (s == 3) {
-          tl.c[P_R] += 1;
-        }
-        if (full_count) {
-          if (d.pp[i] == 1) tl.c[P_POTP] += 1;
-          if (d.paralyzed[i] == 1) tl.c[P_P] += 1;
-        }
-      }
-      st4 = (st4 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
-    }
-    if (st4 != st4_in && i0 < live_hi) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = st4;
-    if (do_tally) {
-      const int64_t cw = wave_sum_i64(chunk_w);
-      if (lane == 0) d.chunk_sum[(base >> 8) + wave] = cw;
-    }
-  }
-  (void)K;
-
-  // Workgroup reduction of the tally -> one partial per span.
-  __shared__ int64_t red[WG / 64][NPART];
-  for (int q = 0; q < 27; ++q) {
-    const int32_t v = wave_sum_i32(tl.c[q]);
-    if (lane == 0) red[wave][q] = v;
-  }
   {
-    const int64_t v = wave_sum_i64(tl.sw);
-    if (lane == 0) red[wave][P_SW] = v;
-  }
-  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
-    const int64_t v = wave_sum_i64(tl.ib[k]);
-    if (lane == 0) red[wave][P_IB0 + k] = v;
-  }
-  __syncthreads();
-  if (tid < NPART) {
-    int64_t acc = 0;
-    for (int w = 0; w < WG / 64; ++w) acc += red[w][tid];
-    d.partials[(size_t)blockIdx.x * NPART + tid] = acc;
+    const int sidx = blockIdx.x;
+    const lp_sim_params& sp = d.sims[sim];
+    const int64_t nb_lo = KIND == KIND_FULL ? d.nb_begin[node] : live_hi;
+    const uint64_t seed = sp.seed;
+    const int64_t sbase = sp.slot_base;
+    const uint64_t thr_ri = do_ri ? d.thr_ri[node] : 0, thr_ipv = do_ri ? d.thr_ipv[node] : 0;
+    if (tid < L_N) lc[tid] = 0;
+    if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+
+    // ---- phase A ----
+    uint32_t st[U], st0[U];
+    float4 rk[U];
+    int4 dd[U], db[U];
+    uint2 rt[U];
+    uint32_t mw[U];
+    bool tv[U];
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+      const bool v = (u < span.z) && (i0 < live_hi);
+      tv[u] = v;
+      st[u] = v ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+      st0[u] = st[u];
+      rk[u] = (v && do_tally) ? *reinterpret_cast<const float4*>(d.risk + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (do_deaths) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+      if (do_ri || do_sia) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
+      if (do_ri) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
+      if (do_sia) db[u] = v ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
+    });
+    __syncthreads();
+
+    // ---- phase B ----
+    const bool vd_work = KIND == KIND_FULL && (do_deaths || nb_lo < live_hi);
+    bool ei[U];
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+      bool any = false;
+      if (tv[u]) {
+        if (vd_work) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int64_t i = i0 + a;
+            if (i >= live_hi) continue;
+            int8_t s = byte_of(st[u], a);
+            if (i >= nb_lo) {
+              const uint64_t slot = (uint64_t)(i - sbase);
+              d.dob[i] = t;
+              d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
+              d.node_id[i] = (int16_t)node;
+              if (sp.newborn_ri_timer >= 0) {
+                d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
+                if (do_ri) {
+                  const uint32_t hv = (uint32_t)(uint16_t)sp.newborn_ri_timer << (16 * (a & 1));
+                  const uint32_t hm = 0xffffu << (16 * (a & 1));
+                  if (a < 2) rt[u].x = (rt[u].x & ~hm) | hv; else rt[u].y = (rt[u].y & ~hm) | hv;
+                }
+              }
+              if (do_sia) { if (a == 0) db[u].x = t; else if (a == 1) db[u].y = t; else if (a == 2) db[u].z = t; else db[u].w = t; }
+              s = 0;
+            } else if (do_deaths && s >= 0 && lane_of(dd[u], a) <= t) {
+              atomicAdd(&lc[L_DEATHS], 1);
+              if (d.pp[i] == 1) atomicAdd(&lc[L_DPP], 1);
+              if (d.paralyzed[i] == 1) atomicAdd(&lc[L_DP], 1);
+              s = -1;
+            }
+            st[u] = (st[u] & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+          }
+        }
+        // bytes equal to 1 or 2 (E or I): test all four lanes of the word at once
+        const uint32_t w = st[u];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int8_t s = byte_of(w, a);
+          any = any || ((s == 1 || s == 2) && i0 + a < live_hi);
+        }
+      }
+      ei[u] = any && need_ei;
+    });
+
+    // ---- phase C ----
+    uint2 xe[U], xi[U], xp[U];
+    uint32_t xs[U], xpp[U], xipv[U];
+    float4 xin[U];
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+      if (ei[u]) {
+        xe[u] = *reinterpret_cast<const uint2*>(d.exp_timer + i0);
+        xi[u] = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
+        xp[u] = *reinterpret_cast<const uint2*>(d.par_timer + i0);
+        xs[u] = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
+        xpp[u] = reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2];
+        xipv[u] = reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2];
+        xin[u] = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+      } else {
+        xe[u] = xi[u] = xp[u] = make_uint2(0u, 0u);
+        xs[u] = xpp[u] = xipv[u] = 0u;
+        xin[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    });
+
+    // ---- phase D ----
+    int32_t c_s = 0, c_spos = 0, c_r = 0;
+    int64_t c_sw = 0;
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+      int64_t chunk_w = 0;
+      if (tv[u] && plain_day && !ei[u]) {
+        // fast path: S and R agents only, no campaign today
+        if (do_tally) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int8_t s = byte_of(st[u], a);
+            const bool live = i0 + a < live_hi;
+            if (live && s == 0) {
+              const int64_t wt = weight_q24(lane_of(rk[u], a));
+              c_s += 1;
+              c_spos += (wt > 0);
+              c_sw += wt;
+              chunk_w += wt;
+            }
+            c_r += (live && s == 3);
+          }
+        }
+        if (vd_work && st[u] != st0[u]) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = st[u];
+      } else if (tv[u]) {
+        uint32_t w = st[u];
+        uint2 e2 = xe[u], i2 = xi[u], p2 = xp[u], r2 = rt[u];
+        uint32_t ppw = xpp[u];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int64_t i = i0 + a;
+          if (i >= live_hi) continue;
+          int8_t s = byte_of(w, a);
+          const uint64_t slot = (uint64_t)(i - sbase);
+          int k = byte_of(xs[u], a) & (LP_MAX_STRAINS - 1);
+          const int sh16 = 16 * (a & 1);
+          const uint32_t m16 = 0xffffu << sh16;
+          if (do_ds && (s == 1 || s == 2)) {              // disease_state_step_kernel (model.py:419-452)
+            if (s == 1) {
+              const uint32_t ew = a < 2 ? e2.x : e2.y;
+              const uint16_t et = (uint16_t)((ew >> sh16) & 0xffffu);
+              if (et == 0) s = 2;
+              else {
+                const uint32_t nw = (ew & ~m16) | ((uint32_t)(et - 1) << sh16);
+                if (a < 2) e2.x = nw; else e2.y = nw;
+              }
+            }
+            if (s == 2) {
+              const uint32_t iw = a < 2 ? i2.x : i2.y;
+              const uint16_t it = (uint16_t)((iw >> sh16) & 0xffffu);
+              if (it == 0) s = 3;
+              else {
+                const uint32_t nw = (iw & ~m16) | ((uint32_t)(it - 1) << sh16);
+                if (a < 2) i2.x = nw; else i2.y = nw;
+              }
+              if (byte_of(xs[u], a) == 0) {
+                const uint32_t pw = a < 2 ? p2.x : p2.y;
+                const uint16_t pt = (uint16_t)((pw >> sh16) & 0xffffu);
+                if (pt == 0 && byte_of(ppw, a) == -1) {
+                  const uint32_t bm = 0xffu << (8 * a);
+                  if (byte_of(xipv[u], a) == 0) {
+                    ppw = (ppw & ~bm) | (1u << (8 * a));
+                    atomicAdd(&lc[L_NEWPP], 1);
+                    const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
+                    if ((uint64_t)o.x < sp.thr_paralysis) { d.paralyzed[i] = 1; atomicAdd(&lc[L_NEWP], 1); }
+                  } else {
+                    ppw = ppw & ~bm;
+                  }
+                }
+                if (pt > 0) {
+                  const uint32_t nw = (pw & ~m16) | ((uint32_t)(pt - 1) << sh16);
+                  if (a < 2) p2.x = nw; else p2.y = nw;
+                }
+              }
+            }
+          }
+          if (KIND == KIND_FULL && (do_ri || do_sia) && s >= 0 && byte_of(mw[u], a) != 1) {
+            if (do_ri) {                                  // fast_ri (model.py:1826-1854)
+              const int32_t step = sp.ri_step;
+              const uint32_t rw = a < 2 ? r2.x : r2.y;
+              const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - step);
+              const uint32_t nw = (rw & ~m16) | ((uint32_t)(uint16_t)timer << sh16);
+              if (a < 2) r2.x = nw; else r2.y = nw;
+              bool eligible = false;
+              if (t == step) eligible = timer <= 0 && timer >= -step;
+              else if (t > step) eligible = timer <= 0 && timer > -step;
+              if (eligible) {
+                const uint4 o = draw(seed, slot, t, ST_RI);
+                if ((uint64_t)o.x < thr_ri) {
+                  atomicAdd(&lc[L_RIV], 1);
+                  if (s == 0) { s = 1; k = sp.ri_strain; d.strain[i] = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
+                }
+                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); d.ipv[i] = 1; }
+              }
+            }
+            if (do_sia) {                                 // fast_sia (model.py:2033-2059), per event
+              const int32_t age = t - lane_of(db[u], a);
+              for (int e = 0; e < sia_n; ++e) {
+                const int ev = sia_b + e;
+                if (!(d.sia_min_age[ev] <= age && age <= d.sia_max_age[ev])) continue;
+                const uint64_t t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
+                if (t1 == 0) continue;
+                const uint4 o = draw(seed, slot, t, ST_SIA | ((uint32_t)e << 8));
+                if ((uint64_t)o.x < t1) {
+                  const bool last = (e == sia_n - 1);
+                  if (last) atomicAdd(&lc[L_SIAV], 1);
+                  if (s == 0 && (uint64_t)o.x < d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1]) {
+                    s = 1;
+                    k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
+                    d.strain[i] = (int8_t)k;
+                    atomicAdd(&lc[L_SIAPK0 + k], 1);
+                    if (last) atomicAdd(&lc[L_SIAP], 1);
+                  }
+                }
+              }
+            }
+          }
+          if (do_tally && s >= 0) {                       // tx_step_prep / count_SEIRP tally
+            if (s == 0) {
+              const int64_t wt = weight_q24(lane_of(rk[u], a));
+              c_s += 1;
+              c_spos += (wt > 0);
+              c_sw += wt;
+              chunk_w += wt;
+            } else if (s == 3) {
+              c_r += 1;
+            } else if (s == 1) {
+              atomicAdd(&lc[L_E0 + k], 1);
+            } else if (s == 2) {
+              atomicAdd(&lc[L_I0 + k], 1);
+              const int64_t wv = weight_q24(lane_of(xin[u], a) * sp.strain_r0[k]);
+              atomicAdd(&lib[k], (unsigned long long)wv);
+            }
+            if (KIND == KIND_FULL && full_count) {
+              if (d.pp[i] == 1 || byte_of(ppw, a) == 1) atomicAdd(&lc[L_POTP], 1);
+              if (d.paralyzed[i] == 1) atomicAdd(&lc[L_P], 1);
+            }
+          }
+          w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+        }
+        if (w != st0[u]) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+        if (ei[u]) {
+          if (e2.x != xe[u].x || e2.y != xe[u].y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
+          if (i2.x != xi[u].x || i2.y != xi[u].y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
+          if (p2.x != xp[u].x || p2.y != xp[u].y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
+          if (ppw != xpp[u]) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+        }
+        if (do_ri && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
+      }
+      if (do_tally && u < span.z && ((int64_t)span.y + u) * LP_TILE < live_hi) {
+        const int64_t cw = wave_sum_i64(chunk_w);
+        if (lane == 0) d.chunk_sum[(((int64_t)span.y + u) * LP_TILE >> 8) + wave] = cw;
+      }
+    });
+
+    // ---- span partial ----
+    {
+      const int32_t a0 = wave_sum_i32(c_s), a1 = wave_sum_i32(c_spos), a2 = wave_sum_i32(c_r);
+      const int64_t a3 = wave_sum_i64(c_sw);
+      if (lane == 0) { red[wave][0] = a0; red[wave][1] = a1; red[wave][2] = a2; red[wave][3] = a3; }
+    }
+    __syncthreads();
+    if (tid < NPART) {
+      int64_t v = 0;
+      if (tid == P_S || tid == P_SPOS || tid == P_R || tid == P_SW) {
+        const int q = tid == P_S ? 0 : tid == P_SPOS ? 1 : tid == P_R ? 2 : 3;
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w) v += red[w][q];
+      } else if (tid >= P_E0 && tid < P_E0 + 4) v = lc[L_E0 + tid - P_E0];
+      else if (tid >= P_I0 && tid < P_I0 + 4) v = lc[L_I0 + tid - P_I0];
+      else if (tid >= P_SIAPK0 && tid < P_SIAPK0 + 4) v = lc[L_SIAPK0 + tid - P_SIAPK0];
+      else if (tid >= P_IB0 && tid < P_IB0 + 4) v = (int64_t)lib[tid - P_IB0];
+      else {
+        switch (tid) {
+          case P_POTP: v = lc[L_POTP]; break;
+          case P_P: v = lc[L_P]; break;
+          case P_DEATHS: v = lc[L_DEATHS]; break;
+          case P_DPP: v = lc[L_DPP]; break;
+          case P_DP: v = lc[L_DP]; break;
+          case P_NEWPP: v = lc[L_NEWPP]; break;
+          case P_NEWP: v = lc[L_NEWP]; break;
+          case P_RIV: v = lc[L_RIV]; break;
+          case P_RIP: v = lc[L_RIP]; break;
+          case P_IPV: v = lc[L_IPV]; break;
+          case P_SIAV: v = lc[L_SIAV]; break;
+          case P_SIAP: v = lc[L_SIAP]; break;
+          default: break;
+        }
+      }
+      d.partials[(size_t)sidx * NPART + tid] = v;
+    }
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup exclusive scan of a node's chunk weights; returns the total.
-// `atomic_io` reads chunk sums / writes the prefix with agent-scope atomics.
+// Exclusive scan of a node's span weights (partial field P_SW of spans [s0, s1))
+// into prefix[s]; returns the total.  One wavefront scans 64 spans per step.
+// ATOMIC_IO reads the weights / writes the prefix with agent-scope atomics.
 template <bool ATOMIC_IO>
-__device__ int64_t node_chunk_scan(const int64_t* cs, int64_t* prefix, int64_t c0, int64_t c1) {
-  __shared__ int64_t sh[WG];
-  const int tid = threadIdx.x;
-  const int64_t nch = c1 - c0;
-  const int64_t per = (nch + WG - 1) / WG;
-  const int64_t a = c0 + (int64_t)tid * per;
-  const int64_t b = (a + per < c1) ? a + per : c1;
-  int64_t s = 0;
-  for (int64_t c = a; c < b; ++c) {
-    const int64_t v = ATOMIC_IO ? __hip_atomic_load(cs + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cs[c];
-    s += v;
+__device__ int64_t node_span_scan(int64_t* partials, int64_t* prefix, int s0, int s1) {
+  __shared__ int64_t total_sh;
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid < 64) {
+    int64_t carry = 0;
+    for (int base = s0; base < s1; base += 64) {
+      const int s = base + lane;
+      int64_t v = 0;
+      if (s < s1) {
+        int64_t* p = partials + (size_t)s * NPART + P_SW;
+        v = ATOMIC_IO ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+      }
+      const int64_t incl = wave_incl_scan_i64(v, lane);
+      if (s < s1) {
+        const int64_t ex = carry + incl - v;
+        if (ATOMIC_IO) __hip_atomic_store(prefix + s, ex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else prefix[s] = ex;
+      }
+      carry += __shfl(incl, 63, 64);
+    }
+    if (tid == 0) total_sh = carry;
   }
-  sh[tid] = s;
   __syncthreads();
-  for (int o = 1; o < WG; o <<= 1) {
-    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
-  }
-  int64_t run = sh[tid] - s;  // exclusive
-  for (int64_t c = a; c < b; ++c) {
-    const int64_t v = ATOMIC_IO ? __hip_atomic_load(cs + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cs[c];
-    if (ATOMIC_IO) __hip_atomic_store(prefix + c, run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else prefix[c] = run;
-    run += v;
-  }
-  const int64_t total = sh[WG - 1];
+  const int64_t total = total_sh;
   __syncthreads();
   return total;
 }
 
-// Per node: reduce partials, chunk prefix, results rows of VD / DS / RI / SIA.
+// Per node: reduce span partials, span prefix, results rows of VD / DS / RI / SIA.
 __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_count) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x;
@@ -394,18 +568,34 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_co
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
   __shared__ int64_t tally[NPART];
+  __shared__ int64_t grp[WG / NPART][NPART];
   const int tid = threadIdx.x;
+  const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
+  {
+    // thread (f = tid % 32, g = tid / 32) sums field f of spans g, g+8, ...: coalesced
+    const int f = tid % NPART, g = tid / NPART;
+    int64_t acc = 0;
+    constexpr int G = WG / NPART;
+    for (int s = s0 + g; s < s1; s += 4 * G) {
+      const int64_t a0 = d.partials[(size_t)s * NPART + f];
+      const int64_t a1 = s + G < s1 ? d.partials[(size_t)(s + G) * NPART + f] : 0;
+      const int64_t a2 = s + 2 * G < s1 ? d.partials[(size_t)(s + 2 * G) * NPART + f] : 0;
+      const int64_t a3 = s + 3 * G < s1 ? d.partials[(size_t)(s + 3 * G) * NPART + f] : 0;
+      acc += (a0 + a1) + (a2 + a3);
+    }
+    grp[g][f] = acc;
+  }
+  __syncthreads();
   if (tid < NPART) {
     int64_t acc = 0;
-    for (int s = d.node_span_begin[n]; s < d.node_span_begin[n + 1]; ++s) acc += d.partials[(size_t)s * NPART + tid];
+#pragma unroll
+    for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
     tally[tid] = acc;
     d.node_tally[(size_t)n * NPART + tid] = acc;
   }
   __syncthreads();
   if (flags & LP_DAY_TX) {
-    const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
-    const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
-    const int64_t W = node_chunk_scan<false>(d.chunk_sum, d.chunk_prefix, c0, c1);
+    const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
     if (tid == 0) d.W_round[n] = W;
   }
   if (tid != 0) return;
@@ -452,18 +642,57 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_co
 }
 
 // ---------------------------------------------------------------------------
-// Per node: network mix, FOI, count draw.  Workgroup per node.
-__global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
-  const int32_t t = d.day[0];
-  const int i = blockIdx.x;
-  const int sim = d.node_sim[i];
-  const lp_sim_params& sp = d.sims[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  const int tid = threadIdx.x, lane = tid & 63;
-  if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
-    if (tid == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
-    return;
+// True in exactly one workgroup of the launch: the last one to get here.  Every
+// workgroup's global writes before the call are visible to it afterwards.
+__device__ bool last_block(uint32_t* ticket) {
+  __shared__ bool am_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t v = atomicAdd(ticket, 1u);
+    am_last = (v == gridDim.x - 1);
+    if (am_last) atomicExch(ticket, 0u);
   }
+  __syncthreads();
+  if (am_last) __threadfence();
+  return am_last;
+}
+
+// One workgroup: exclusive scan of the probes of sampling round `round` over
+// nodes (probe_off), and reservation of their space in the selected list.
+__device__ void plan_scan(const Dev& d, int round) {
+  __shared__ int64_t sh[WG];
+  const int tid = threadIdx.x;
+  const int N = d.n_nodes;
+  const int per = (N + WG - 1) / WG;
+  const int a = tid * per, b = (a + per < N) ? a + per : N;
+  int64_t s = 0;
+  for (int n = a; n < b; ++n) s += __hip_atomic_load(d.m_round + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sh[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < WG; o <<= 1) {
+    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = sh[tid] - s;
+  for (int n = a; n < b; ++n) {
+    d.probe_off[n] = run;
+    run += __hip_atomic_load(d.m_round + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (tid == WG - 1) {
+    int64_t M = sh[WG - 1];
+    const int64_t base = __hip_atomic_load(d.sel_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (base + M > d.sel_cap) { atomicOr(d.err, ERR_OVERFLOW); M = 0; }
+    d.probe_off[N] = M;
+    d.round_base[round] = base;
+    d.sel_count[0] = base + M;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& sp, int32_t t, int tid, int lane) {
   const int K = d.K;
   const int ns = sp.n_nodes, li = i - sp.node_begin;
   const float* NT = d.network_t + sp.net_off + (size_t)li * ns;
@@ -587,51 +816,43 @@ __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
   }
 }
 
+// Per node: network mix, FOI, count draw.  Workgroup per node.
+__global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int i = blockIdx.x;
+  const int sim = d.node_sim[i];
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
+    if (tid == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
+  } else {
+    mix_foi_node(d, i, sim, sp, t, tid, lane);
+  }
+}
+
+
 // ---------------------------------------------------------------------------
-// Sampling round r >= 2: refresh the node's chunk prefix from the decremented
-// chunk sums, and the number of probes still needed.
+// Sampling round r >= 2: refresh the node's span prefix from the decremented
+// span weights, and the number of probes still needed.
 __global__ void __launch_bounds__(WG) k_prefix(Dev d) {
   const int n = blockIdx.x;
   const int32_t need = d.size[n] - d.n_uniq[n];
-  if (d.take_all[n] || need <= 0 || d.size[n] == 0) { if (threadIdx.x == 0) d.m_round[n] = 0; return; }
-  const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
-  const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
-  const int64_t W = node_chunk_scan<false>(d.chunk_sum, d.chunk_prefix, c0, c1);
-  if (threadIdx.x == 0) { d.W_round[n] = W; d.m_round[n] = W > 0 ? need : 0; }
-}
-
-// Single workgroup: exclusive scan of probes over nodes, reserve list space.
-__global__ void __launch_bounds__(1024) k_plan(Dev d, int round) {
-  __shared__ int64_t sh[1024];
-  const int tid = threadIdx.x;
-  const int N = d.n_nodes;
-  const int per = (N + 1023) / 1024;
-  const int a = tid * per, b = (a + per < N) ? a + per : N;
-  int64_t s = 0;
-  for (int n = a; n < b; ++n) s += d.m_round[n];
-  sh[tid] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
-  }
-  int64_t run = sh[tid] - s;
-  for (int n = a; n < b; ++n) { d.probe_off[n] = run; run += d.m_round[n]; }
-  if (tid == 1023) {
-    int64_t M = sh[1023];
-    const int64_t base = d.sel_count[0];
-    if (base + M > d.sel_cap) { atomicOr(d.err, ERR_OVERFLOW); M = 0; }
-    d.probe_off[N] = M;
-    d.round_base[round] = base;
-    d.sel_count[0] = base + M;
+  if (d.take_all[n] || need <= 0 || d.size[n] == 0) {
+    if (threadIdx.x == 0) d.m_round[n] = 0;
+  } else {
+    const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, d.node_span_begin[n], d.node_span_begin[n + 1]);
+    if (threadIdx.x == 0) { d.W_round[n] = W; d.m_round[n] = W > 0 ? need : 0; }
   }
 }
 
-// One wavefront per probe: target T in [0, W), find the chunk with prefix <= T by a
-// 64-ary search, then the agent inside the chunk by a wave prefix scan.  Writes the
-// hit (slot | node << 40) to the selected list; nothing is mutated here.
+// Single workgroup: plan sampling round `round` (probe offsets over nodes, list space).
+__global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, round); }
+
+// One wavefront per probe: target T in [0, W); the span with prefix <= T by a
+// 64-ary search over the node's span prefix, the chunk inside the span by a wave
+// scan of its chunk weights, the agent inside the chunk by a wave scan of the
+// agents' weights.  Returns the slot, or -1 if the weights were inconsistent.
 __device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t p) {
   int a = 0, b = N;   // largest n in [0, N) with off[n] <= p < off[n+1]
   while (b - a > 1) { const int m = (a + b) >> 1; if (off[m] <= p) a = m; else b = m; }
@@ -639,23 +860,37 @@ __device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t 
 }
 
 template <bool ATOMIC_IO>
+__device__ __forceinline__ int64_t ld64(const int64_t* p) {
+  return ATOMIC_IO ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+}
+
+template <bool ATOMIC_IO>
 __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
-  int64_t lo = d.seg_begin[node] / LP_CHUNK;
-  int64_t hi = (d.live_end[node] + LP_CHUNK - 1) / LP_CHUNK;   // search [lo, hi)
+  int64_t lo = d.node_span_begin[node];
+  int64_t hi = d.node_span_begin[node + 1];   // search spans [lo, hi)
   while (hi - lo > 1) {
-    const int64_t span = hi - lo;
-    const int64_t step = (span + 63) / 64;
+    const int64_t range = hi - lo;
+    const int64_t step = (range + 63) / 64;
     const int64_t c = lo + (int64_t)lane * step;
-    int64_t v = INT64_MAX;
-    if (c < hi) v = ATOMIC_IO ? __hip_atomic_load(prefix + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : prefix[c];
-    const uint64_t ok = __ballot(v <= T);        // lanes are monotone: lane 0 always true
+    const int64_t v = (c < hi) ? ld64<ATOMIC_IO>(prefix + c) : INT64_MAX;
+    const uint64_t ok = __ballot(v <= T);        // monotone: lane 0 always true
     const int L = 63 - __clzll(ok);
     lo = lo + (int64_t)L * step;
     const int64_t nhi = lo + step;
     hi = nhi < hi ? nhi : hi;
   }
-  const int64_t c = lo;
-  const int64_t cp = ATOMIC_IO ? __hip_atomic_load(prefix + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : prefix[c];
+  const int64_t s = lo;
+  const int64_t sp_base = ld64<ATOMIC_IO>(prefix + s);
+  const int4 span = d.spans[s];
+  const int64_t c0 = (int64_t)span.y * (LP_TILE / LP_CHUNK);
+  const int nc = span.z * (LP_TILE / LP_CHUNK);
+  const int64_t cw = lane < nc ? ld64<ATOMIC_IO>(d.chunk_sum + c0 + lane) : 0;
+  const int64_t cincl = wave_incl_scan_i64(cw, lane);
+  const uint64_t cb = __ballot(lane < nc && sp_base + cincl > T);
+  if (cb == 0) return -1;
+  const int Lc = __ffsll((unsigned long long)cb) - 1;
+  const int64_t cp = sp_base + __shfl(cincl, Lc, 64) - __shfl(cw, Lc, 64);
+  const int64_t c = c0 + Lc;
   const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
   uint32_t st4;
   if (ATOMIC_IO) st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -694,7 +929,7 @@ __global__ void __launch_bounds__(WG) k_resolve(Dev d, int round) {
     const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
     const int64_t W = d.W_round[node];
     const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
-    const int64_t slot = W > 0 ? resolve_probe<false>(d, node, T, d.chunk_prefix, lane) : -1;
+    const int64_t slot = W > 0 ? resolve_probe<false>(d, node, T, d.span_prefix, lane) : -1;
     if (lane == 0) {
       if (slot < 0) { d.sel_list[base + p] = SEL_INVALID; atomicOr(d.err, ERR_SAMPLING); }
       else d.sel_list[base + p] = (uint64_t)slot | ((uint64_t)node << 40);
@@ -716,6 +951,15 @@ __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
   }
 }
 
+// Remove a claimed agent's weight from its chunk and its span.
+__device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
+  const unsigned long long dw = (unsigned long long)(-weight_q24(d.risk[slot]));
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
+  const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
+  const int64_t s = d.node_span_begin[node] + (tile - first) / d.tpw;
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
+}
+
 __global__ void __launch_bounds__(WG) k_mark(Dev d, int round) {
   const int64_t M = d.probe_off[d.n_nodes];
   const int64_t base = d.round_base[round];
@@ -726,8 +970,7 @@ __global__ void __launch_bounds__(WG) k_mark(Dev d, int round) {
     const int node = (int)(e >> 40);
     if (slot >= d.capacity || node >= d.n_nodes) { d.sel_list[base + p] = SEL_INVALID; continue; }
     if (claim(d.state, slot)) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)),
-                (unsigned long long)(-weight_q24(d.risk[slot])));
+      unweight(d, node, slot);
       atomicAdd(d.n_uniq + node, 1);
     } else {
       d.sel_list[base + p] = SEL_INVALID;
@@ -777,13 +1020,12 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
     return;
   }
   const lp_sim_params& sp = d.sims[d.node_sim[n]];
-  const int64_t c0 = d.seg_begin[n] / LP_CHUNK;
-  const int64_t c1 = (d.live_end[n] + LP_CHUNK - 1) / LP_CHUNK;
+  const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
   for (int round = R_GRID + 1;; ++round) {
     if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); return; }
     const int32_t need = d.size[n] - __hip_atomic_load(d.n_uniq + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (need <= 0) return;
-    const int64_t W = node_chunk_scan<true>(d.chunk_sum, d.fin_prefix, c0, c1);
+    const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
     if (W <= 0) return;
     if (tid == 0) {
       const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
@@ -816,8 +1058,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       if (e == SEL_INVALID) continue;
       const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
       if (claim(d.state, slot)) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)),
-                  (unsigned long long)(-weight_q24(d.risk[slot])));
+        unweight(d, n, slot);
         atomicAdd(d.n_uniq + n, 1);
       } else {
         __hip_atomic_store(d.sel_list + base + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -850,11 +1091,7 @@ __global__ void __launch_bounds__(WG) k_expose(Dev d) {
   }
 }
 
-// Per node: transmission incidence and the log rows.
-__global__ void k_day_end(Dev d, int mask, int tx_adjust) {
-  const int32_t t = d.day[0];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= d.n_nodes) return;
+__device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_adjust) {
   const int sim = d.node_sim[n];
   if (sim_stopped(d, sim, t)) return;
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
@@ -889,6 +1126,14 @@ __global__ void k_day_end(Dev d, int mask, int tx_adjust) {
   }
 }
 
-__global__ void k_reset_sel(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0) d.sel_count[0] = 0; }
+// Per node: transmission incidence and the log rows; with `advance`, the last
+// workgroup moves the day counter to the next tick.
+__global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, int advance) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < d.n_nodes) day_end_node(d, n, t, mask, tx_adjust);
+  if (advance && last_block(d.ticket + 2) && threadIdx.x == 0) d.day[0] = t + 1;
+}
+
 
 }  // namespace lp

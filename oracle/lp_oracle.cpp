@@ -235,6 +235,7 @@ struct orc_model {
 };
 
 int64_t orc_struct_size(void) { return (int64_t)sizeof(orc_model); }
+int32_t orc_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_philox(const uint32_t* ctr, uint64_t seed, uint32_t* out) {
   for (int i = 0; i < 4; ++i) out[i] = ctr[i];

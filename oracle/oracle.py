@@ -88,6 +88,7 @@ def lib():
         L = ctypes.CDLL(LIB)
         vp, i32, i64, f32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_uint64
         L.orc_struct_size.restype = i64
+        L.orc_max_threads.restype = i32
         L.orc_philox.argtypes = [vp, u64, vp]
         L.orc_expm1f_c.argtypes = [f32]
         L.orc_expm1f_c.restype = f32
