@@ -55,7 +55,7 @@ __device__ __forceinline__ float u24(uint32_t r) { return (float)(r >> 8) * 5.96
 // 15-bit head and a tail so k*head is exact), Taylor polynomial of expm1 on
 // |r| <= ln2/2 through r^8/8!, reconstruction by ldexp.  log: x = 2^e f,
 // f in (sqrt(1/2), sqrt(2)], log f = 2 atanh(s), s = (f-1)/(f+1), odd series
-// through s^9.  Every step is one IEEE-rounded operation in a fixed order.
+// through s^13.  Every step is one IEEE-rounded operation in a fixed order.
 constexpr float LN2_HI = 0.693145751953125f;
 constexpr float LN2_LO = 1.42860676533018672e-06f;
 constexpr float LOG2E = 1.44269502162933349609f;
@@ -97,7 +97,9 @@ __device__ __forceinline__ float logf_lp(float y) {
   if (f > 1.41421356f) { f = f * 0.5f; e += 1; }
   const float s = (f - 1.0f) / (f + 1.0f);
   const float z = s * s;
-  float p = 0.111111111111f;
+  float p = 0.0769230769231f;
+  p = p * z + 0.0909090909091f;
+  p = p * z + 0.111111111111f;
   p = p * z + 0.142857142857f;
   p = p * z + 0.2f;
   p = p * z + 0.333333333333f;

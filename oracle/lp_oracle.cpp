@@ -107,7 +107,9 @@ inline float orc_logf(float y) {  // y > 0, normal
   if (f > 1.41421356f) { f = f * 0.5f; e += 1; }
   const float s = (f - 1.0f) / (f + 1.0f);
   const float z = s * s;
-  float p = 0.111111111111f;                  // 1/9
+  float p = 0.0769230769231f;                // 1/13
+  p = p * z + 0.0909090909091f;               // 1/11
+  p = p * z + 0.111111111111f;                // 1/9
   p = p * z + 0.142857142857f;                // 1/7
   p = p * z + 0.2f;
   p = p * z + 0.333333333333f;
