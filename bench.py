@@ -12,12 +12,17 @@ Also reported on the same JSON line:
                 (formula in DESIGN.md) over its HIP-event-timed duration, against
                 MI355X's 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
-                threads) on the same initial population for the first --cpu-days days
+                threads) on the same initial population for the first --cpu-days days,
+                plus one further day on a single thread
+The HBM traffic of the agent pass (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch,
+calibrated on a plain copy) is read from profiles/k_agents_national_traffic.md
+when that file was measured on the same workload and window; otherwise null.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -62,13 +67,44 @@ def agent_pass_bytes(res, flags, t0, t1):
     return total
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_agents_national_traffic.md")
+
+
+def workload_key(scale, warmup, steps, tpw):
+    return f"scale={scale:g},warmup={warmup},steps={steps},tiles_per_block={tpw}"
+
+
+def measured_traffic(key, path=TRAFFIC_PROFILE):
+    """HBM bytes per agent-pass launch from the committed rocprofv3 summary, for this workload only."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic: (\{.*\})$", open(path).read(), re.M)
+    if not m:
+        return None
+    rec = json.loads(m.group(1))
+    return rec if rec.get("workload") == key else None
+
+
+def dump_days(path, res, flags, t0, t1):
+    """Per-day N_live, N_EI (start of day), N_draw (agents of nodes with transmission draws) and flags."""
+    pop = res["pop"].astype(np.int64)
+    ei = res["E"].astype(np.int64) + res["I"].astype(np.int64)
+    tx_new = res["new_exposed"].astype(np.int64) - res["ri_protected"] - res["sia_protected"]
+    with open(path, "w") as f:
+        f.write("t,flags,n_live,n_ei,n_draw\n")
+        for t in range(t0, t1):
+            n_draw = int(pop[t][tx_new[t] > 0].sum())
+            f.write(f"{t},{int(flags[t])},{int(pop[t].sum())},{int(ei[t - 1].sum())},{n_draw}\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2550)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0, help="population scale (1.0 = ~2.3e8 agents)")
-    ap.add_argument("--cpu-days", type=int, default=8, help="days of the CPU oracle baseline sample (0 = skip)")
+    ap.add_argument("--cpu-days", type=int, default=24, help="days of the CPU oracle baseline sample (0 = skip)")
+    ap.add_argument("--dump-days", default="", help="write per-day N_live / N_EI / N_draw / flags to this CSV")
     ap.add_argument("--graph", type=int, default=0, help="1: replay one hipGraph per day (no per-kernel events)")
     ap.add_argument("--tiles-per-block", type=int, default=0)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -132,16 +168,20 @@ def main():
     elapsed = time.perf_counter() - t0
     launches, kernel_ms = (0, 0.0) if args.graph else eng.agent_pass_timing()
     sim._mark_device_newer()
-    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "new_exposed")}
+    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "new_exposed", "ri_protected", "sia_protected")}
     agent_days = int(res["pop"][t_begin:t_end].astype(np.int64).sum())
     value = agent_days / elapsed
     log(f"[bench] {args.steps} days in {elapsed:.3f}s -> {value:.4e} agent-days/s; "
         f"exposures {int(res['new_exposed'][t_begin:t_end].sum())}")
 
+    if args.dump_days:
+        dump_days(args.dump_days, res, day_flags, t_begin, t_end)
     roofline = None
     if launches:
         bytes_alg = agent_pass_bytes(res, day_flags, t_begin, t_end)
         achieved = bytes_alg / (kernel_ms / 1e3) / 1e9
+        tpw = int(sim.pars.tiles_per_block or 2)
+        tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
         roofline = {
             "bound": "hbm",
             "kernel": "k_agents",
@@ -149,11 +189,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (tr["bytes_per_launch"] if tr else None),
             "launches": launches,
             "avg_launch_us": round(kernel_ms * 1e3 / launches, 2),
             "bytes_per_launch": int(bytes_alg // launches),
         }
+        if tr:
+            roofline["traffic_over_algorithmic"] = round(tr["bytes_per_launch"] / (bytes_alg / launches), 3)
         log(f"[bench] k_agents: {launches} launches, avg {kernel_ms * 1e3 / launches:.1f} us, {achieved:.0f} GB/s")
 
     cpu = None
@@ -168,6 +210,14 @@ def main():
         cpu_s = time.perf_counter() - tc
         cpu_ad = int(oracle_sim.res["pop"][1 : 1 + days].astype(np.int64).sum())
         threads = int(orc.lib().orc_max_threads())
+        single = None
+        if days + 1 < sim.nt:
+            oracle_sim.m.n_threads = 1
+            tc = time.perf_counter()
+            oracle_sim.day(days + 1)
+            one_s = time.perf_counter() - tc
+            single = {"value": round(int(oracle_sim.res["pop"][days + 1]) / one_s, 1), "cores": 1,
+                      "sample": f"day {days + 1} ({one_s:.1f}s)"}
         cpu = {
             "value": round(cpu_ad / cpu_s, 1),
             "unit": "agent-days/s",
@@ -175,6 +225,7 @@ def main():
             "kind": "port",
             "sample": f"days 1-{days} of the same {n0}-agent population ({cpu_s:.1f}s, OpenMP {threads} threads, "
                       f"host nproc {os.cpu_count()})",
+            "single_thread": single,
         }
         log(f"[bench] cpu oracle: {cpu['value']:.4e} agent-days/s over {days} days ({cpu_s:.1f}s)")
 

@@ -86,7 +86,7 @@ typedef struct {
   int32_t nt;              /* ticks = dur + 1 */
   int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
   int32_t tiles_per_block; /* tiles per workgroup in the agent pass (0 = default) */
-  int32_t mix_mode;        /* 0 = per-node tree reduction, 1 = batched MFMA contraction */
+  int32_t mix_mode;        /* must be 0: per-node fixed-order tree reduction of the network mix */
   int32_t reserved[7];
 } lp_config;
 

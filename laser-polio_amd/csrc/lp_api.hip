@@ -244,7 +244,7 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
   *out = nullptr;
   if (cfg->capacity <= 0 || cfg->capacity % LP_TILE != 0 || cfg->n_nodes <= 0 || cfg->n_sims <= 0 ||
       cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->n_sia_events < 0 ||
-      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 40)
+      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 40 || cfg->mix_mode != 0)
     return LP_ERR_ARG;
   lp_ctx* c = new lp_ctx();
   c->cfg = *cfg;
