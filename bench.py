@@ -44,10 +44,13 @@ def log(*a):
 def agent_pass_bytes(res, flags, t0, t1):
     """Algorithmic bytes of the agent pass over ticks [t0, t1).
 
-    per day: N_live*(1 state + 4 risk) + N_EI*17 (E/I timers r/w, strain, infectivity,
-    paralysis fields) + VD step day: N_live*4 (date_of_death) + RI day: N_live*5
-    (ri_timer r/w + chronically_missed) + SIA day: N_live*5 (date_of_birth + missed).
-    N_live = live agents that day (results.pop), N_EI = E+I at the start of the day.
+    Every day: N_live * 1 (state) + N_EI * 17 (E/I timers read and written 2*(2+2),
+    strain 1, infectivity 4, paralysis fields 4).  VD step days add N_live * 4
+    (date_of_death); RI days N_live * 4 (ri_timer read and written); SIA days
+    N_live * 4 (date_of_birth); RI or SIA days N_live * 1 (chronically_missed).
+    The susceptible statistics are carried across days, so no day reads the risk
+    column of every agent.  N_live = live agents that day (results.pop), N_EI = E+I
+    at the start of the day.
     """
     from laser_polio_amd import engine as em
 
@@ -56,13 +59,15 @@ def agent_pass_bytes(res, flags, t0, t1):
     total = 0
     for t in range(t0, t1):
         n_live, n_ei, f = int(pop[t]), int(ei[t - 1]), int(flags[t])
-        b = 5 * n_live + 17 * n_ei
+        b = 1 * n_live + 17 * n_ei
         if f & em.DAY_VD_STEP:
             b += 4 * n_live
         if f & em.DAY_RI:
-            b += 5 * n_live
+            b += 4 * n_live
         if f & em.DAY_SIA:
-            b += 5 * n_live
+            b += 4 * n_live
+        if f & (em.DAY_RI | em.DAY_SIA):
+            b += 1 * n_live
         total += b
     return total
 
@@ -180,7 +185,7 @@ def main():
     if launches:
         bytes_alg = agent_pass_bytes(res, day_flags, t_begin, t_end)
         achieved = bytes_alg / (kernel_ms / 1e3) / 1e9
-        tpw = int(sim.pars.tiles_per_block or 2)
+        tpw = int(sim.pars.tiles_per_block or 8)
         tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
         roofline = {
             "bound": "hbm",
@@ -216,7 +221,7 @@ def main():
             tc = time.perf_counter()
             oracle_sim.day(days + 1)
             one_s = time.perf_counter() - tc
-            single = {"value": round(int(oracle_sim.res["pop"][days + 1]) / one_s, 1), "cores": 1,
+            single = {"value": round(int(oracle_sim.res["pop"][days + 1].astype(np.int64).sum()) / one_s, 1), "cores": 1,
                       "sample": f"day {days + 1} ({one_s:.1f}s)"}
         cpu = {
             "value": round(cpu_ad / cpu_s, 1),

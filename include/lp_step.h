@@ -85,7 +85,7 @@ typedef struct {
   int32_t n_strains;       /* 1..LP_MAX_STRAINS */
   int32_t nt;              /* ticks = dur + 1 */
   int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
-  int32_t tiles_per_block; /* tiles per workgroup in the agent pass (0 = default) */
+  int32_t tiles_per_block; /* tiles of 1024 slots per span: 1, 2, 4, 8 or 16 (0 = default 8) */
   int32_t mix_mode;        /* must be 0: per-node fixed-order tree reduction of the network mix */
   int32_t reserved[7];
 } lp_config;
@@ -144,6 +144,12 @@ int lp_step_ri(lp_ctx* ctx, int32_t t);
 int lp_step_sia(lp_ctx* ctx, int32_t t);
 int lp_step_transmission(lp_ctx* ctx, int32_t t);
 int lp_log(lp_ctx* ctx, int32_t t);
+
+/* The caller changed bound buffers between steps (the reference lets host code
+ * edit sim.people / sim.results between component steps, e.g. seed_schedule
+ * injections, model.py:772-786).  The next agent pass recounts the per-node and
+ * per-chunk susceptible statistics the library otherwise carries from day to day. */
+int lp_invalidate(lp_ctx* ctx);
 
 /* Wait for queued work; returns LP_ERR_CAPACITY / LP_ERR_OVERFLOW if the device flagged one. */
 int lp_sync(lp_ctx* ctx);

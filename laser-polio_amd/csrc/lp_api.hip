@@ -103,6 +103,7 @@ struct lp_ctx {
   std::vector<void*> allocs;
   hipGraphExec_t graph[2] = {nullptr, nullptr};   // plain day, full day
   bool graph_ready[2] = {false, false};
+  bool recount_pending = true;                     // next agent pass recounts carried statistics
   std::vector<int32_t> day_flags_h;
   std::string err;
   bool timing = false;
@@ -170,23 +171,30 @@ int32_t host_stage_flags(int32_t flags, int mask) {
   return f;
 }
 
-int day_kind(const lp_ctx* c, int32_t t, int mask, int full_count) {
+int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
+  if (recount) return lp::KIND_RECOUNT;
   const int32_t f = host_stage_flags(c->day_flags_h[t], mask);
-  return (!(f & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) && !full_count) ? lp::KIND_PLAIN : lp::KIND_FULL;
+  return !(f & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) ? lp::KIND_PLAIN : lp::KIND_FULL;
 }
 
+// Agent pass of one day: k_plain over U = tiles per span on plain days,
+// k_agents in groups of 2 tiles otherwise.
 template <int U>
-void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask, int full_count) {
+void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
   if (kind == lp::KIND_PLAIN)
-    hipLaunchKernelGGL((lp::k_agents<U, lp::KIND_PLAIN>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask, full_count);
+    hipLaunchKernelGGL((lp::k_plain<U>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
+  else if (kind == lp::KIND_FULL)
+    hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_FULL>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
   else
-    hipLaunchKernelGGL((lp::k_agents<U, lp::KIND_FULL>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask, full_count);
+    hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
 }
 
-// The kernel sequence of one tick.  kind: agent-pass variant; mask: stages;
-// full_count: recount paralysis flags from scratch; tx_adjust: apply today's
-// transmission exposures to the log; advance: move the day counter at the end.
-int enqueue_day(lp_ctx* c, int kind, int mask, int full_count, int tx_adjust, bool advance) {
+// The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
+// recounts every carried statistic and the paralysis counts); mask: stages;
+// tx_adjust: apply today's transmission exposures to the log; advance: move
+// the day counter at the end.
+int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
+  const int recount = kind == lp::KIND_RECOUNT ? 1 : 0;
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
@@ -202,13 +210,14 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int full_count, int tx_adjust, bo
     HIPCHK(c, hipEventRecord(c->ev[c->ev_used], s));
   }
   switch (c->tpw) {
-    case 1: launch_agents_u<1>(c, s, kind, mask, full_count); break;
-    case 2: launch_agents_u<2>(c, s, kind, mask, full_count); break;
-    case 8: launch_agents_u<8>(c, s, kind, mask, full_count); break;
-    default: launch_agents_u<4>(c, s, kind, mask, full_count); break;
+    case 1: launch_agents_u<1>(c, s, kind, mask); break;
+    case 2: launch_agents_u<2>(c, s, kind, mask); break;
+    case 4: launch_agents_u<4>(c, s, kind, mask); break;
+    case 16: launch_agents_u<16>(c, s, kind, mask); break;
+    default: launch_agents_u<8>(c, s, kind, mask); break;
   }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
-  hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, full_count);
+  hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
     for (int r = 1; r <= lp::R_GRID; ++r) {
@@ -324,8 +333,9 @@ int lp_prepare(lp_ctx* c) {
     for (int e = 0; e < g.n_sia_events; ++e)
       if (ss[e] < 0 || ss[e] >= g.n_strains) return fail(c, LP_ERR_ARG, "SIA vaccine strain out of range");
   }
-  int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 2;
-  if (tpw != 1 && tpw != 2 && tpw != 4 && tpw != 8) return fail(c, LP_ERR_ARG, "tiles_per_block must be 1, 2, 4 or 8");
+  int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 8;
+  if (tpw != 1 && tpw != 2 && tpw != 4 && tpw != 8 && tpw != 16)
+    return fail(c, LP_ERR_ARG, "tiles_per_block must be 1, 2, 4, 8 or 16");
   c->tpw = tpw;
   std::vector<int4> spans;
   std::vector<int32_t> nspan(N + 1, 0);
@@ -369,6 +379,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.node_tally, (size_t)N * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.potp_run, N))) return rc;
   if ((rc = alloc(c, &d.p_run, N))) return rc;
+  if ((rc = alloc(c, &d.node_carry, (size_t)N * 3))) return rc;
   if ((rc = alloc(c, &d.p_dbg, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.expected_dbg, N))) return rc;
@@ -404,16 +415,22 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   int rc;
   int32_t t = t_begin;
   if (t == 0) {
-    if ((rc = enqueue_day(c, lp::KIND_FULL, LP_STAGE_LOG, 1, 0, true))) return rc;
+    if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true))) return rc;
+    c->recount_pending = false;
     t = 1;
   }
   for (; t < t_end; ++t) {
+    if (c->recount_pending) {
+      if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true))) return rc;
+      c->recount_pending = false;
+      continue;
+    }
     const int kind = day_kind(c, t, LP_STAGE_ALL, 0);
     if (use_graph && !c->timing) {
       if (!c->graph_ready[kind]) {
         hipGraph_t graph;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_day(c, kind, LP_STAGE_ALL, 0, 1, true);
+        rc = enqueue_day(c, kind, LP_STAGE_ALL, 1, true);
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (rc) return rc;
         if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -423,7 +440,7 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       }
       HIPCHK(c, hipGraphLaunch(c->graph[kind], c->stream));
     } else {
-      if ((rc = enqueue_day(c, kind, LP_STAGE_ALL, 0, 1, true))) return rc;
+      if ((rc = enqueue_day(c, kind, LP_STAGE_ALL, 1, true))) return rc;
     }
   }
   return LP_OK;
@@ -437,9 +454,15 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t);
   const bool fused_tx_log = (mask & LP_STAGE_TRANSMISSION) && (mask & LP_STAGE_LOG);
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
-  const int full_count = log_only ? 1 : 0;
-  return enqueue_day(c, day_kind(c, t, mask, full_count), mask, full_count,
-                     fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+  const int recount = (log_only || c->recount_pending) ? 1 : 0;
+  c->recount_pending = false;
+  return enqueue_day(c, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+}
+
+int lp_invalidate(lp_ctx* c) {
+  if (!c) return LP_ERR_ARG;
+  c->recount_pending = true;
+  return LP_OK;
 }
 
 int lp_step_vital(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_VITAL); }
@@ -458,6 +481,7 @@ int lp_sync(lp_ctx* c) {
   if (e & lp::ERR_CAPACITY) return fail(c, LP_ERR_CAPACITY, "a node ran out of agent slots for births");
   if (e & lp::ERR_OVERFLOW) return fail(c, LP_ERR_OVERFLOW, "selected-agent list overflow");
   if (e & lp::ERR_SAMPLING) return fail(c, LP_ERR_STATE, "infection sampling found an inconsistent chunk prefix");
+  if (e & lp::ERR_KIND) return fail(c, LP_ERR_STATE, "agent pass launched with the wrong day variant");
   return LP_OK;
 }
 

@@ -50,7 +50,7 @@ enum : int {
   P_E0 = 15, P_I0 = 19, P_SIAPK0 = 23, P_SW = 27, P_IB0 = 28
 };
 
-enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4 };
+enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4, ERR_KIND = 8 };
 
 struct Dev {
   // agent columns
@@ -78,6 +78,7 @@ struct Dev {
   int64_t* partials; int64_t* chunk_sum;
   int64_t* nb_begin; int32_t* births;
   int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
+  int64_t* node_carry;   // [nodes][3] carried counts: S, S with positive weight, R
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -166,17 +167,21 @@ __global__ void k_day_begin(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-// The agent pass.  Persistent workgroups walk the span table (a span = up to U
-// tiles of 1024 slots of one node).  Thread tid owns slots [tile*1024 + 4*tid, +4)
-// of each tile, so each wave covers one 256-slot sampling chunk per tile.
-//   phase A  the day's streaming columns of all U tiles (state, risk; dod on VD
-//            step days; ri_timer + chronically_missed on RI days; dob on SIA days)
+// The agent pass.  One workgroup per span (up to U tiles of 1024 slots of one
+// node).  Thread tid owns slots [tile*1024 + 4*tid, +4) of each tile, so each
+// wave covers one 256-slot sampling chunk per tile.
+//   phase A  the state words of all U tiles, then the day's campaign / death
+//            columns (dod on VD step days; ri_timer + chronically_missed on RI
+//            days; dob on SIA days; risk only when recounting)
 //   phase B  births and deaths                          model.py:1719-1732, 1775-1777
 //   phase C  one vector load per E/I column for every quad holding an E or I agent
-//   phase D  quads without E/I on plain days: tally only (fast path); otherwise
-//            disease progression, RI, SIA and the tally from registers
-// Common tallies stay in registers, rare ones (E/I, deaths, paralysis,
-// vaccination) go through LDS atomics; one partial per span, no global atomics.
+//   phase D  disease progression, RI, SIA and the tallies
+// The susceptible statistics the transmission step needs (per node: #S, #S with
+// positive weight, #R; per span and per chunk: the fixed-point weight sum of
+// S agents) are exact integers carried from day to day.  Every change of an
+// agent into or out of S (birth, death, RI, SIA, transmission) or into or out
+// of R applies its exact delta, so only a RECOUNT pass reads every agent's risk.
+// Rare tallies go through LDS atomics; one partial per span, no global atomics.
 enum : int { L_E0 = 0, L_I0 = 4, L_SIAPK0 = 8, L_POTP = 12, L_P, L_DEATHS, L_DPP, L_DP, L_NEWPP, L_NEWP,
              L_RIV, L_RIP, L_IPV, L_SIAV, L_SIAP, L_N };
 
@@ -193,76 +198,307 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Day kinds of the agent pass: PLAIN days have no births, deaths, RI or SIA
-// (about 80% of days) and compile to a small kernel; FULL handles everything.
-enum : int { KIND_PLAIN = 0, KIND_FULL = 1 };
+// Day kinds of the agent pass:
+//   PLAIN    no births, deaths, RI or SIA (about 86% of days): k_plain reads the
+//            state column and the E/I columns of quads holding an E or I agent
+//   FULL     VD step / RI / SIA days: k_agents adds the campaign and death columns
+//   RECOUNT  the first pass and any pass after the host changed bound buffers:
+//            k_agents recounts every carried statistic from the agent columns
+enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 2 };
 
-__device__ __forceinline__ bool is_plain_day(int32_t flags, int full_count) {
-  return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) && !full_count;
+__device__ __forceinline__ bool is_plain_day(int32_t flags) {
+  return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA));
 }
 
-template <int U, int KIND>
-__global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) {
-  // independent loads first: span descriptor and day; then node/sim/day tables
+// Position of the j-th (0-based) set bit of m; m has more than j set bits.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int j) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int c = __popcll(m & ((1ull << w) - 1));
+    if (j >= c) { j -= c; m >>= w; pos += w; }
+  }
+  return pos;
+}
+
+// Disease progression of one E or I agent (disease_state_step_kernel,
+// model.py:419-452) on the quad's packed columns: a = agent in the quad, s its
+// state (1 or 2, updated), e2 / i2 / p2 the quad's exposure / infection /
+// paralysis timers, ppw its potentially_paralyzed bytes.  Returns true on I -> R.
+__device__ __forceinline__ bool progress_agent(int a, int8_t& s, int strain, uint2& e2, uint2& i2, uint2& p2,
+                                               uint32_t& ppw, uint32_t ipvw, uint64_t seed, uint64_t slot, int32_t t,
+                                               uint64_t thr_paralysis, int8_t* paralyzed_i, int32_t* lc_newpp,
+                                               int32_t* lc_newp) {
+  const int sh16 = 16 * (a & 1);
+  const uint32_t m16 = 0xffffu << sh16;
+  bool to_r = false;
+  if (s == 1) {
+    const uint32_t ew = a < 2 ? e2.x : e2.y;
+    const uint16_t et = (uint16_t)((ew >> sh16) & 0xffffu);
+    if (et == 0) s = 2;
+    else {
+      const uint32_t nw = (ew & ~m16) | ((uint32_t)(et - 1) << sh16);
+      if (a < 2) e2.x = nw; else e2.y = nw;
+    }
+  }
+  if (s == 2) {
+    const uint32_t iw = a < 2 ? i2.x : i2.y;
+    const uint16_t it = (uint16_t)((iw >> sh16) & 0xffffu);
+    if (it == 0) { s = 3; to_r = true; }
+    else {
+      const uint32_t nw = (iw & ~m16) | ((uint32_t)(it - 1) << sh16);
+      if (a < 2) i2.x = nw; else i2.y = nw;
+    }
+    if (strain == 0) {
+      const uint32_t pw = a < 2 ? p2.x : p2.y;
+      const uint16_t pt = (uint16_t)((pw >> sh16) & 0xffffu);
+      if (pt == 0 && byte_of(ppw, a) == -1) {
+        const uint32_t bm = 0xffu << (8 * a);
+        if (byte_of(ipvw, a) == 0) {
+          ppw = (ppw & ~bm) | (1u << (8 * a));
+          atomicAdd(lc_newpp, 1);
+          const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
+          if ((uint64_t)o.x < thr_paralysis) { *paralyzed_i = 1; atomicAdd(lc_newp, 1); }
+        } else {
+          ppw = ppw & ~bm;
+        }
+      }
+      if (pt > 0) {
+        const uint32_t nw = (pw & ~m16) | ((uint32_t)(pt - 1) << sh16);
+        if (a < 2) p2.x = nw; else p2.y = nw;
+      }
+    }
+  }
+  return to_r;
+}
+
+// Writes the span partial: carried-count deltas from red[][], tallies from LDS.
+__device__ __forceinline__ void write_partial(const Dev& d, int tid, const int64_t (*red)[4], const int32_t* lc,
+                                              const unsigned long long* lib, bool recount, bool sw_delta) {
+  if (tid >= NPART) return;
+  int64_t v = 0;
+  if (tid == P_S || tid == P_SPOS || tid == P_R || tid == P_SW) {
+    const int q = tid == P_S ? 0 : tid == P_SPOS ? 1 : tid == P_R ? 2 : 3;
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) v += red[w][q];
+  } else if (tid >= P_E0 && tid < P_E0 + 4) v = lc[L_E0 + tid - P_E0];
+  else if (tid >= P_I0 && tid < P_I0 + 4) v = lc[L_I0 + tid - P_I0];
+  else if (tid >= P_SIAPK0 && tid < P_SIAPK0 + 4) v = lc[L_SIAPK0 + tid - P_SIAPK0];
+  else if (tid >= P_IB0 && tid < P_IB0 + 4) v = (int64_t)lib[tid - P_IB0];
+  else {
+    switch (tid) {
+      case P_POTP: v = lc[L_POTP]; break;
+      case P_P: v = lc[L_P]; break;
+      case P_DEATHS: v = lc[L_DEATHS]; break;
+      case P_DPP: v = lc[L_DPP]; break;
+      case P_DP: v = lc[L_DP]; break;
+      case P_NEWPP: v = lc[L_NEWPP]; break;
+      case P_NEWP: v = lc[L_NEWP]; break;
+      case P_RIV: v = lc[L_RIV]; break;
+      case P_RIP: v = lc[L_RIP]; break;
+      case P_IPV: v = lc[L_IPV]; break;
+      case P_SIAV: v = lc[L_SIAV]; break;
+      case P_SIAP: v = lc[L_SIAP]; break;
+      default: break;
+    }
+  }
+  int64_t* slot = d.partials + (size_t)blockIdx.x * NPART + tid;
+  if (tid != P_SW) *slot = v;
+  else if (recount) *slot = v;
+  else if (sw_delta && v != 0) *slot += v;
+}
+
+// ---------------------------------------------------------------------------
+// PLAIN days.  The workgroup loads the state words of all U tiles of its span at
+// once, then every wave compacts its quads holding an E or I agent (ballot +
+// prefix of popcounts) so that each lane takes one such quad per round: one
+// round of E/I column loads serves up to 64 quads of any of the U tiles.
+// S statistics do not change on these days; I -> R adds to the R delta.
+template <int U>
+__global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
   const int4 span = d.spans[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ uint32_t s_st[U][WG];
+  __shared__ int32_t lc[L_N];
+  __shared__ unsigned long long lib[LP_MAX_STRAINS];
+  __shared__ int64_t red[WG / 64][4];
+  uint32_t st[U];
+  static_for<U>([&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+    st[u] = (u < span.z) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+  });
   const int32_t t = d.day[0];
   const int node = span.x, sim = span.w;
   const int64_t live_hi = d.live_end[node];
   const int32_t stop_day = d.stop_day[sim];
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  const bool plain = is_plain_day(flags, full_count);
-  if ((KIND == KIND_PLAIN) != plain || stop_day < t) return;
-  const bool do_deaths = KIND == KIND_FULL && (flags & LP_DAY_VD_STEP);
+  if (stop_day < t) return;
+  if (!is_plain_day(flags)) {
+    if (tid < NPART && tid != P_SW) d.partials[(size_t)blockIdx.x * NPART + tid] = 0;
+    if (tid == 0) atomicOr(d.err, ERR_KIND);
+    return;
+  }
   const bool do_ds = flags & LP_DAY_DS;
-  const bool do_ri = KIND == KIND_FULL && (flags & LP_DAY_RI);
-  const bool do_sia = KIND == KIND_FULL && (flags & LP_DAY_SIA);
   const bool do_tally = (flags & LP_DAY_TX) != 0;
   const bool need_ei = do_ds || do_tally;
-  const bool plain_day = KIND == KIND_PLAIN;
+  const lp_sim_params& sp = d.sims[sim];
+  if (tid < L_N) lc[tid] = 0;
+  if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+
+  // quads holding a live E or I agent, per tile, as wave masks
+  uint64_t m[U];
+  int n_ei = 0;
+  static_for<U>([&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+    bool q = false;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int8_t s = byte_of(st[u], a);
+      q = q || ((s == 1 || s == 2) && i0 + a < live_hi);
+    }
+    m[u] = __ballot(q && need_ei);
+    n_ei += __popcll(m[u]);
+    s_st[u][tid] = st[u];
+  });
+  __syncthreads();
+
+  int32_t c_r = 0;
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  for (int base = 0; base < n_ei; base += 64) {
+    int j = base + lane, su = -1, src = 0;
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int c = __popcll(m[u]);
+      if (su < 0) {
+        if (j < c) { su = u; src = nth_set_bit(m[u], j); }
+        else j -= c;
+      }
+    });
+    if (su < 0) continue;
+    const int qt = wave * 64 + src;
+    const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
+    const uint2 xe = *reinterpret_cast<const uint2*>(d.exp_timer + i0);
+    const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
+    const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
+    const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
+    const uint32_t xpp = reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2];
+    const uint32_t xipv = reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2];
+    const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t w0 = s_st[su][qt];
+    uint32_t w = w0;
+    uint2 e2 = xe, i2 = xi, p2 = xp;
+    uint32_t ppw = xpp;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = i0 + a;
+      if (i >= live_hi) continue;
+      int8_t s = byte_of(w, a);
+      if (s != 1 && s != 2) continue;
+      const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
+      if (do_ds && progress_agent(a, s, byte_of(xs, a), e2, i2, p2, ppw, xipv, seed, (uint64_t)(i - sbase), t,
+                                  sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
+        c_r += 1;
+      if (do_tally) {                                   // tx_step_prep / count_SEIRP tally of E and I
+        if (s == 1) {
+          atomicAdd(&lc[L_E0 + k], 1);
+        } else if (s == 2) {
+          atomicAdd(&lc[L_I0 + k], 1);
+          atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
+        }
+      }
+      w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+    }
+    if (w != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+    if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
+    if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
+    if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
+    if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+  }
+  {
+    const int32_t a2 = wave_sum_i32(c_r);
+    if (lane == 0) { red[wave][0] = 0; red[wave][1] = 0; red[wave][2] = a2; red[wave][3] = 0; }
+  }
+  __syncthreads();
+  write_partial(d, tid, red, lc, lib, false, false);
+}
+
+// ---------------------------------------------------------------------------
+// FULL and RECOUNT days: the span's tiles in groups of G, each group through
+// phases A-D; tallies accumulate over the groups, one partial per span.
+template <int G, int KIND>
+__global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
+  constexpr bool RECOUNT = KIND == KIND_RECOUNT;
+  const int4 span0 = d.spans[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t t = d.day[0];
+  const int node = span0.x, sim = span0.w;
+  const int64_t live_hi = d.live_end[node];
+  const int32_t stop_day = d.stop_day[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if (stop_day < t) return;
+  if (!RECOUNT && is_plain_day(flags)) {
+    if (tid < NPART && tid != P_SW) d.partials[(size_t)blockIdx.x * NPART + tid] = 0;
+    if (tid == 0) atomicOr(d.err, ERR_KIND);
+    return;
+  }
+  const bool do_deaths = flags & LP_DAY_VD_STEP;
+  const bool do_ds = flags & LP_DAY_DS;
+  const bool do_ri = flags & LP_DAY_RI;
+  const bool do_sia = flags & LP_DAY_SIA;
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const bool need_ei = do_ds || do_tally;
   const int32_t sia_b = do_sia ? d.day_sia_begin[t] : 0;
   const int32_t sia_n = do_sia ? d.day_sia_count[t] : 0;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   __shared__ int32_t lc[L_N];
   __shared__ unsigned long long lib[LP_MAX_STRAINS];
   __shared__ int64_t red[WG / 64][4];
 
-  {
-    const int sidx = blockIdx.x;
-    const lp_sim_params& sp = d.sims[sim];
-    const int64_t nb_lo = KIND == KIND_FULL ? d.nb_begin[node] : live_hi;
-    const uint64_t seed = sp.seed;
-    const int64_t sbase = sp.slot_base;
-    const uint64_t thr_ri = do_ri ? d.thr_ri[node] : 0, thr_ipv = do_ri ? d.thr_ipv[node] : 0;
-    if (tid < L_N) lc[tid] = 0;
-    if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+  const lp_sim_params& sp = d.sims[sim];
+  const int64_t nb_lo = d.nb_begin[node];
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  const uint64_t thr_ri = do_ri ? d.thr_ri[node] : 0, thr_ipv = do_ri ? d.thr_ipv[node] : 0;
+  if (tid < L_N) lc[tid] = 0;
+  if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+  __syncthreads();
 
+  // carried-statistic deltas (absolute counts when recounting)
+  int32_t c_s = 0, c_spos = 0, c_r = 0;
+  int64_t c_sw = 0;
+
+  for (int g0 = 0; g0 < span0.z; g0 += G) {
+    const int4 span = make_int4(node, span0.y + g0, span0.z - g0 < G ? span0.z - g0 : G, sim);
     // ---- phase A ----
-    uint32_t st[U], st0[U];
-    float4 rk[U];
-    int4 dd[U], db[U];
-    uint2 rt[U];
-    uint32_t mw[U];
-    bool tv[U];
-    static_for<U>([&](auto uc) {
+    uint32_t st[G], st0[G];
+    float4 rk[G];
+    int4 dd[G], db[G];
+    uint2 rt[G];
+    uint32_t mw[G];
+    bool tv[G];
+    static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
       const bool v = (u < span.z) && (i0 < live_hi);
       tv[u] = v;
-      st[u] = v ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+      st[u] = (u < span.z) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
       st0[u] = st[u];
-      rk[u] = (v && do_tally) ? *reinterpret_cast<const float4*>(d.risk + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (RECOUNT) rk[u] = v ? *reinterpret_cast<const float4*>(d.risk + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
       if (do_deaths) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
       if (do_ri || do_sia) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
       if (do_ri) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
       if (do_sia) db[u] = v ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
     });
-    __syncthreads();
+    int64_t cw[G];
+    static_for<G>([&](auto uc) { cw[decltype(uc)::value] = 0; });
 
-    // ---- phase B ----
-    const bool vd_work = KIND == KIND_FULL && (do_deaths || nb_lo < live_hi);
-    bool ei[U];
-    static_for<U>([&](auto uc) {
+    // ---- phase B: births and deaths ----
+    const bool vd_work = do_deaths || nb_lo < live_hi;
+    bool ei[G];
+    static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
       bool any = false;
@@ -287,17 +523,28 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
                 }
               }
               if (do_sia) { if (a == 0) db[u].x = t; else if (a == 1) db[u].y = t; else if (a == 2) db[u].z = t; else db[u].w = t; }
+              if (!RECOUNT) {                             // a newborn enters S
+                const int64_t w = weight_q24(d.risk[i]);
+                c_s += 1; c_spos += (w > 0); c_sw += w; cw[u] += w;
+              }
               s = 0;
             } else if (do_deaths && s >= 0 && lane_of(dd[u], a) <= t) {
               atomicAdd(&lc[L_DEATHS], 1);
               if (d.pp[i] == 1) atomicAdd(&lc[L_DPP], 1);
               if (d.paralyzed[i] == 1) atomicAdd(&lc[L_DP], 1);
+              if (!RECOUNT) {
+                if (s == 0) {                             // a susceptible leaves S
+                  const int64_t w = weight_q24(d.risk[i]);
+                  c_s -= 1; c_spos -= (w > 0); c_sw -= w; cw[u] -= w;
+                } else if (s == 3) {
+                  c_r -= 1;
+                }
+              }
               s = -1;
             }
             st[u] = (st[u] & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
           }
         }
-        // bytes equal to 1 or 2 (E or I): test all four lanes of the word at once
         const uint32_t w = st[u];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -308,11 +555,11 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
       ei[u] = any && need_ei;
     });
 
-    // ---- phase C ----
-    uint2 xe[U], xi[U], xp[U];
-    uint32_t xs[U], xpp[U], xipv[U];
-    float4 xin[U];
-    static_for<U>([&](auto uc) {
+    // ---- phase C: E/I columns of quads holding an E or I agent ----
+    uint2 xe[G], xi[G], xp[G];
+    uint32_t xs[G], xpp[G], xipv[G];
+    float4 xin[G];
+    static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
       if (ei[u]) {
@@ -330,34 +577,30 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
       }
     });
 
-    // ---- phase D ----
-    int32_t c_s = 0, c_spos = 0, c_r = 0;
-    int64_t c_sw = 0;
-    static_for<U>([&](auto uc) {
+    // ---- phase D: progression, RI, SIA, tallies ----
+    static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
-      int64_t chunk_w = 0;
-      if (tv[u] && plain_day && !ei[u]) {
-        // fast path: S and R agents only, no campaign today
-        if (do_tally) {
+      if (tv[u] && !ei[u] && !do_ri && !do_sia) {
+        // no E/I in the quad and no campaign today: only a recount looks at it
+        if (RECOUNT) {
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
             const int8_t s = byte_of(st[u], a);
             const bool live = i0 + a < live_hi;
             if (live && s == 0) {
               const int64_t wt = weight_q24(lane_of(rk[u], a));
-              c_s += 1;
-              c_spos += (wt > 0);
-              c_sw += wt;
-              chunk_w += wt;
+              c_s += 1; c_spos += (wt > 0); c_sw += wt; cw[u] += wt;
             }
             c_r += (live && s == 3);
+            if (live && s >= 0 && d.paralyzed[i0 + a] == 1) atomicAdd(&lc[L_P], 1);
+            if (live && s >= 0 && d.pp[i0 + a] == 1) atomicAdd(&lc[L_POTP], 1);
           }
         }
         if (vd_work && st[u] != st0[u]) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = st[u];
       } else if (tv[u]) {
         uint32_t w = st[u];
-        uint2 e2 = xe[u], i2 = xi[u], p2 = xp[u], r2 = rt[u];
+        uint2 e2 = xe[u], i2 = xi[u], p2 = xp[u], r2 = do_ri ? rt[u] : make_uint2(0u, 0u);
         uint32_t ppw = xpp[u];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -368,46 +611,13 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
           int k = byte_of(xs[u], a) & (LP_MAX_STRAINS - 1);
           const int sh16 = 16 * (a & 1);
           const uint32_t m16 = 0xffffu << sh16;
-          if (do_ds && (s == 1 || s == 2)) {              // disease_state_step_kernel (model.py:419-452)
-            if (s == 1) {
-              const uint32_t ew = a < 2 ? e2.x : e2.y;
-              const uint16_t et = (uint16_t)((ew >> sh16) & 0xffffu);
-              if (et == 0) s = 2;
-              else {
-                const uint32_t nw = (ew & ~m16) | ((uint32_t)(et - 1) << sh16);
-                if (a < 2) e2.x = nw; else e2.y = nw;
-              }
-            }
-            if (s == 2) {
-              const uint32_t iw = a < 2 ? i2.x : i2.y;
-              const uint16_t it = (uint16_t)((iw >> sh16) & 0xffffu);
-              if (it == 0) s = 3;
-              else {
-                const uint32_t nw = (iw & ~m16) | ((uint32_t)(it - 1) << sh16);
-                if (a < 2) i2.x = nw; else i2.y = nw;
-              }
-              if (byte_of(xs[u], a) == 0) {
-                const uint32_t pw = a < 2 ? p2.x : p2.y;
-                const uint16_t pt = (uint16_t)((pw >> sh16) & 0xffffu);
-                if (pt == 0 && byte_of(ppw, a) == -1) {
-                  const uint32_t bm = 0xffu << (8 * a);
-                  if (byte_of(xipv[u], a) == 0) {
-                    ppw = (ppw & ~bm) | (1u << (8 * a));
-                    atomicAdd(&lc[L_NEWPP], 1);
-                    const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
-                    if ((uint64_t)o.x < sp.thr_paralysis) { d.paralyzed[i] = 1; atomicAdd(&lc[L_NEWP], 1); }
-                  } else {
-                    ppw = ppw & ~bm;
-                  }
-                }
-                if (pt > 0) {
-                  const uint32_t nw = (pw & ~m16) | ((uint32_t)(pt - 1) << sh16);
-                  if (a < 2) p2.x = nw; else p2.y = nw;
-                }
-              }
-            }
+          if (do_ds && (s == 1 || s == 2)) {
+            if (progress_agent(a, s, byte_of(xs[u], a), e2, i2, p2, ppw, xipv[u], seed, slot, t, sp.thr_paralysis,
+                               d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]) && !RECOUNT)
+              c_r += 1;                                   // I -> R
           }
-          if (KIND == KIND_FULL && (do_ri || do_sia) && s >= 0 && byte_of(mw[u], a) != 1) {
+          if ((do_ri || do_sia) && s >= 0 && byte_of(mw[u], a) != 1) {
+            const int8_t s_before = s;
             if (do_ri) {                                  // fast_ri (model.py:1826-1854)
               const int32_t step = sp.ri_step;
               const uint32_t rw = a < 2 ? r2.x : r2.y;
@@ -447,26 +657,30 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
                 }
               }
             }
-          }
-          if (do_tally && s >= 0) {                       // tx_step_prep / count_SEIRP tally
-            if (s == 0) {
-              const int64_t wt = weight_q24(lane_of(rk[u], a));
-              c_s += 1;
-              c_spos += (wt > 0);
-              c_sw += wt;
-              chunk_w += wt;
-            } else if (s == 3) {
-              c_r += 1;
-            } else if (s == 1) {
-              atomicAdd(&lc[L_E0 + k], 1);
-            } else if (s == 2) {
-              atomicAdd(&lc[L_I0 + k], 1);
-              const int64_t wv = weight_q24(lane_of(xin[u], a) * sp.strain_r0[k]);
-              atomicAdd(&lib[k], (unsigned long long)wv);
+            if (!RECOUNT && s_before == 0 && s != 0) {    // vaccinated out of S
+              const int64_t wt = weight_q24(d.risk[i]);
+              c_s -= 1; c_spos -= (wt > 0); c_sw -= wt; cw[u] -= wt;
             }
-            if (KIND == KIND_FULL && full_count) {
+          }
+          if (s >= 0) {
+            if (RECOUNT) {
+              if (s == 0) {
+                const int64_t wt = weight_q24(lane_of(rk[u], a));
+                c_s += 1; c_spos += (wt > 0); c_sw += wt; cw[u] += wt;
+              } else if (s == 3) {
+                c_r += 1;
+              }
               if (d.pp[i] == 1 || byte_of(ppw, a) == 1) atomicAdd(&lc[L_POTP], 1);
               if (d.paralyzed[i] == 1) atomicAdd(&lc[L_P], 1);
+            }
+            if (do_tally) {                               // tx_step_prep / count_SEIRP tally of E and I
+              if (s == 1) {
+                atomicAdd(&lc[L_E0 + k], 1);
+              } else if (s == 2) {
+                atomicAdd(&lc[L_I0 + k], 1);
+                const int64_t wv = weight_q24(lane_of(xin[u], a) * sp.strain_r0[k]);
+                atomicAdd(&lib[k], (unsigned long long)wv);
+              }
             }
           }
           w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
@@ -480,50 +694,27 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask, int full_count) 
         }
         if (do_ri && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
       }
-      if (do_tally && u < span.z && ((int64_t)span.y + u) * LP_TILE < live_hi) {
-        const int64_t cw = wave_sum_i64(chunk_w);
-        if (lane == 0) d.chunk_sum[(((int64_t)span.y + u) * LP_TILE >> 8) + wave] = cw;
-      }
-    });
-
-    // ---- span partial ----
-    {
-      const int32_t a0 = wave_sum_i32(c_s), a1 = wave_sum_i32(c_spos), a2 = wave_sum_i32(c_r);
-      const int64_t a3 = wave_sum_i64(c_sw);
-      if (lane == 0) { red[wave][0] = a0; red[wave][1] = a1; red[wave][2] = a2; red[wave][3] = a3; }
-    }
-    __syncthreads();
-    if (tid < NPART) {
-      int64_t v = 0;
-      if (tid == P_S || tid == P_SPOS || tid == P_R || tid == P_SW) {
-        const int q = tid == P_S ? 0 : tid == P_SPOS ? 1 : tid == P_R ? 2 : 3;
-#pragma unroll
-        for (int w = 0; w < WG / 64; ++w) v += red[w][q];
-      } else if (tid >= P_E0 && tid < P_E0 + 4) v = lc[L_E0 + tid - P_E0];
-      else if (tid >= P_I0 && tid < P_I0 + 4) v = lc[L_I0 + tid - P_I0];
-      else if (tid >= P_SIAPK0 && tid < P_SIAPK0 + 4) v = lc[L_SIAPK0 + tid - P_SIAPK0];
-      else if (tid >= P_IB0 && tid < P_IB0 + 4) v = (int64_t)lib[tid - P_IB0];
-      else {
-        switch (tid) {
-          case P_POTP: v = lc[L_POTP]; break;
-          case P_P: v = lc[L_P]; break;
-          case P_DEATHS: v = lc[L_DEATHS]; break;
-          case P_DPP: v = lc[L_DPP]; break;
-          case P_DP: v = lc[L_DP]; break;
-          case P_NEWPP: v = lc[L_NEWPP]; break;
-          case P_NEWP: v = lc[L_NEWP]; break;
-          case P_RIV: v = lc[L_RIV]; break;
-          case P_RIP: v = lc[L_RIP]; break;
-          case P_IPV: v = lc[L_IPV]; break;
-          case P_SIAV: v = lc[L_SIAV]; break;
-          case P_SIAP: v = lc[L_SIAP]; break;
-          default: break;
+      // chunk weight sums: rewritten when recounting, adjusted by their delta otherwise
+      // (the condition is uniform over the workgroup: wave_sum needs every lane)
+      if (u < span.z && (RECOUNT || ((int64_t)span.y + u) * LP_TILE < live_hi)) {
+        const int64_t cs = wave_sum_i64(cw[u]);
+        const int64_t c = (((int64_t)span.y + u) * LP_TILE >> 8) + wave;
+        if (lane == 0) {
+          if (RECOUNT) d.chunk_sum[c] = cs;
+          else if (cs != 0) d.chunk_sum[c] += cs;
         }
       }
-      d.partials[(size_t)sidx * NPART + tid] = v;
-    }
-    __syncthreads();
+    });
   }
+
+  // ---- span partial ----
+  {
+    const int32_t a0 = wave_sum_i32(c_s), a1 = wave_sum_i32(c_spos), a2 = wave_sum_i32(c_r);
+    const int64_t a3 = wave_sum_i64(c_sw);
+    if (lane == 0) { red[wave][0] = a0; red[wave][1] = a1; red[wave][2] = a2; red[wave][3] = a3; }
+  }
+  __syncthreads();
+  write_partial(d, tid, red, lc, lib, RECOUNT, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -560,7 +751,7 @@ __device__ int64_t node_span_scan(int64_t* partials, int64_t* prefix, int s0, in
 }
 
 // Per node: reduce span partials, span prefix, results rows of VD / DS / RI / SIA.
-__global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_count) {
+__global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x;
   const int sim = d.node_sim[n];
@@ -591,9 +782,18 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_co
 #pragma unroll
     for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
     tally[tid] = acc;
-    d.node_tally[(size_t)n * NPART + tid] = acc;
   }
   __syncthreads();
+  if (tid < 3) {
+    // carried per-node counts: S, S with positive weight, R (the spans hold deltas)
+    const int f = tid == 0 ? P_S : tid == 1 ? P_SPOS : P_R;
+    int64_t* cv = d.node_carry + (size_t)n * 3 + tid;
+    const int64_t v = recount ? tally[f] : *cv + tally[f];
+    *cv = v;
+    tally[f] = v;
+  }
+  __syncthreads();
+  if (tid < NPART) d.node_tally[(size_t)n * NPART + tid] = tally[tid];
   if (flags & LP_DAY_TX) {
     const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
     if (tid == 0) d.W_round[n] = W;
@@ -632,7 +832,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int full_co
     }
     d.r_newexp[row + n] += tot;
   }
-  if (full_count) {
+  if (recount) {
     d.potp_run[n] = (int32_t)tally[P_POTP];
     d.p_run[n] = (int32_t)tally[P_P];
   } else {
@@ -1013,6 +1213,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       if (sel && sh_base >= 0) {
         const int pos = woff + __popcll(bal & ((1ull << lane) - 1));
         d.state[i] = SEL_MARK;
+        unweight(d, n, i);
         d.sel_list[sh_base + pos] = (uint64_t)i | ((uint64_t)n << 40);
       }
       __syncthreads();
@@ -1107,6 +1308,9 @@ __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_ad
   if ((mask & LP_STAGE_TRANSMISSION) && tx_adjust) {
     d.r_newexp[row] += ne_tot;
     for (int k = 0; k < K; ++k) d.r_newexp_bs[row * K + k] += ne[k];
+    // every agent exposed by transmission left S and had a positive weight
+    d.node_carry[(size_t)n * 3 + 0] -= ne_tot;
+    d.node_carry[(size_t)n * 3 + 1] -= ne_tot;
   }
   if (mask & LP_STAGE_LOG) {
     int32_t e = 0, in = 0;

@@ -104,6 +104,7 @@ def load_library(path=LIB_PATH):
     for f in ("lp_step_vital", "lp_step_disease", "lp_step_ri", "lp_step_sia", "lp_step_transmission", "lp_log"):
         getattr(lib, f).argtypes = [vp, i32]
     lib.lp_sync.argtypes = [vp]
+    lib.lp_invalidate.argtypes = [vp]
     lib.lp_stop_days.argtypes = [vp, ctypes.POINTER(i32), i32]
     lib.lp_debug_buffer.argtypes = [vp, ctypes.c_char_p]
     lib.lp_debug_buffer.restype = vp
@@ -184,6 +185,7 @@ class Engine:
         self.sync()
         self.tensors[name].copy_(self.torch.from_numpy(a.reshape(-1)))
         self.torch.cuda.synchronize()
+        self._check(self.lib.lp_invalidate(self.ctx))
 
     def pull(self, name, out):
         """Device -> host copy into `out` (numpy, same size and dtype as bound)."""
