@@ -185,7 +185,7 @@ def main():
     if launches:
         bytes_alg = agent_pass_bytes(res, day_flags, t_begin, t_end)
         achieved = bytes_alg / (kernel_ms / 1e3) / 1e9
-        tpw = int(sim.pars.tiles_per_block or 8)
+        tpw = int(sim.pars.tiles_per_block or 16)
         tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
         roofline = {
             "bound": "hbm",

@@ -101,8 +101,8 @@ struct lp_ctx {
   int32_t tpw = 4;
   int32_t agent_grid = 1;
   std::vector<void*> allocs;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};   // plain day, full day
-  bool graph_ready[2] = {false, false};
+  hipGraphExec_t graph[8] = {};                    // one per day kind: plain, FULL_* combinations
+  bool graph_ready[8] = {};
   bool recount_pending = true;                     // next agent pass recounts carried statistics
   std::vector<int32_t> day_flags_h;
   std::string err;
@@ -171,22 +171,30 @@ int32_t host_stage_flags(int32_t flags, int mask) {
   return f;
 }
 
+// Day kind: KIND_RECOUNT, KIND_PLAIN, or the OR of FULL_VD / FULL_RI / FULL_SIA.
 int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
   if (recount) return lp::KIND_RECOUNT;
   const int32_t f = host_stage_flags(c->day_flags_h[t], mask);
-  return !(f & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA)) ? lp::KIND_PLAIN : lp::KIND_FULL;
+  return ((f & LP_DAY_VD_STEP) ? lp::FULL_VD : 0) | ((f & LP_DAY_RI) ? lp::FULL_RI : 0) |
+         ((f & LP_DAY_SIA) ? lp::FULL_SIA : 0);
 }
 
 // Agent pass of one day: k_plain over U = tiles per span on plain days,
-// k_agents in groups of 2 tiles otherwise.
+// k_full<kind> on FULL days, k_agents in groups of 2 tiles when recounting.
 template <int U>
 void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
-  if (kind == lp::KIND_PLAIN)
-    hipLaunchKernelGGL((lp::k_plain<U>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
-  else if (kind == lp::KIND_FULL)
-    hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_FULL>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
-  else
-    hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), dim3(c->n_spans), dim3(lp::WG), 0, s, c->d, mask);
+  const dim3 g(c->n_spans), b(lp::WG);
+  switch (kind) {
+    case lp::KIND_PLAIN: hipLaunchKernelGGL((lp::k_plain<U>), g, b, 0, s, c->d, mask); break;
+    case 1: hipLaunchKernelGGL((lp::k_full<1>), g, b, 0, s, c->d, mask); break;
+    case 2: hipLaunchKernelGGL((lp::k_full<2>), g, b, 0, s, c->d, mask); break;
+    case 3: hipLaunchKernelGGL((lp::k_full<3>), g, b, 0, s, c->d, mask); break;
+    case 4: hipLaunchKernelGGL((lp::k_full<4>), g, b, 0, s, c->d, mask); break;
+    case 5: hipLaunchKernelGGL((lp::k_full<5>), g, b, 0, s, c->d, mask); break;
+    case 6: hipLaunchKernelGGL((lp::k_full<6>), g, b, 0, s, c->d, mask); break;
+    case 7: hipLaunchKernelGGL((lp::k_full<7>), g, b, 0, s, c->d, mask); break;
+    default: hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), g, b, 0, s, c->d, mask); break;
+  }
 }
 
 // The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
@@ -213,8 +221,8 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
     case 1: launch_agents_u<1>(c, s, kind, mask); break;
     case 2: launch_agents_u<2>(c, s, kind, mask); break;
     case 4: launch_agents_u<4>(c, s, kind, mask); break;
-    case 16: launch_agents_u<16>(c, s, kind, mask); break;
-    default: launch_agents_u<8>(c, s, kind, mask); break;
+    case 8: launch_agents_u<8>(c, s, kind, mask); break;
+    default: launch_agents_u<16>(c, s, kind, mask); break;
   }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount);
@@ -284,7 +292,8 @@ int lp_bind(lp_ctx* c, const char* name, void* ptr, int32_t dtype, int64_t n) {
 
 int lp_set_stream(lp_ctx* c, void* s) {
   if (!c) return LP_ERR_ARG;
-  if (c->graph_ready[0] || c->graph_ready[1]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
+  for (int k = 0; k < 8; ++k)
+    if (c->graph_ready[k]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
   c->stream = reinterpret_cast<hipStream_t>(s);
   return LP_OK;
 }
@@ -333,7 +342,7 @@ int lp_prepare(lp_ctx* c) {
     for (int e = 0; e < g.n_sia_events; ++e)
       if (ss[e] < 0 || ss[e] >= g.n_strains) return fail(c, LP_ERR_ARG, "SIA vaccine strain out of range");
   }
-  int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 8;
+  int tpw = g.tiles_per_block > 0 ? g.tiles_per_block : 16;
   if (tpw != 1 && tpw != 2 && tpw != 4 && tpw != 8 && tpw != 16)
     return fail(c, LP_ERR_ARG, "tiles_per_block must be 1, 2, 4, 8 or 16");
   c->tpw = tpw;
@@ -531,7 +540,7 @@ const char* lp_last_error(lp_ctx* c) { return c ? c->err.c_str() : "null context
 void lp_destroy(lp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 2; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+  for (int k = 0; k < 8; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

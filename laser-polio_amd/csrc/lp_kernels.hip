@@ -204,7 +204,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 //   FULL     VD step / RI / SIA days: k_agents adds the campaign and death columns
 //   RECOUNT  the first pass and any pass after the host changed bound buffers:
 //            k_agents recounts every carried statistic from the agent columns
-enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 2 };
+enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 8 };   // FULL days use kinds 1..7 (FULL_* bits)
 
 __device__ __forceinline__ bool is_plain_day(int32_t flags) {
   return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA));
@@ -426,8 +426,267 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-// FULL and RECOUNT days: the span's tiles in groups of G, each group through
-// phases A-D; tallies accumulate over the groups, one partial per span.
+// FULL days (VD step, RI and / or SIA; F = OR of the FULL_* bits).  The span's
+// tiles go through in groups of G: one round of dense loads (state and the
+// day's campaign / death columns), births, deaths, RI and SIA per agent in
+// registers, then the group's quads holding an agent that was E or I after
+// births and deaths are compacted per wave for disease progression and the
+// E / I tally, as on plain days.  Agents exposed by RI or SIA today are
+// tallied as E directly and are not progressed (DiseaseState runs before RI and
+// SIA in the reference day, model.py:226-241).
+enum : int { FULL_VD = 1, FULL_RI = 2, FULL_SIA = 4 };
+
+template <int F>
+__global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
+  constexpr int G = 4;
+  constexpr bool VD = F & FULL_VD, RI = F & FULL_RI, SIA = F & FULL_SIA;
+  const int4 span0 = d.spans[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t t = d.day[0];
+  const int node = span0.x, sim = span0.w;
+  const int64_t live_hi = d.live_end[node];
+  const int32_t stop_day = d.stop_day[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if (stop_day < t) return;
+  const int want = ((flags & LP_DAY_VD_STEP) ? FULL_VD : 0) | ((flags & LP_DAY_RI) ? FULL_RI : 0) |
+                   ((flags & LP_DAY_SIA) ? FULL_SIA : 0);
+  if (want != F) {
+    if (tid < NPART && tid != P_SW) d.partials[(size_t)blockIdx.x * NPART + tid] = 0;
+    if (tid == 0) atomicOr(d.err, ERR_KIND);
+    return;
+  }
+  const bool do_ds = flags & LP_DAY_DS;
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const bool need_ei = do_ds || do_tally;
+  const int32_t sia_b = SIA ? d.day_sia_begin[t] : 0;
+  const int32_t sia_n = SIA ? d.day_sia_count[t] : 0;
+
+  __shared__ uint32_t s_w0[G][WG], s_w1[G][WG], s_w2[G][WG];   // state words: original, after VD, after RI / SIA
+  __shared__ int32_t lc[L_N];
+  __shared__ unsigned long long lib[LP_MAX_STRAINS];
+  __shared__ int64_t red[WG / 64][4];
+
+  const lp_sim_params& sp = d.sims[sim];
+  const int64_t nb_lo = VD ? d.nb_begin[node] : live_hi;
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  const uint64_t thr_ri = RI ? d.thr_ri[node] : 0, thr_ipv = RI ? d.thr_ipv[node] : 0;
+  // does any of today's campaigns target this node?  (uniform over the workgroup)
+  bool sia_here = false;
+  if (SIA)
+    for (int e = 0; e < sia_n; ++e) sia_here = sia_here || d.sia_thr[((size_t)(sia_b + e) * d.n_nodes + node) * 2] != 0;
+  const bool camp = RI || sia_here;
+  if (tid < L_N) lc[tid] = 0;
+  if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+  __syncthreads();
+
+  int32_t c_s = 0, c_spos = 0, c_r = 0;
+  int64_t c_sw = 0;
+
+  for (int g0 = 0; g0 < span0.z; g0 += G) {
+    const int tile0 = span0.y + g0;
+    const int ng = span0.z - g0 < G ? span0.z - g0 : G;
+    // ---- dense loads ----
+    uint32_t st[G], mw[G];
+    int4 dd[G], db[G];
+    uint2 rt[G];
+    static_for<G>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
+      const bool v = (u < ng) && (i0 < live_hi);
+      st[u] = (u < ng) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+      if (VD) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+      if (camp) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
+      if (RI) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
+      if (SIA) db[u] = (v && sia_here) ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
+    });
+
+    // ---- births, deaths, RI, SIA per agent ----
+    uint64_t m[G];
+    int n_ei = 0;
+    static_for<G>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
+      const uint32_t w0 = st[u];
+      uint32_t w1 = w0, w2 = w0;
+      uint2 r2 = RI ? rt[u] : make_uint2(0u, 0u);
+      int64_t cw = 0;
+      bool q = false;
+      if (u < ng && i0 < live_hi) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int64_t i = i0 + a;
+          if (i >= live_hi) continue;
+          int8_t s = byte_of(w0, a);
+          const uint64_t slot = (uint64_t)(i - sbase);
+          const int sh16 = 16 * (a & 1);
+          const uint32_t m16 = 0xffffu << sh16;
+          if (VD) {
+            if (i >= nb_lo) {                             // newborn (model.py:1719-1732)
+              d.dob[i] = t;
+              d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
+              d.node_id[i] = (int16_t)node;
+              if (sp.newborn_ri_timer >= 0) {
+                d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
+                if (RI) {
+                  const uint32_t hv = (uint32_t)(uint16_t)sp.newborn_ri_timer << sh16;
+                  if (a < 2) r2.x = (r2.x & ~m16) | hv; else r2.y = (r2.y & ~m16) | hv;
+                }
+              }
+              if (SIA) { if (a == 0) db[u].x = t; else if (a == 1) db[u].y = t; else if (a == 2) db[u].z = t; else db[u].w = t; }
+              const int64_t w = weight_q24(d.risk[i]);     // a newborn enters S
+              c_s += 1; c_spos += (w > 0); c_sw += w; cw += w;
+              s = 0;
+            } else if (s >= 0 && lane_of(dd[u], a) <= t) { // get_deaths (model.py:1775-1777)
+              atomicAdd(&lc[L_DEATHS], 1);
+              if (d.pp[i] == 1) atomicAdd(&lc[L_DPP], 1);
+              if (d.paralyzed[i] == 1) atomicAdd(&lc[L_DP], 1);
+              if (s == 0) {
+                const int64_t w = weight_q24(d.risk[i]);
+                c_s -= 1; c_spos -= (w > 0); c_sw -= w; cw -= w;
+              } else if (s == 3) {
+                c_r -= 1;
+              }
+              s = -1;
+            }
+            w1 = (w1 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+          }
+          q = q || s == 1 || s == 2;
+          if (camp && s >= 0 && byte_of(mw[u], a) != 1) {
+            const int8_t s_before = s;
+            int k = 0;
+            if (RI) {                                     // fast_ri (model.py:1826-1854)
+              const int32_t step = sp.ri_step;
+              const uint32_t rw = a < 2 ? r2.x : r2.y;
+              const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - step);
+              const uint32_t nw = (rw & ~m16) | ((uint32_t)(uint16_t)timer << sh16);
+              if (a < 2) r2.x = nw; else r2.y = nw;
+              bool eligible = false;
+              if (t == step) eligible = timer <= 0 && timer >= -step;
+              else if (t > step) eligible = timer <= 0 && timer > -step;
+              if (eligible) {
+                const uint4 o = draw(seed, slot, t, ST_RI);
+                if ((uint64_t)o.x < thr_ri) {
+                  atomicAdd(&lc[L_RIV], 1);
+                  if (s == 0) { s = 1; k = sp.ri_strain; d.strain[i] = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
+                }
+                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); d.ipv[i] = 1; }
+              }
+            }
+            if (SIA && sia_here) {                        // fast_sia (model.py:2033-2059), per event
+              const int32_t age = t - lane_of(db[u], a);
+              for (int e = 0; e < sia_n; ++e) {
+                const int ev = sia_b + e;
+                if (!(d.sia_min_age[ev] <= age && age <= d.sia_max_age[ev])) continue;
+                const uint64_t t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
+                if (t1 == 0) continue;
+                const uint4 o = draw(seed, slot, t, ST_SIA | ((uint32_t)e << 8));
+                if ((uint64_t)o.x < t1) {
+                  const bool last = (e == sia_n - 1);
+                  if (last) atomicAdd(&lc[L_SIAV], 1);
+                  if (s == 0 && (uint64_t)o.x < d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1]) {
+                    s = 1;
+                    k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
+                    d.strain[i] = (int8_t)k;
+                    atomicAdd(&lc[L_SIAPK0 + k], 1);
+                    if (last) atomicAdd(&lc[L_SIAP], 1);
+                  }
+                }
+              }
+            }
+            if (s_before == 0 && s != 0) {                // vaccinated out of S, counted as E today
+              const int64_t wt = weight_q24(d.risk[i]);
+              c_s -= 1; c_spos -= (wt > 0); c_sw -= wt; cw -= wt;
+              if (do_tally) atomicAdd(&lc[L_E0 + k], 1);
+            }
+          }
+          w2 = (w2 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+        }
+      }
+      if (!VD) w1 = w0;
+      m[u] = __ballot(q && need_ei);
+      n_ei += __popcll(m[u]);
+      const bool in_ei = (m[u] >> lane) & 1ull;
+      if (u < ng) {
+        if (!in_ei && w2 != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w2;
+        if (RI && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
+      }
+      s_w0[u][tid] = w0; s_w1[u][tid] = w1; s_w2[u][tid] = w2;
+      // chunk weight delta (uniform condition: wave_sum needs every lane)
+      if (u < ng && ((int64_t)tile0 + u) * LP_TILE < live_hi) {
+        const int64_t cs = wave_sum_i64(cw);
+        if (lane == 0 && cs != 0) d.chunk_sum[(((int64_t)tile0 + u) * LP_TILE >> 8) + wave] += cs;
+      }
+    });
+    __syncthreads();
+
+    // ---- disease progression and E / I tally of the compacted quads ----
+    for (int base = 0; base < n_ei; base += 64) {
+      int j = base + lane, su = -1, src = 0;
+      static_for<G>([&](auto uc) {
+        constexpr int u = decltype(uc)::value;
+        const int c = __popcll(m[u]);
+        if (su < 0) {
+          if (j < c) { su = u; src = nth_set_bit(m[u], j); }
+          else j -= c;
+        }
+      });
+      if (su < 0) continue;
+      const int qt = wave * 64 + src;
+      const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
+      const uint2 xe = *reinterpret_cast<const uint2*>(d.exp_timer + i0);
+      const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
+      const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
+      const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
+      const uint32_t xpp = reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2];
+      const uint32_t xipv = reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2];
+      const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint32_t w0 = s_w0[su][qt], w1 = s_w1[su][qt];
+      uint32_t w = s_w2[su][qt];
+      uint2 e2 = xe, i2 = xi, p2 = xp;
+      uint32_t ppw = xpp;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int64_t i = i0 + a;
+        if (i >= live_hi) continue;
+        int8_t s = byte_of(w1, a);
+        if (s != 1 && s != 2) continue;
+        const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
+        if (do_ds && progress_agent(a, s, byte_of(xs, a), e2, i2, p2, ppw, xipv, seed, (uint64_t)(i - sbase), t,
+                                    sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
+          c_r += 1;
+        if (do_tally) {
+          if (s == 1) {
+            atomicAdd(&lc[L_E0 + k], 1);
+          } else if (s == 2) {
+            atomicAdd(&lc[L_I0 + k], 1);
+            atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
+          }
+        }
+        w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+      }
+      if (w != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+      if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
+      if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
+      if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
+      if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+    }
+    __syncthreads();
+  }
+
+  {
+    const int32_t a0 = wave_sum_i32(c_s), a1 = wave_sum_i32(c_spos), a2 = wave_sum_i32(c_r);
+    const int64_t a3 = wave_sum_i64(c_sw);
+    if (lane == 0) { red[wave][0] = a0; red[wave][1] = a1; red[wave][2] = a2; red[wave][3] = a3; }
+  }
+  __syncthreads();
+  write_partial(d, tid, red, lc, lib, false, true);
+}
+
+// ---------------------------------------------------------------------------
+// RECOUNT passes: the span's tiles in groups of G, each group through phases
+// A-D (every agent's risk, paralysis flags and state recounted); tallies
+// accumulate over the groups, one partial per span.
 template <int G, int KIND>
 __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
   constexpr bool RECOUNT = KIND == KIND_RECOUNT;
