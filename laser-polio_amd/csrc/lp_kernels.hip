@@ -210,6 +210,14 @@ __device__ __forceinline__ bool is_plain_day(int32_t flags) {
   return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA));
 }
 
+// True if any byte of the packed state word is 1 (E) or 2 (I): the classic
+// zero-byte test (v - 0x01..) & ~v & 0x80.. on w ^ 0x01.. and w ^ 0x02.. (exact
+// for "any byte": a borrow only propagates out of a byte that is itself zero).
+__device__ __forceinline__ bool has_ei_byte(uint32_t w) {
+  const uint32_t x1 = w ^ 0x01010101u, x2 = w ^ 0x02020202u;
+  return ((((x1 - 0x01010101u) & ~x1) | ((x2 - 0x01010101u) & ~x2)) & 0x80808080u) != 0;
+}
+
 // Position of the j-th (0-based) set bit of m; m has more than j set bits.
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int j) {
   int pos = 0;
@@ -346,18 +354,13 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
   if (tid < L_N) lc[tid] = 0;
   if (tid < LP_MAX_STRAINS) lib[tid] = 0;
 
-  // quads holding a live E or I agent, per tile, as wave masks
+  // quads holding an E or I agent, per tile, as wave masks (slots past the
+  // node's live range hold -1, so no range test is needed)
   uint64_t m[U];
   int n_ei = 0;
   static_for<U>([&](auto uc) {
     constexpr int u = decltype(uc)::value;
-    const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
-    bool q = false;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int8_t s = byte_of(st[u], a);
-      q = q || ((s == 1 || s == 2) && i0 + a < live_hi);
-    }
+    const bool q = has_ei_byte(st[u]);
     m[u] = __ballot(q && need_ei);
     n_ei += __popcll(m[u]);
     s_st[u][tid] = st[u];
