@@ -228,14 +228,11 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
-    for (int r = 1; r <= lp::R_GRID; ++r) {
-      if (r > 1) hipLaunchKernelGGL(lp::k_prefix, dim3(N), dim3(lp::WG), 0, s, d);
-      hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, r);
-      hipLaunchKernelGGL(lp::k_resolve, dim3(2048), dim3(lp::WG), 0, s, d, r);
-      hipLaunchKernelGGL(lp::k_mark, dim3(512), dim3(lp::WG), 0, s, d, r);
-    }
+    hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
+    const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
+    hipLaunchKernelGGL(lp::k_sample, dim3(2048), dim3(lp::WG), off_lds, s, d);
+    hipLaunchKernelGGL(lp::k_commit, dim3(512), dim3(lp::WG), 0, s, d);
     hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);
-    hipLaunchKernelGGL(lp::k_expose, dim3(512), dim3(lp::WG), 0, s, d);
   }
   hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust, advance ? 1 : 0);
   HIPCHK(c, hipGetLastError());

@@ -4,29 +4,30 @@
 // VitalDynamics -> DiseaseState -> RI -> SIA -> Transmission -> log):
 //   k_day_begin    per node: births (count + newborn slot range), per-day resets,
 //                  early-stop rule                     model.py:1711-1719, 788-795
-//   k_agents       ONE streaming pass over every live agent slot, node-segmented:
-//                  deaths, newborn init, E->I->R timers + paralysis, RI, SIA, and
-//                  the tally the transmission step and the log need; one partial
-//                  per span (no global atomics) and per-chunk (256 agents)
-//                  susceptible weight sums      model.py:1767-1781, 393-454,
-//                                               1784-1855, 1994-2060, 980-1007, 902-929
-//                  (a PLAIN variant for days without births/deaths/RI/SIA)
-//   k_node_reduce  per node: reduce span partials, exclusive prefix of the span
-//                  weights, results rows of VD/DS/RI/SIA
+//   agent pass     ONE pass over the agent slots, node-segmented, one workgroup
+//                  per span of tiles, one partial per span (no global atomics):
+//                  k_plain on days without births/deaths/RI/SIA (state column +
+//                  E/I columns of the quads holding E/I), k_full<VD|RI|SIA> on
+//                  the other days (adds the death / campaign columns), k_agents
+//                  when every carried statistic is recounted
+//                                     model.py:1767-1781, 393-454, 1784-1855,
+//                                              1994-2060, 980-1007, 902-929
+//   k_node_reduce  per node: reduce span partials, carried S / R counts, exclusive
+//                  prefix of the span weights, results rows of VD/DS/RI/SIA
 //   k_mix_foi      per node: network mix (fixed-order tree), seasonality, r0 scalars,
 //                  rate -> probability, expected exposures, Poisson / zero-inflated NB
 //                  count draw (chunks in parallel across the wave)  model.py:1327-1407
-//   k_plan/k_resolve/k_mark (round 1), k_prefix/k_plan/k_resolve/k_mark (round 2),
-//   k_finish
+//   k_plan, k_sample, k_commit, k_finish
 //                  weighted sampling without replacement by rounds of with-
 //                  replacement probes + unique (model.py:1089-1122): every probe
 //                  searches its node's span prefix, then a span's chunks, then one
-//                  chunk, one wavefront per probe; k_finish takes the remaining
-//                  rounds and the nodes where every susceptible is exposed
-//   k_expose       per selected agent: strain draw, S -> E       model.py:1125-1147
+//                  chunk, one wavefront per probe; round 1 grid-wide (k_commit
+//                  exposes its agents), later rounds and the take-all nodes per
+//                  node in k_finish; each exposed agent's strain is drawn as it
+//                  is exposed (S -> E)                          model.py:1125-1147
 //   k_day_end      per node: log rows + transmission incidence; its last
 //                  workgroup advances the day counter      model.py:1436-1441, 1461-1483
-// A day is replayed as one hipGraph (one graph per agent-pass variant).
+// A day is replayed as one hipGraph (one graph per day kind).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <limits.h>
@@ -40,7 +41,6 @@ constexpr int WG = 256;
 constexpr int NPART = 32;           // fields of a tally partial
 constexpr int8_t SEL_MARK = 0x7F;   // state byte of an agent selected for exposure today
 constexpr uint64_t SEL_INVALID = ~0ull;
-constexpr int R_GRID = 2;           // sampling rounds run as grid-wide kernels; later rounds in k_finish
 constexpr int MAX_ROUNDS = 4096;    // bound on the finisher's rounds (each round claims >= 1 agent)
 
 // Tally fields.
@@ -506,6 +506,7 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
 
     // ---- births, deaths, RI, SIA per agent ----
     uint64_t m[G];
+    uint32_t ipv_new[G];   // today's IPV doses, written after the group's disease progression
     int n_ei = 0;
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
@@ -515,6 +516,7 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
       uint2 r2 = RI ? rt[u] : make_uint2(0u, 0u);
       int64_t cw = 0;
       bool q = false;
+      ipv_new[u] = 0;
       if (u < ng && i0 < live_hi) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -573,7 +575,7 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
                   atomicAdd(&lc[L_RIV], 1);
                   if (s == 0) { s = 1; k = sp.ri_strain; d.strain[i] = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
                 }
-                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); d.ipv[i] = 1; }
+                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); ipv_new[u] |= 1u << (8 * a); }
               }
             }
             if (SIA && sia_here) {                        // fast_sia (model.py:2033-2059), per event
@@ -673,6 +675,18 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
       if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
       if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
       if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+    }
+    // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
+    if (RI) {
+      static_for<G>([&](auto uc) {
+        constexpr int u = decltype(uc)::value;
+        if (ipv_new[u]) {
+          const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            if ((ipv_new[u] >> (8 * a)) & 1u) d.ipv[i0 + a] = 1;
+        }
+      });
     }
     __syncthreads();
   }
@@ -1295,26 +1309,15 @@ __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
 
 
 // ---------------------------------------------------------------------------
-// Sampling round r >= 2: refresh the node's span prefix from the decremented
-// span weights, and the number of probes still needed.
-__global__ void __launch_bounds__(WG) k_prefix(Dev d) {
-  const int n = blockIdx.x;
-  const int32_t need = d.size[n] - d.n_uniq[n];
-  if (d.take_all[n] || need <= 0 || d.size[n] == 0) {
-    if (threadIdx.x == 0) d.m_round[n] = 0;
-  } else {
-    const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, d.node_span_begin[n], d.node_span_begin[n + 1]);
-    if (threadIdx.x == 0) { d.W_round[n] = W; d.m_round[n] = W > 0 ? need : 0; }
-  }
-}
+// Infection sampling (model.py:1089-1122): rounds of `need` with-replacement
+// probes proportional to the remaining selection weights; the distinct agents
+// hit are exposed.  Round 1 runs grid-wide (k_plan, k_sample, k_commit);
+// k_finish then takes each node on its own for its later rounds and handles the
+// take-all nodes.
 
 // Single workgroup: plan sampling round `round` (probe offsets over nodes, list space).
 __global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, round); }
 
-// One wavefront per probe: target T in [0, W); the span with prefix <= T by a
-// 64-ary search over the node's span prefix, the chunk inside the span by a wave
-// scan of its chunk weights, the agent inside the chunk by a wave scan of the
-// agents' weights.  Returns the slot, or -1 if the weights were inconsistent.
 __device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t p) {
   int a = 0, b = N;   // largest n in [0, N) with off[n] <= p < off[n+1]
   while (b - a > 1) { const int m = (a + b) >> 1; if (off[m] <= p) a = m; else b = m; }
@@ -1326,7 +1329,13 @@ __device__ __forceinline__ int64_t ld64(const int64_t* p) {
   return ATOMIC_IO ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
 }
 
-template <bool ATOMIC_IO>
+// One wavefront per probe: target T in [0, W); the span with prefix <= T by a
+// 64-ary search over the node's span prefix, the chunk inside the span by a wave
+// scan of its chunk weights, the agent inside the chunk by a wave scan of the
+// agents' weights.  MARKED_AS_S: an agent already claimed in this round
+// (SEL_MARK) keeps its weight, so that all probes of a round see the weights of
+// the round's start.  Returns the slot, or -1 if the weights were inconsistent.
+template <bool ATOMIC_IO, bool MARKED_AS_S>
 __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
   int64_t lo = d.node_span_begin[node];
   int64_t hi = d.node_span_begin[node + 1];   // search spans [lo, hi)
@@ -1355,14 +1364,18 @@ __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_
   const int64_t c = c0 + Lc;
   const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
   uint32_t st4;
-  if (ATOMIC_IO) st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else st4 = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
+  if (ATOMIC_IO || MARKED_AS_S)
+    st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    st4 = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
   const float4 rk = *reinterpret_cast<const float4*>(d.risk + i0);
   int64_t w[4];
-  w[0] = ((st4 & 0xffu) == 0) ? weight_q24(rk.x) : 0;
-  w[1] = (((st4 >> 8) & 0xffu) == 0) ? weight_q24(rk.y) : 0;
-  w[2] = (((st4 >> 16) & 0xffu) == 0) ? weight_q24(rk.z) : 0;
-  w[3] = (((st4 >> 24) & 0xffu) == 0) ? weight_q24(rk.w) : 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const uint32_t b = (st4 >> (8 * a)) & 0xffu;
+    const bool s_like = b == 0 || (MARKED_AS_S && b == (uint32_t)(uint8_t)SEL_MARK);
+    w[a] = s_like ? weight_q24(lane_of(rk, a)) : 0;
+  }
   const int64_t lsum = w[0] + w[1] + w[2] + w[3];
   const int64_t incl = wave_incl_scan_i64(lsum, lane);
   int64_t cum = cp + (incl - lsum);
@@ -1375,31 +1388,8 @@ __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_
   return c * LP_CHUNK + (int64_t)L * 4 + ha;
 }
 
-__global__ void __launch_bounds__(WG) k_resolve(Dev d, int round) {
-  const int32_t t = d.day[0];
-  const int N = d.n_nodes;
-  const int64_t M = d.probe_off[N];
-  const int lane = threadIdx.x & 63;
-  const int64_t nw = (int64_t)gridDim.x * (WG / 64);
-  const int64_t base = d.round_base[round];
-  for (int64_t p = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); p < M; p += nw) {
-    const int node = (int)find_node(d.probe_off, N, p);
-    const int64_t j = p - d.probe_off[node];
-    const lp_sim_params& sp = d.sims[d.node_sim[node]];
-    const uint4 o = philox((uint32_t)j, (uint32_t)(node - sp.node_begin), (uint32_t)t,
-                           ST_PROBE | ((uint32_t)round << 8), sp.seed);
-    const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
-    const int64_t W = d.W_round[node];
-    const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
-    const int64_t slot = W > 0 ? resolve_probe<false>(d, node, T, d.span_prefix, lane) : -1;
-    if (lane == 0) {
-      if (slot < 0) { d.sel_list[base + p] = SEL_INVALID; atomicOr(d.err, ERR_SAMPLING); }
-      else d.sel_list[base + p] = (uint64_t)slot | ((uint64_t)node << 40);
-    }
-  }
-}
-
-// Claim a hit: state byte 0 -> SEL_MARK with a word CAS; duplicates become invalid.
+// Claim a hit: state byte 0 -> SEL_MARK with a word CAS; false if the agent was
+// already claimed (a duplicate probe) or is not susceptible.
 __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
   uint32_t* w = reinterpret_cast<uint32_t*>(state) + (slot >> 2);
   const int sh = 8 * (int)(slot & 3);
@@ -1413,7 +1403,7 @@ __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
   }
 }
 
-// Remove a claimed agent's weight from its chunk and its span.
+// Remove an exposed agent's weight from its chunk and its span.
 __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
   const unsigned long long dw = (unsigned long long)(-weight_q24(d.risk[slot]));
   atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
@@ -1422,69 +1412,97 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
   atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
 }
 
-__global__ void __launch_bounds__(WG) k_mark(Dev d, int round) {
+// S -> E of a selected agent with its strain drawn by cumulative strain
+// probability (model.py:1125-1147); counted in the node's new exposures.
+__device__ __forceinline__ void expose_agent(const Dev& d, int node, const lp_sim_params& sp, int64_t slot, int32_t t) {
+  const uint4 o = draw(sp.seed, (uint64_t)(slot - sp.slot_base), t, ST_STRAIN);
+  int k = 0;
+  for (int s = 0; s < d.K; ++s) {
+    if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
+  }
+  d.state[slot] = 1;
+  d.strain[slot] = (int8_t)k;
+  atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+}
+
+// Sampling round 1, grid-wide: one wavefront per probe resolves it and claims
+// the agent.  The probe offsets are staged in LDS when they fit (dynamic shared
+// memory of (n_nodes + 1) * 8 bytes, else none).
+constexpr int LDS_OFF_MAX = 8192;
+__global__ void __launch_bounds__(WG) k_sample(Dev d) {
+  extern __shared__ int64_t s_off[];
+  const int N = d.n_nodes;
+  const int64_t M = d.probe_off[N];
+  const int64_t nw = (int64_t)gridDim.x * (WG / 64);
+  const int64_t first = (int64_t)blockIdx.x * (WG / 64);
+  if (first >= M) return;
+  const bool lds = N + 1 <= LDS_OFF_MAX;
+  if (lds) {
+    for (int i = threadIdx.x; i <= N; i += WG) s_off[i] = d.probe_off[i];
+    __syncthreads();
+  }
+  const int64_t* off = lds ? s_off : d.probe_off;
+  const int32_t t = d.day[0];
+  const int lane = threadIdx.x & 63;
+  const int64_t base = d.round_base[1];
+  for (int64_t p = first + (threadIdx.x >> 6); p < M; p += nw) {
+    const int node = (int)find_node(off, N, p);
+    const int64_t j = p - off[node];
+    const lp_sim_params& sp = d.sims[d.node_sim[node]];
+    const uint4 o = philox((uint32_t)j, (uint32_t)(node - sp.node_begin), (uint32_t)t, ST_PROBE | (1u << 8), sp.seed);
+    const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+    const int64_t W = d.W_round[node];
+    const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
+    const int64_t slot = W > 0 ? resolve_probe<false, true>(d, node, T, d.span_prefix, lane) : -1;
+    if (lane == 0) {
+      uint64_t e = SEL_INVALID;
+      if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
+      else if (claim(d.state, slot)) { atomicAdd(d.n_uniq + node, 1); e = (uint64_t)slot | ((uint64_t)node << 40); }
+      d.sel_list[base + p] = e;
+    }
+  }
+}
+
+// Grid-wide, after round 1: every agent claimed in round 1 loses its weight and
+// is exposed.
+__global__ void __launch_bounds__(WG) k_commit(Dev d) {
+  const int32_t t = d.day[0];
   const int64_t M = d.probe_off[d.n_nodes];
-  const int64_t base = d.round_base[round];
+  const int64_t base = d.round_base[1];
   for (int64_t p = (int64_t)blockIdx.x * WG + threadIdx.x; p < M; p += (int64_t)gridDim.x * WG) {
     const uint64_t e = d.sel_list[base + p];
     if (e == SEL_INVALID) continue;
     const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
     const int node = (int)(e >> 40);
-    if (slot >= d.capacity || node >= d.n_nodes) { d.sel_list[base + p] = SEL_INVALID; continue; }
-    if (claim(d.state, slot)) {
-      unweight(d, node, slot);
-      atomicAdd(d.n_uniq + node, 1);
-    } else {
-      d.sel_list[base + p] = SEL_INVALID;
-    }
+    if (node >= d.n_nodes || slot >= d.live_end[node] || slot < d.seg_begin[node]) { atomicOr(d.err, ERR_SAMPLING); continue; }
+    unweight(d, node, slot);
+    expose_agent(d, node, d.sims[d.node_sim[node]], slot, t);
   }
 }
 
-// Per node: take-all nodes (size >= #positive-weight susceptibles) select every
-// such agent; nodes still short after the grid rounds finish their rounds here.
+// Per node: take-all nodes (size >= #positive-weight susceptibles) expose every
+// such agent; otherwise the node's later rounds (2, 3, ...) run here until
+// `size` distinct agents are exposed or no weight is left.
 __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int64_t sh_base;
-  __shared__ int32_t sh_cnt;
   __shared__ int64_t sh_W;
   if (d.size[n] == 0) return;
+  const lp_sim_params& sp = d.sims[d.node_sim[n]];
   if (d.take_all[n]) {
     const int64_t lo = d.seg_begin[n], hi = d.live_end[n];
-    for (int64_t b0 = lo; b0 < hi; b0 += WG) {
-      const int64_t i = b0 + tid;
-      bool sel = false;
-      if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) sel = true;
-      const uint64_t bal = __ballot(sel);
-      if (tid == 0) sh_cnt = 0;
-      __syncthreads();
-      int woff = 0;
-      if (lane == 0) woff = atomicAdd(&sh_cnt, __popcll(bal));
-      woff = __shfl(woff, 0, 64);
-      __syncthreads();
-      if (tid == 0) {
-        const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)sh_cnt);
-        if (b + sh_cnt > d.sel_cap) {
-          atomicOr(d.err, ERR_OVERFLOW);
-          for (int64_t q = b; q < d.sel_cap; ++q) d.sel_list[q] = SEL_INVALID;
-          sh_base = -1;
-        } else sh_base = b;
-      }
-      __syncthreads();
-      if (sel && sh_base >= 0) {
-        const int pos = woff + __popcll(bal & ((1ull << lane) - 1));
-        d.state[i] = SEL_MARK;
+    for (int64_t i = lo + tid; i < hi; i += WG) {
+      if (d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
-        d.sel_list[sh_base + pos] = (uint64_t)i | ((uint64_t)n << 40);
+        expose_agent(d, n, sp, i, t);
       }
-      __syncthreads();
     }
     return;
   }
-  const lp_sim_params& sp = d.sims[d.node_sim[n]];
   const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
-  for (int round = R_GRID + 1;; ++round) {
+  for (int round = 2;; ++round) {
     if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); return; }
     const int32_t need = d.size[n] - __hip_atomic_load(d.n_uniq + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (need <= 0) return;
@@ -1494,7 +1512,6 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
       if (b + need > d.sel_cap) {
         atomicOr(d.err, ERR_OVERFLOW);
-        for (int64_t q = b; q < d.sel_cap; ++q) d.sel_list[q] = SEL_INVALID;
         sh_base = -1;
       } else sh_base = b;
       sh_W = W;
@@ -1507,7 +1524,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
                              ST_PROBE | ((uint32_t)round << 8), sp.seed);
       const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
       const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
-      const int64_t slot = resolve_probe<true>(d, n, T, d.fin_prefix, lane);
+      const int64_t slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
       if (lane == 0) {
         const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
         if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
@@ -1529,28 +1546,13 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
     }
     __threadfence();
     __syncthreads();
-  }
-}
-
-// Every valid selected slot: strain by cumulative strain probability, S -> E.
-__global__ void __launch_bounds__(WG) k_expose(Dev d) {
-  const int32_t t = d.day[0];
-  const int64_t M = d.sel_count[0] < d.sel_cap ? d.sel_count[0] : d.sel_cap;
-  for (int64_t p = (int64_t)blockIdx.x * WG + threadIdx.x; p < M; p += (int64_t)gridDim.x * WG) {
-    const uint64_t e = d.sel_list[p];
-    if (e == SEL_INVALID) continue;
-    const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-    const int node = (int)(e >> 40);
-    if (slot >= d.capacity || node >= d.n_nodes || d.state[slot] != SEL_MARK) continue;
-    const lp_sim_params& sp = d.sims[d.node_sim[node]];
-    const uint4 o = draw(sp.seed, (uint64_t)(slot - sp.slot_base), t, ST_STRAIN);
-    int k = 0;
-    for (int s = 0; s < d.K; ++s) {
-      if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
+    for (int j = tid; j < need; j += WG) {
+      const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (e == SEL_INVALID) continue;
+      expose_agent(d, n, sp, (int64_t)(e & ((1ull << 40) - 1)), t);
     }
-    d.state[slot] = 1;
-    d.strain[slot] = (int8_t)k;
-    atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+    __threadfence();
+    __syncthreads();
   }
 }
 

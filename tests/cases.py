@@ -65,6 +65,26 @@ def make(case, seed=7, **over):
                  r0_scalars=np.array([1.0, 1.0]), init_immun=0.0, init_prev=[5, 0], individual_heterogeneity=False,
                  dur_inf=lp.constant(4), distances=np.array([[0, 10], [10, 0]]), stop_if_no_cases=True)
         comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
+    elif case == "campaigns":
+        # SIA on a plain day (t=10), on a VD step day (t=7) and on a VD + RI day (t=14):
+        # every combination of births/deaths, RI and SIA in one day
+        p = dict(dur=45, init_pop=np.array([9000, 7000, 5000]), cbr=np.array([45.0]), r0=9,
+                 r0_scalars=np.array([1.0, 1.2, 0.9]), init_immun=0.4, init_prev=[15, 3, 0],
+                 distances=np.array([[0, 40, 70], [40, 0, 40], [70, 40, 0]]), vx_prob_ri=0.8, vx_prob_ipv=0.3,
+                 missed_frac=0.05, ri_newborn_timer=14, ri_age_days=(14, 28), vx_prob_sia=[0.7, 0.9, 0.6],
+                 sia_schedule=[{"date": "2019-01-08", "nodes": [0, 1], "age_range": (0, 5 * 365), "vaccinetype": "nOPV2"},
+                               {"date": "2019-01-11", "nodes": [2], "age_range": (0, 10 * 365), "vaccinetype": "topv"},
+                               {"date": "2019-01-15", "nodes": [0, 1, 2], "age_range": (0, 3 * 365), "vaccinetype": "mOPV2"}],
+                 step_size_RI_ABM=14)
+    elif case == "ri_only":
+        # RI and SIA without VitalDynamics: RI-only days and an SIA on an RI day
+        p = dict(dur=40, init_pop=np.array([8000, 6000]), cbr=None, r0=8, r0_scalars=np.array([1.0, 1.1]),
+                 init_immun=0.3, init_prev=[10, 0], distances=np.array([[0, 30], [30, 0]]), vx_prob_ri=0.9,
+                 vx_prob_ipv=0.5, ri_age_days=(0, 60), vx_prob_sia=[0.8, 0.8],
+                 sia_schedule=[{"date": "2019-01-15", "nodes": [0, 1], "age_range": (0, 5 * 365), "vaccinetype": "nOPV2"},
+                               {"date": "2019-01-20", "nodes": [1], "age_range": (0, 5 * 365), "vaccinetype": "topv"}],
+                 step_size_RI_ABM=14)
+        comps = [lp.DiseaseState_ABM, lp.RI_ABM, lp.SIA_ABM, lp.Transmission_ABM]
     else:
         raise KeyError(case)
     base.update(p)
@@ -74,4 +94,4 @@ def make(case, seed=7, **over):
     return sim
 
 
-CASES = ["small3", "sia", "c0mini", "hot", "zinb", "stop"]
+CASES = ["small3", "sia", "c0mini", "hot", "zinb", "stop", "campaigns", "ri_only"]

@@ -60,3 +60,40 @@ def test_per_component_steps_match_oracle(case):
     ref = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop).run()
     sim._run_per_tick()
     _compare(sim, ref)
+
+
+def _host_edit(state, inf_timer, risk, slots):
+    """The edit a host would make between days: some susceptibles become infectious."""
+    sus = slots[state[slots] == 0][:40]
+    state[sus] = 2
+    inf_timer[sus] = 6
+    risk[slots[:25]] *= 2.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_host_edit_between_days_matches_oracle(use_graph):
+    """Agent columns written by the host mid-run (lp_invalidate -> recount of the carried
+    susceptible statistics) give the same continuation as the oracle with the same edit."""
+    print(f"[parity-host-edit] graph={use_graph}", flush=True)
+    sim = make("campaigns")
+    k = 16
+    ref = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop)
+    for t in range(0, k):
+        ref.day(t)
+    slots = np.arange(sim.people.seg_begin[1], sim.people.seg_begin[1] + 3000)
+    _host_edit(ref.cols["disease_state"], ref.cols["infection_timer"], ref.cols["acq_risk_multiplier"], slots)
+    for t in range(k, sim.nt):
+        ref.day(t)
+
+    eng = sim._ensure_engine()
+    eng.run_days(0, k, use_graph=use_graph)
+    eng.sync()
+    sim._mark_device_newer()
+    p = sim.people
+    _host_edit(p.disease_state, p.infection_timer, p.acq_risk_multiplier, slots)
+    eng = sim._ensure_engine()          # pushes the edited columns (and invalidates)
+    eng.run_days(k, sim.nt, use_graph=use_graph)
+    eng.sync()
+    sim._mark_device_newer()
+    _compare(sim, ref)
