@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 enum : int { FULL_VD = 1, FULL_RI = 2, FULL_SIA = 4 };
 
 template <int F>
-__global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
+__global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   constexpr int G = 4;
   constexpr bool VD = F & FULL_VD, RI = F & FULL_RI, SIA = F & FULL_SIA;
   const int4 span0 = d.spans[blockIdx.x];
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
 
     // ---- births, deaths, RI, SIA per agent ----
     uint64_t m[G];
-    uint32_t ipv_new[G];   // today's IPV doses, written after the group's disease progression
+    uint32_t ipv_new = 0;  // today's IPV doses (bit 4u + a), written after the group's disease progression
     int n_ei = 0;
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
@@ -516,7 +516,6 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
       uint2 r2 = RI ? rt[u] : make_uint2(0u, 0u);
       int64_t cw = 0;
       bool q = false;
-      ipv_new[u] = 0;
       if (u < ng && i0 < live_hi) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -575,7 +574,7 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
                   atomicAdd(&lc[L_RIV], 1);
                   if (s == 0) { s = 1; k = sp.ri_strain; d.strain[i] = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
                 }
-                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); ipv_new[u] |= 1u << (8 * a); }
+                if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); ipv_new |= 1u << (4 * u + a); }
               }
             }
             if (SIA && sia_here) {                        // fast_sia (model.py:2033-2059), per event
@@ -677,15 +676,13 @@ __global__ void __launch_bounds__(WG) k_full(Dev d, int mask) {
       if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
-    if (RI) {
+    if (RI && ipv_new) {
       static_for<G>([&](auto uc) {
         constexpr int u = decltype(uc)::value;
-        if (ipv_new[u]) {
-          const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
+        const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
-            if ((ipv_new[u] >> (8 * a)) & 1u) d.ipv[i0 + a] = 1;
-        }
+        for (int a = 0; a < 4; ++a)
+          if ((ipv_new >> (4 * u + a)) & 1u) d.ipv[i0 + a] = 1;
       });
     }
     __syncthreads();
