@@ -8,14 +8,14 @@ runs ``--warmup`` days untimed, then times exactly ``--steps`` days bracketed by
 torch.cuda.synchronize().  value = sum over timed days of live agents / seconds.
 
 Also reported on the same JSON line:
-  roofline      the agent-pass kernel (k_agents): algorithmic bytes per day
-                (formula in DESIGN.md) over its HIP-event-timed duration, against
-                MI355X's 8 TB/s HBM peak
+  roofline      the agent pass (k_plain on plain days, k_full on VD / RI / SIA days):
+                algorithmic bytes per day (agent_pass_bytes below) over its
+                HIP-event-timed duration, against MI355X's 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
                 threads) on the same initial population for the first --cpu-days days,
                 plus one further day on a single thread
 The HBM traffic of the agent pass (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch,
-calibrated on a plain copy) is read from profiles/k_agents_national_traffic.md
+calibrated on a plain copy) is read from profiles/agent_pass_national_traffic.md
 when that file was measured on the same workload and window; otherwise null.
 """
 
@@ -130,7 +130,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU node sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+              "sharding the nodes over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -189,7 +189,7 @@ def main():
         tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
         roofline = {
             "bound": "hbm",
-            "kernel": "k_agents",
+            "kernel": "agent pass (k_plain / k_full)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
