@@ -8,6 +8,7 @@ liblpstep.so (csrc/), bound through the C ABI of include/lp_step.h.
 """
 
 from . import distributions
+from .batch import BatchedSims
 from .distributions import constant, exponential, gamma, lognormal, lognormal_int, normal, poisson, uniform
 from .laserframe import LaserFrame, PeopleFrame
 from .model import RI_ABM, SEIR_ABM, SIA_ABM, DiseaseState_ABM, SimTables, Transmission_ABM, VitalDynamics_ABM

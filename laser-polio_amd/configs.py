@@ -15,6 +15,12 @@ under-5s (nOPV2), 2555 days, seed 42.
 ``spatial14(scale)`` -- the 14-node case: populations LogNormal(ln 3e5, 0.5)
 (about 4.5e6 agents), symmetric distances U(20, 250) km, no RI or SIA, 365 days;
 otherwise as above.
+
+``replicate_pars(i, master_seed, scale)`` -- replicate i of the batched-replicates
+workload: a spatial14-type simulation with node count uniform 8..40, total
+population uniform 0.5..6 million (times `scale`), R0 uniform 8..18, seasonal
+amplitude uniform 0..0.6, gravity k uniform 0.1..1 and c uniform 1.5..2.5, and its
+own seed; 365 days.
 """
 
 import numpy as np
@@ -94,14 +100,30 @@ def nigeria_national_pars(scale=1.0, seed=42, dur=2555, n=774):
     return PropertySet(p)
 
 
+def replicate_pars(i, master_seed=2024, scale=1.0, dur=365):
+    rng = np.random.default_rng([master_seed, i])
+    n = int(rng.integers(8, 41))
+    total = rng.uniform(0.5e6, 6e6) * scale
+    w = rng.lognormal(0.0, 0.5, n)
+    pop = np.maximum(1, np.round(total * w / w.sum())).astype(np.int64)
+    d = rng.uniform(20, 250, (n, n))
+    d = np.triu(d, 1)
+    d = d + d.T
+    r0 = float(rng.uniform(8, 18))
+    p = _common(int(rng.integers(1, 2**31)), r0, n, rng)
+    p.update(dur=dur, init_pop=pop, distances=d, seasonal_amplitude=float(rng.uniform(0.0, 0.6)),
+             gravity_k=float(rng.uniform(0.1, 1.0)), gravity_c=float(rng.uniform(1.5, 2.5)))
+    return PropertySet(p)
+
+
 def build(pars, components=None):
     """SEIR_ABM with the benchmark component list (RI/SIA only when configured)."""
     sim = SEIR_ABM(pars)
     if components is None:
         components = [VitalDynamics_ABM, DiseaseState_ABM, Transmission_ABM]
-        if pars.vx_prob_ri is not None:
+        if sim.pars.vx_prob_ri is not None:
             components.append(RI_ABM)
-        if pars.sia_schedule:
+        if sim.pars.sia_schedule:
             components.append(SIA_ABM)
     sim.components = components
     return sim

@@ -529,7 +529,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
             if (i >= nb_lo) {                             // newborn (model.py:1719-1732)
               d.dob[i] = t;
               d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
-              d.node_id[i] = (int16_t)node;
+              d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
               if (sp.newborn_ri_timer >= 0) {
                 d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
                 if (RI) {
@@ -786,7 +786,7 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
               const uint64_t slot = (uint64_t)(i - sbase);
               d.dob[i] = t;
               d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
-              d.node_id[i] = (int16_t)node;
+              d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
               if (sp.newborn_ri_timer >= 0) {
                 d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
                 if (do_ri) {

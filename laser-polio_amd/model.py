@@ -397,11 +397,8 @@ class SEIR_ABM:
                 out[bind] = a
         return out
 
-    def _ensure_engine(self):
-        if self._engine is not None:
-            self._push_touched()
-            return self._engine
-        tb = self.export_tables()
+    def sim_params(self, tb):
+        """lp_sim_params record of this simulation (slot / node / network offsets 0)."""
         sp = np.zeros(1, dtype=engmod.SIM_PARAMS_DTYPE)
         sp["seed"] = tb.seed
         sp["slot_base"] = 0
@@ -417,6 +414,14 @@ class SEIR_ABM:
         sp["ri_step"] = tb.ri_step
         sp["newborn_ri_timer"] = tb.newborn_ri_timer
         sp["last_seed_day"] = tb.last_seed_day
+        return sp
+
+    def _ensure_engine(self):
+        if self._engine is not None:
+            self._push_touched()
+            return self._engine
+        tb = self.export_tables()
+        sp = self.sim_params(tb)
         arrays = dict(tb.cols)
         arrays.update(
             seg_begin=tb.seg_begin,

@@ -1,0 +1,70 @@
+"""Batched replicates: one library context advancing several simulations.
+
+CPU tests check the packing; the GPU test checks that every simulation of a
+batch ends bit-identical to the oracle run of that simulation alone (draws are
+keyed by sim seed and sim-local slot / node, so the batch must not change them).
+"""
+
+import numpy as np
+import pytest
+
+import laser_polio_amd as lp
+from laser_polio_amd import configs
+from oracle.oracle import OracleSim
+
+
+def _replicates(n, scale=0.004, dur=40):
+    return [configs.build(configs.replicate_pars(i, scale=scale, dur=dur)) for i in range(n)]
+
+
+def test_replicate_workload_varies_per_sim():
+    ps = [configs.replicate_pars(i) for i in range(12)]
+    nodes = [len(p.init_pop) for p in ps]
+    assert min(nodes) >= 8 and max(nodes) <= 40 and len(set(nodes)) > 3
+    tot = [p.init_pop.sum() for p in ps]
+    assert 0.5e6 * 0.99 <= min(tot) and max(tot) <= 6e6 * 1.01
+    assert len({p.seed for p in ps}) == 12
+    assert all(8 <= p.r0 <= 18 for p in ps)
+
+
+def test_batch_packing():
+    sims = _replicates(3)
+    b = lp.BatchedSims(sims)
+    a, tb = b.arrays, b.tables
+    caps = [t.capacity for t in tb]
+    assert b.cfg.capacity == sum(caps) and b.cfg.n_sims == 3
+    assert b.cfg.n_nodes == sum(t.n_nodes for t in tb)
+    sp = a["sim_params"]
+    assert list(sp["slot_base"]) == [0, caps[0], caps[0] + caps[1]]
+    assert list(sp["node_begin"]) == list(np.cumsum([0] + [t.n_nodes for t in tb])[:-1])
+    assert list(sp["net_off"]) == list(np.cumsum([0] + [t.n_nodes ** 2 for t in tb])[:-1])
+    assert list(sp["seed"]) == [t.seed for t in tb]
+    for k, t in enumerate(tb):
+        na, nb = b.node_off[k], b.node_off[k + 1]
+        assert np.array_equal(a["seg_begin"][na:nb], t.seg_begin + b.slot_off[k])
+        assert np.array_equal(a["node_sim"][na:nb], np.full(t.n_nodes, k))
+        assert np.array_equal(a["season"].reshape(t.nt, 3)[:, k], t.season)
+        assert np.array_equal(a["disease_state"][b.slot_off[k]:b.slot_off[k + 1]], t.cols["disease_state"])
+        assert np.array_equal(a["pop"][0, na:nb], sims[k].pars.init_pop)
+
+
+def test_batch_rejects_mismatched_schedules():
+    a = configs.build(configs.replicate_pars(0, scale=0.004, dur=40))
+    b = configs.build(configs.replicate_pars(1, scale=0.004, dur=50))
+    with pytest.raises(ValueError):
+        lp.BatchedSims([a, b])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_matches_each_sim_alone(use_graph):
+    print(f"[batch-parity] graph={use_graph}", flush=True)
+    from test_gpu_parity import _compare
+
+    sims = _replicates(5)
+    refs = [OracleSim(s.export_tables(), pop0=s.pars.init_pop).run() for s in sims]
+    b = lp.BatchedSims(sims).run(use_graph=use_graph)
+    for s, ref in zip(sims, refs):
+        assert s.t == ref.ticks_run
+        _compare(s, ref)
+    b.close()
