@@ -112,6 +112,12 @@ def main():
     ap.add_argument("--dump-days", default="", help="write per-day N_live / N_EI / N_draw / flags to this CSV")
     ap.add_argument("--graph", type=int, default=0, help="1: replay one hipGraph per day (no per-kernel events)")
     ap.add_argument("--tiles-per-block", type=int, default=0)
+    ap.add_argument("--workload", choices=["national", "replicates"], default="national",
+                    help="national: one 774-node simulation (default); replicates: --sims independent "
+                         "simulations of uneven sizes batched in one context, against serial runs of a sample")
+    ap.add_argument("--sims", type=int, default=512, help="replicates: simulations in the batch")
+    ap.add_argument("--serial-sample", type=int, default=16, help="replicates: simulations also run one at a time")
+    ap.add_argument("--workers", type=int, default=16, help="replicates: host processes initialising simulations")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="instead of the benchmark: copy this many GiB device-to-device 5 times (a plain copy kernel "
                          "of known size, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -132,6 +138,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the nodes over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "replicates":
+        replicates(args)
+        return
 
     import torch
 
@@ -258,6 +268,104 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def replicates(args):
+    """Batched replicates: `--sims` simulations of the replicate workload (node count 8-40,
+    0.5-6 M agents each at scale 1, R0 8-18, seasonality amplitude 0-0.6, gravity k 0.1-1,
+    c 1.5-2.5, distinct seeds, 365 days) in one library context, every kernel launch of a
+    day covering all of them; against the same simulations run one at a time (one context
+    each, the same kernels and hipGraphs).  Serial time is measured on an evenly spaced
+    sample and extended to the whole set by a least-squares fit of time = a * days + b *
+    agent-days over the sample.  Host initialisation and uploads are not timed."""
+    lp = __graft_entry__.load_package()
+    from laser_polio_amd import configs
+    from laser_polio_amd.batch import build_replicates
+
+    S, dur = args.sims, 365
+    pars_list = [configs.replicate_pars(i, scale=args.scale, dur=dur) for i in range(S)]
+    t_init = time.perf_counter()
+    batch = build_replicates(pars_list, configs.build, workers=max(1, args.workers), log=log)
+    n0 = int(sum(int(p.init_pop.sum()) for p in pars_list))
+    log(f"[bench] replicates: {S} simulations, {batch.cfg.n_nodes} nodes, {n0} agents, capacity "
+        f"{batch.cfg.capacity}, host init {time.perf_counter() - t_init:.1f}s")
+
+    import torch
+
+    try:
+        if args.tiles_per_block:
+            tpw, tune = args.tiles_per_block, None
+            batch.tiles_per_block = tpw
+            batch.cfg.tiles_per_block = tpw
+        else:
+            tpw, tune = batch.autotune()
+            log(f"[bench] autotune tiles_per_block: " + ", ".join(f"{k}: {v * 1e3:.1f} ms" for k, v in tune.items())
+                + f" -> {tpw}")
+        sample = sorted(set(np.linspace(0, S - 1, min(args.serial_sample, S)).round().astype(int).tolist()))
+        singles = [batch.subset([k], tiles_per_block=tpw) for k in sample]
+        t_up = time.perf_counter()
+        eng = batch.prepare()
+        torch.cuda.synchronize()
+        log(f"[bench] upload {time.perf_counter() - t_up:.1f}s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run_days(0, batch.cfg.nt, use_graph=True)
+        eng.sync()
+        torch.cuda.synchronize()
+        el_b = time.perf_counter() - t0
+        ad_b = batch.agent_days(1, batch.cfg.nt)
+        per_sim_ad = []
+        pop = np.empty_like(batch.arrays["pop"])
+        eng.pull("pop", pop)
+        for k in range(S):
+            per_sim_ad.append(int(pop[1:, batch.node_off[k]:batch.node_off[k + 1]].astype(np.int64).sum()))
+        batch.close()
+        log(f"[bench] batched: {S} sims x {dur} days in {el_b:.3f}s -> {S / el_b * 3600:.4e} sims/hr, "
+            f"{ad_b / el_b:.4e} agent-days/s")
+
+        times, ads = [], []
+        for k, one in zip(sample, singles):
+            e1 = one.prepare()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e1.run_days(0, one.cfg.nt, use_graph=True)
+            e1.sync()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            ads.append(one.agent_days(1, one.cfg.nt))
+            one.close()
+        A = np.vstack([np.full(len(times), dur, float), np.array(ads, float)]).T
+        coef, *_ = np.linalg.lstsq(A, np.array(times), rcond=None)
+        serial_total = float(coef[0] * dur * S + coef[1] * sum(per_sim_ad))
+        serial_total = max(serial_total, float(np.sum(times)))
+        log(f"[bench] serial sample: {len(times)} sims in {np.sum(times):.3f}s ({np.mean(times) * 1e3:.1f} ms each); "
+            f"fit {coef[0] * 1e6:.1f} us/day + {coef[1] * 1e12:.2f} ps/agent-day -> {serial_total:.2f}s for all {S}")
+    finally:
+        batch.close()
+    out = {
+        "metric": "replicate-sims/hr batched vs serial (replicate workload, 365 days)",
+        "value": round(S / el_b * 3600, 1),
+        "unit": "sims/hr",
+        "n_gpus": 1,
+        "steps": dur,
+        "warmup": 0,
+        "ms_per_step": round(el_b * 1e3 / dur, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"replicates: {S} simulations, {batch.cfg.n_nodes} nodes, {n0} agents "
+                               f"(scale {args.scale}), 365 days, batched in one context",
+                   "tiles_per_block": int(tpw)},
+        "serial_sims_per_hr": round(S / serial_total * 3600, 1),
+        "speedup_vs_serial": round(serial_total / el_b, 2),
+        "batched_agent_days_per_s": round(ad_b / el_b, 1),
+        "serial_sample": {"sims": len(times), "seconds": round(float(np.sum(times)), 4),
+                          "fit_us_per_day": round(float(coef[0]) * 1e6, 2),
+                          "fit_ps_per_agent_day": round(float(coef[1]) * 1e12, 3)},
     }
     print(json.dumps(out), flush=True)
 

@@ -16,28 +16,62 @@ same day schedule (vital-dynamics / RI step days), the stock component classes
 and no ``seed_schedule`` injections (those run host code between stages).
 SIA campaigns must be the same events (day, vaccine, ages) in every simulation;
 which nodes they target may differ.
+
+``build_replicates(pars_list, workers)`` initialises the simulations of a large
+batch in worker processes that write the agent columns straight into shared
+memory at their batch offsets (host initialisation is the slow part of a
+512-simulation batch), returning a ``BatchedSims`` without per-sim objects.
 """
+
+import multiprocessing as mp
+import time
+from multiprocessing import shared_memory
 
 import numpy as np
 
 from . import engine as engmod
-from .model import AGENT_COLUMNS, RESULT_ARRAYS, SEIR_ABM
+from .laserframe import PeopleFrame
+from .model import AGENT_COLUMNS, RESULT_ARRAYS, SEIR_ABM, node_capacities, resolve_pars
+
+
+class SimPack:
+    """What a batch needs of one simulation: its tables (agent columns may live in a
+    shared buffer instead), its lp_sim_params record and its initial results arrays."""
+
+    def __init__(self, tables, sim_params, results, sim=None):
+        self.tables, self.sim_params, self.results, self.sim = tables, sim_params, results, sim
+
+    @classmethod
+    def of(cls, sim):
+        if not isinstance(sim, SEIR_ABM) or not sim._fused_ok():
+            raise ValueError("every simulation must be a SEIR_ABM with the stock components")
+        ds = sim._instance("DiseaseState_ABM")
+        if ds is not None and len(ds.seed_schedule):
+            raise ValueError("seed_schedule injections are not supported in a batch")
+        if sim.t != 0:
+            raise ValueError("simulations must not have been run")
+        tb = sim.export_tables()
+        return cls(tb, sim.sim_params(tb), sim._results_arrays(), sim)
 
 
 class BatchedSims:
     def __init__(self, sims, tiles_per_block=0, timing=False):
         if not sims:
             raise ValueError("BatchedSims needs at least one simulation")
-        self.sims = list(sims)
-        for s in self.sims:
-            if not isinstance(s, SEIR_ABM) or not s._fused_ok():
-                raise ValueError("every simulation must be a SEIR_ABM with the stock components")
-            ds = s._instance("DiseaseState_ABM")
-            if ds is not None and len(ds.seed_schedule):
-                raise ValueError("seed_schedule injections are not supported in a batch")
-            if s.t != 0:
-                raise ValueError("simulations must not have been run")
-        self.tables = [s.export_tables() for s in self.sims]
+        self._init([SimPack.of(s) for s in sims], None, tiles_per_block, timing)
+
+    @classmethod
+    def from_packs(cls, packs, cols=None, tiles_per_block=0, timing=False, owner=None):
+        """Batch of SimPacks; `cols` = the already concatenated agent columns, if any."""
+        self = cls.__new__(cls)
+        self._init(list(packs), cols, tiles_per_block, timing)
+        self._owner = owner
+        return self
+
+    def _init(self, packs, cols, tiles_per_block, timing):
+        self.packs = packs
+        self.sims = [p.sim for p in packs]
+        self.tables = [p.tables for p in packs]
         t0 = self.tables[0]
         for tb in self.tables[1:]:
             if tb.nt != t0.nt or tb.n_strains != t0.n_strains:
@@ -51,21 +85,23 @@ class BatchedSims:
                 raise ValueError("SIA events must be the same in every simulation")
         self.timing = timing
         self.tiles_per_block = int(tiles_per_block)
-        self._pack()
+        self._owner = None
+        self._pack(cols)
         self.engine = None
 
     # -- packing --------------------------------------------------------------------
-    def _pack(self):
+    def _pack(self, cols):
         tbs, S = self.tables, len(self.tables)
         nt, K = tbs[0].nt, tbs[0].n_strains
         caps = np.array([tb.capacity for tb in tbs], np.int64)
         nns = np.array([tb.n_nodes for tb in tbs], np.int64)
         self.slot_off = np.concatenate([[0], np.cumsum(caps)])
         self.node_off = np.concatenate([[0], np.cumsum(nns)])
-        net_sizes = nns * nns
-        net_off = np.concatenate([[0], np.cumsum(net_sizes)])
+        net_off = np.concatenate([[0], np.cumsum(nns * nns)])
         N = int(self.node_off[-1])
-        arrays = {name: np.concatenate([tb.cols[name] for tb in tbs]) for name in AGENT_COLUMNS}
+        if cols is None:
+            cols = {name: np.concatenate([tb.cols[name] for tb in tbs]) for name in AGENT_COLUMNS}
+        arrays = dict(cols)
         arrays["seg_begin"] = np.concatenate([tb.seg_begin + o for tb, o in zip(tbs, self.slot_off)])
         arrays["live_end"] = np.concatenate([tb.live_end + o for tb, o in zip(tbs, self.slot_off)])
         arrays["seg_end"] = np.concatenate([tb.seg_end + o for tb, o in zip(tbs, self.slot_off)])
@@ -73,7 +109,7 @@ class BatchedSims:
         for key, attr in (("r0_scalar", "r0s"), ("keep", "keep"), ("birth_rate_step", "birth_rate_step"),
                           ("thr_ri", "thr_ri"), ("thr_ipv", "thr_ipv")):
             arrays[key] = np.concatenate([getattr(tb, attr) for tb in tbs])
-        sp = np.concatenate([s.sim_params(tb) for s, tb in zip(self.sims, tbs)])
+        sp = np.concatenate([p.sim_params for p in self.packs])
         sp["slot_base"] = self.slot_off[:-1]
         sp["net_off"] = net_off[:-1]
         sp["node_begin"] = self.node_off[:-1]
@@ -90,9 +126,8 @@ class BatchedSims:
             arrays["sia_min_age"] = tbs[0].sia_min_age
             arrays["sia_max_age"] = tbs[0].sia_max_age
             arrays["sia_thr"] = np.concatenate([tb.sia_thr.reshape(E, tb.n_nodes, 2) for tb in tbs], axis=1).reshape(-1)
-        for attr, (bind, per_strain) in RESULT_ARRAYS.items():
-            parts = [s._results_arrays()[bind] for s in self.sims]
-            arrays[bind] = np.ascontiguousarray(np.concatenate(parts, axis=1))
+        for bind, _ in RESULT_ARRAYS.values():
+            arrays[bind] = np.ascontiguousarray(np.concatenate([p.results[bind] for p in self.packs], axis=1))
         self.arrays = arrays
         self.cfg = engmod.LPConfig(capacity=int(self.slot_off[-1]), n_nodes=N, n_sims=S, n_strains=K, nt=nt,
                                    n_sia_events=E, tiles_per_block=self.tiles_per_block, mix_mode=0)
@@ -116,6 +151,8 @@ class BatchedSims:
         return self
 
     def scatter(self):
+        if any(s is None for s in self.sims):
+            raise ValueError("this batch was built without per-simulation objects")
         eng = self.engine
         cols = {}
         for name in AGENT_COLUMNS:
@@ -152,7 +189,120 @@ class BatchedSims:
         t1 = self.cfg.nt if t1 is None else t1
         return int(pop[t0:t1].astype(np.int64).sum())
 
+    def subset(self, ks, tiles_per_block=None):
+        """A new batch of simulations `ks` (their packed columns copied out of this one)."""
+        ks = list(ks)
+        cols = {name: np.concatenate([self.arrays[name][self.slot_off[k]:self.slot_off[k + 1]] for k in ks])
+                for name in AGENT_COLUMNS}
+        packs = [self.packs[k] for k in ks]
+        tpw = self.tiles_per_block if tiles_per_block is None else tiles_per_block
+        return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
+
+    def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True):
+        """Pick tiles_per_block by timing `days` simulated days of the first `max_sims`
+        simulations with each candidate (fresh uploads, so nothing of this batch runs)."""
+        import torch
+
+        ks = range(min(max_sims, len(self.packs)))
+        times = {}
+        for tpw in candidates:
+            sub = self.subset(ks, tiles_per_block=tpw)
+            eng = sub.prepare()
+            eng.run_days(0, 2, use_graph=use_graph)
+            eng.sync()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            eng.run_days(2, min(self.cfg.nt, 2 + days), use_graph=use_graph)
+            eng.sync()
+            torch.cuda.synchronize()
+            times[tpw] = time.perf_counter() - t
+            sub.close()
+        best = min(times, key=times.get)
+        self.tiles_per_block = best
+        self.cfg.tiles_per_block = best
+        return best, times
+
     def close(self):
         if self.engine is not None:
             self.engine.close()
             self.engine = None
+        if self._owner is not None:
+            self._owner.release()
+            self._owner = None
+
+
+# -- parallel host initialisation of a large batch ---------------------------------------
+class _SharedColumns:
+    """One shared-memory block per agent column, sized for the whole batch."""
+
+    def __init__(self, total):
+        self.blocks, self.arrays = {}, {}
+        try:
+            for name, (dt, _) in AGENT_COLUMNS.items():
+                nbytes = max(1, int(total) * np.dtype(dt).itemsize)
+                blk = shared_memory.SharedMemory(create=True, size=nbytes)
+                self.blocks[name] = blk
+                self.arrays[name] = np.ndarray((int(total),), dtype=dt, buffer=blk.buf)
+        except BaseException:
+            self.release()
+            raise
+
+    def release(self):
+        self.arrays = {}
+        for blk in self.blocks.values():
+            try:
+                blk.close()
+                blk.unlink()
+            except FileNotFoundError:
+                pass
+        self.blocks = {}
+
+
+_W = {}
+
+
+def _worker_init(names, total, slot_off, build):
+    _W["blocks"] = {n: shared_memory.SharedMemory(name=b) for n, b in names.items()}
+    _W["arrays"] = {n: np.ndarray((total,), dtype=AGENT_COLUMNS[n][0], buffer=_W["blocks"][n].buf) for n in names}
+    _W["slot_off"], _W["build"] = slot_off, build
+
+
+def _worker_build(args):
+    k, pars = args
+    sim = _W["build"](pars)
+    tb = sim.export_tables()
+    a, b = _W["slot_off"][k], _W["slot_off"][k + 1]
+    if tb.capacity != b - a:
+        raise RuntimeError(f"simulation {k}: capacity {tb.capacity} != planned {b - a}")
+    for name in AGENT_COLUMNS:
+        _W["arrays"][name][a:b] = tb.cols[name]
+    sp, res = sim.sim_params(tb), sim._results_arrays()
+    tb.cols = None
+    return k, tb, sp, res
+
+
+def build_replicates(pars_list, build, workers=8, tiles_per_block=0, timing=False, log=None):
+    """BatchedSims of `build(pars)` for every pars in `pars_list`, initialised in `workers`
+    forked processes that write the agent columns into shared memory.  Must run before the
+    calling process touches the GPU."""
+    caps = []
+    for p in pars_list:
+        rp = resolve_pars(p)
+        pop = np.atleast_1d(rp.init_pop).astype(np.int64)
+        caps.append(int(PeopleFrame(pop, node_capacities(rp)).capacity))
+    slot_off = np.concatenate([[0], np.cumsum(caps)]).astype(np.int64)
+    total = int(slot_off[-1])
+    shared = _SharedColumns(total)
+    try:
+        names = {n: blk.name for n, blk in shared.blocks.items()}
+        packs = [None] * len(pars_list)
+        ctx = mp.get_context("fork")
+        with ctx.Pool(workers, initializer=_worker_init, initargs=(names, total, slot_off, build)) as pool:
+            for done, (k, tb, sp, res) in enumerate(pool.imap_unordered(_worker_build, enumerate(pars_list)), 1):
+                packs[k] = SimPack(tb, sp, res)
+                if log is not None and (done % 32 == 0 or done == len(pars_list)):
+                    log(f"[batch] initialised {done}/{len(pars_list)} simulations")
+        return BatchedSims.from_packs(packs, shared.arrays, tiles_per_block=tiles_per_block, timing=timing, owner=shared)
+    except BaseException:
+        shared.release()
+        raise

@@ -113,6 +113,31 @@ class SimTables:
 
 
 # =============================================================================
+def node_capacities(pars):
+    """Agent slots per node for a run: the initial population plus the births that
+    calc_capacity expects over dur + 100 days, with a 4-sigma margin (newborns of a node
+    stay in its segment)."""
+    pop = np.atleast_1d(pars.init_pop).astype(np.int64)
+    cbr = None if pars.cbr is None else np.broadcast_to(np.atleast_1d(np.asarray(pars.cbr, dtype=np.float64)), pop.shape)
+    cap = np.array(pop, dtype=np.int64)
+    if cbr is not None:
+        for n in range(len(pop)):
+            eb = calc_capacity(pop[n], pars.dur + 100, cbr[n]) - pop[n]
+            if eb > 0:
+                cap[n] = int((1 + 4 / np.sqrt(eb)) * (pop[n] + eb)) + 16
+    return cap
+
+
+def resolve_pars(pars):
+    """The reference defaults overridden by `pars` (unknown keys dropped), as SEIR_ABM uses them."""
+    out = deepcopy(default_pars)
+    if pars is not None:
+        if not isinstance(pars, PropertySet):
+            pars = PropertySet(pars)
+        out <<= {k: v for k, v in pars.items() if k in out}
+    return out
+
+
 class SEIR_ABM:
     """Agent-based SEIR model of polio; disease_state codes -1 dead/unborn, 0 S, 1 E, 2 I, 3 R."""
 
@@ -149,14 +174,7 @@ class SEIR_ABM:
             pop = pars.init_pop
             if np.any(pop <= 0) or np.any(np.isnan(pop.astype(np.float64))):
                 raise ValueError("pop_by_node must be positive & non-nan")
-            cbr = None if pars.cbr is None else np.broadcast_to(np.atleast_1d(np.asarray(pars.cbr, dtype=np.float64)), pop.shape)
-            cap = np.array(pop, dtype=np.int64)
-            if cbr is not None:
-                for n in range(len(pop)):
-                    eb = calc_capacity(pop[n], pars.dur + 100, cbr[n]) - pop[n]
-                    if eb > 0:
-                        cap[n] = int((1 + 4 / np.sqrt(eb)) * (pop[n] + eb)) + 16
-            self.people = PeopleFrame(pop, cap)
+            self.people = PeopleFrame(pop, node_capacities(pars))
             p = self.people
             p.add_scalar_property("disease_state", dtype=np.int8, default=-1)
             for n in range(len(pop)):

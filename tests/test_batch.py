@@ -68,3 +68,20 @@ def test_batch_matches_each_sim_alone(use_graph):
         assert s.t == ref.ticks_run
         _compare(s, ref)
     b.close()
+
+
+def test_parallel_build_matches_serial_packing():
+    from laser_polio_amd.batch import build_replicates
+
+    pars = [configs.replicate_pars(i, scale=0.003, dur=30) for i in range(5)]
+    par = build_replicates(pars, configs.build, workers=3)
+    try:
+        ser = lp.BatchedSims([configs.build(p) for p in pars])
+        assert par.cfg.capacity == ser.cfg.capacity and par.cfg.n_nodes == ser.cfg.n_nodes
+        for name, arr in ser.arrays.items():
+            if name == "sim_params":
+                assert par.arrays[name].tobytes() == arr.tobytes()
+            else:
+                assert np.array_equal(par.arrays[name], arr), name
+    finally:
+        par.close()
