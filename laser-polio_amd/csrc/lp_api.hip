@@ -545,3 +545,4 @@ void lp_destroy(lp_ctx* c) {
 }
 
 }  // extern "C"
+#include "lp_meanfield.hip"

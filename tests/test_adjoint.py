@@ -1,0 +1,161 @@
+"""Adjoint of the node-level transmission map (mean-field rollout).
+
+CPU: the plain-PyTorch rollout's autograd against central finite differences in
+float64 through the full parameterisation (gravity k, a, b, c; seasonality
+amplitude and peak; r0_scalars), and conservation of agents.  GPU: the HIP
+forward against the plain rollout, the hand-written adjoint against autograd
+of the plain rollout (float32) and against central finite differences of the
+plain rollout in float64, and checkpointing that does not change gradients.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from laser_polio_amd import adjoint as adj
+
+
+def _problem(n=24, T=84, seed=3, dtype=torch.float64, device="cpu"):
+    rng = np.random.default_rng(seed)
+    pop = np.round(rng.lognormal(np.log(2e5), 0.6, n))
+    d = rng.uniform(20, 250, (n, n))
+    d = np.triu(d, 1)
+    d = d + d.T
+    S0 = np.round(pop * rng.uniform(0.2, 0.4, n))
+    I0 = np.zeros(n)
+    I0[:3] = [50, 10, 5]
+    E0 = np.round(I0 * 0.3)
+    t = lambda x: torch.tensor(x, dtype=dtype, device=device)  # noqa: E731
+    const = dict(S0=t(S0), E0=t(E0), I0=t(I0), rho=t(rng.uniform(0.8, 1.2, n)), inv_pop=t(1.0 / pop),
+                 iota=14.0 / 24.0, inv_tau_e=1 / 3.0, inv_tau_i=1 / 24.0)
+    params = dict(k=t(0.5), a=t(1.0), b=t(1.0), c=t(2.0), amp=t(0.4), peak=t(180.0),
+                  r0s=t(rng.lognormal(0.0, 0.3, n)))
+    doy = t(((np.arange(T) + 45) % 365) + 1.0)
+    return pop, d, const, params, doy, T
+
+
+def _loss(params, pop, dist, const, doy, y, fn, max_frac=0.3):
+    dt = params["r0s"].dtype
+    net = adj.gravity_network(torch.as_tensor(pop, dtype=dt, device=params["r0s"].device),
+                              torch.as_tensor(dist, dtype=dt, device=params["r0s"].device),
+                              params["k"], params["a"], params["b"], params["c"], max_frac)
+    season = adj.seasonality(params["amp"], params["peak"], doy, 365.0)
+    inc = fn(params["r0s"], net, season, const["S0"], const["E0"], const["I0"], const["rho"], const["inv_pop"],
+             const["iota"], const["inv_tau_e"], const["inv_tau_i"])
+    return adj.poisson_nll(adj.weekly(inc), y)
+
+
+def _target(pop, d, const, params, doy):
+    with torch.no_grad():
+        p2 = {k: v * (1.1 if k in ("r0s", "k") else 1.0) for k, v in params.items()}
+        net = adj.gravity_network(torch.as_tensor(pop, dtype=torch.float64), torch.as_tensor(d, dtype=torch.float64),
+                                  p2["k"].double().cpu(), p2["a"].double().cpu(), p2["b"].double().cpu(),
+                                  p2["c"].double().cpu(), 0.3)
+        season = adj.seasonality(p2["amp"].double().cpu(), p2["peak"].double().cpu(), doy.double().cpu(), 365.0)
+        c = {k: (v.double().cpu() if torch.is_tensor(v) else v) for k, v in const.items()}
+        inc = adj.rollout_reference(p2["r0s"].double().cpu(), net, season, c["S0"], c["E0"], c["I0"], c["rho"],
+                                    c["inv_pop"], c["iota"], c["inv_tau_e"], c["inv_tau_i"])
+        return torch.round(adj.weekly(inc))
+
+
+def _fd(params, name, idx, h_rel, loss_fn):
+    base = {k: v.detach().clone() for k, v in params.items()}
+    v = base[name]
+    x0 = float(v.reshape(-1)[idx])
+    h = h_rel * max(abs(x0), 1.0)
+    plus = {k: t.clone() for k, t in base.items()}
+    minus = {k: t.clone() for k, t in base.items()}
+    plus[name].reshape(-1)[idx] += h
+    minus[name].reshape(-1)[idx] -= h
+    return (float(loss_fn(plus)) - float(loss_fn(minus))) / (2 * h)
+
+
+def test_reference_autograd_matches_central_differences():
+    pop, d, const, params, doy, T = _problem()
+    y = _target(pop, d, const, params, doy)
+    p = {k: v.clone().requires_grad_(True) for k, v in params.items()}
+    L = _loss(p, pop, d, const, doy, y, adj.rollout_reference)
+    L.backward()
+    f = lambda q: _loss(q, pop, d, const, doy, y, adj.rollout_reference)  # noqa: E731
+    for name, idx in [("k", 0), ("a", 0), ("b", 0), ("c", 0), ("amp", 0), ("peak", 0), ("r0s", 0), ("r0s", 7)]:
+        fd = _fd(params, name, idx, 1e-6, f)
+        g = float(p[name].grad.reshape(-1)[idx])
+        assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, idx, g, fd)
+
+
+def test_reference_conserves_agents():
+    pop, d, const, params, doy, T = _problem()
+    net = adj.gravity_network(torch.as_tensor(pop), torch.as_tensor(d), params["k"], params["a"], params["b"],
+                              params["c"], 0.3)
+    season = adj.seasonality(params["amp"], params["peak"], doy, 365.0)
+    inc = adj.rollout_reference(params["r0s"], net, season, const["S0"], const["E0"], const["I0"], const["rho"],
+                                const["inv_pop"], const["iota"], const["inv_tau_e"], const["inv_tau_i"])
+    assert torch.all(inc >= 0)
+    assert torch.all(inc.sum(0) <= const["S0"] + 1e-9)
+
+
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+@pytest.mark.gpu
+def test_hip_forward_matches_reference():
+    dev = _gpu()
+    print("[adjoint] forward", flush=True)
+    pop, d, const, params, doy, T = _problem(n=60, T=120, dtype=torch.float32, device=dev)
+    net = adj.gravity_network(torch.as_tensor(pop, dtype=torch.float32, device=dev),
+                              torch.as_tensor(d, dtype=torch.float32, device=dev),
+                              params["k"], params["a"], params["b"], params["c"], 0.3)
+    season = adj.seasonality(params["amp"], params["peak"], doy, 365.0)
+    args = (params["r0s"], net, season, const["S0"], const["E0"], const["I0"], const["rho"], const["inv_pop"],
+            const["iota"], const["inv_tau_e"], const["inv_tau_i"])
+    got = adj.rollout(*args)
+    want = adj.rollout_reference(*[a.double() if torch.is_tensor(a) else a for a in args]).float()
+    rel = ((got - want).abs().max() / want.abs().max()).item()
+    assert rel < 1e-5, rel
+
+
+@pytest.mark.gpu
+def test_hip_adjoint_matches_autograd_and_central_differences():
+    dev = _gpu()
+    print("[adjoint] gradients", flush=True)
+    pop, d, const32, params32, doy32, T = _problem(n=60, T=120, dtype=torch.float32, device=dev)
+    y = _target(pop, d, const32, params32, doy32).to(dev).float()
+    p_hip = {k: v.clone().requires_grad_(True) for k, v in params32.items()}
+    _loss(p_hip, pop, d, const32, doy32, y, adj.rollout).backward()
+    p_ag = {k: v.clone().requires_grad_(True) for k, v in params32.items()}
+    _loss(p_ag, pop, d, const32, doy32, y, adj.rollout_reference).backward()
+    for name in params32:
+        g, r = p_hip[name].grad.double(), p_ag[name].grad.double()
+        rel = ((g - r).norm() / r.norm().clamp_min(1e-12)).item()
+        assert rel < 1e-4, (name, rel)
+    # central differences of the float64 rollout
+    pop_, d_, c64, p64, doy64, _ = _problem(n=60, T=120, dtype=torch.float64, device="cpu")
+    y64 = y.double().cpu()
+    f = lambda q: _loss(q, pop_, d_, c64, doy64, y64, adj.rollout_reference)  # noqa: E731
+    worst = 0.0
+    for name, idx in [("k", 0), ("a", 0), ("b", 0), ("c", 0), ("amp", 0), ("peak", 0), ("r0s", 0), ("r0s", 1), ("r0s", 2)]:
+        fd = _fd(p64, name, idx, 1e-4, f)
+        g = float(p_hip[name].grad.reshape(-1)[idx])
+        worst = max(worst, abs(g - fd) / max(abs(fd), 1e-3))
+    print(f"[adjoint] worst relative error vs central differences: {worst:.3e}", flush=True)
+    assert worst < 1e-4, worst
+
+
+@pytest.mark.gpu
+def test_checkpoint_length_does_not_change_gradients():
+    dev = _gpu()
+    print("[adjoint] checkpoints", flush=True)
+    pop, d, const, params, doy, T = _problem(n=40, T=90, dtype=torch.float32, device=dev)
+    y = _target(pop, d, const, params, doy).to(dev).float()
+    grads = []
+    for ck in (7, 30, 1000):
+        p = {k: v.clone().requires_grad_(True) for k, v in params.items()}
+        fn = lambda *a, ck=ck: adj.rollout(*a, checkpoint=ck)  # noqa: E731
+        _loss(p, pop, d, const, doy, y, fn).backward()
+        grads.append({k: v.grad.clone() for k, v in p.items()})
+    for g in grads[1:]:
+        for k in g:
+            assert torch.equal(g[k], grads[0][k]), k

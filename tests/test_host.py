@@ -7,6 +7,7 @@ mirror.
 """
 
 import ctypes
+import glob
 import os
 import re
 import subprocess
@@ -21,11 +22,14 @@ from laser_polio_amd.pars import default_run_order
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "lp_step.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def _declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(lp_\w+)\(", text, re.M)))
+    names = set()
+    for h in HEADERS:
+        names |= set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(lp_\w+)\(", open(h).read(), re.M))
+    return sorted(names)
 
 
 def _lib():
@@ -40,6 +44,8 @@ def test_header_declares_the_expected_api():
               "lp_step_disease", "lp_step_ri", "lp_step_sia", "lp_step_transmission", "lp_log", "lp_sync",
               "lp_stop_days", "lp_destroy", "lp_last_error"):
         assert f in names
+    for f in ("lp_mf_days", "lp_mf_adjoint_days"):
+        assert f in names
 
 
 def test_library_exports_every_declared_symbol():
@@ -49,8 +55,9 @@ def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", em.LIB_PATH], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r"\bT (lp_\w+)$", out, re.M))
     assert set(_declared()) <= exported
-    # nothing C++-mangled leaks into the boundary
-    assert not [s for s in re.findall(r"\bT (\S+)$", out, re.M) if s.startswith("_Z") and "lp_" in s]
+    # no entry point of the boundary is exported C++-mangled
+    mangled = [s for s in re.findall(r"\b[TD] (\S+)$", out, re.M) if s.startswith("_Z")]
+    assert not [m for m in mangled for name in _declared() if name in m]
 
 
 def test_abi_version_and_struct_layout():
