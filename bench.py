@@ -112,9 +112,10 @@ def main():
     ap.add_argument("--dump-days", default="", help="write per-day N_live / N_EI / N_draw / flags to this CSV")
     ap.add_argument("--graph", type=int, default=0, help="1: replay one hipGraph per day (no per-kernel events)")
     ap.add_argument("--tiles-per-block", type=int, default=0)
-    ap.add_argument("--workload", choices=["national", "replicates"], default="national",
+    ap.add_argument("--workload", choices=["national", "replicates", "adjoint"], default="national",
                     help="national: one 774-node simulation (default); replicates: --sims independent "
-                         "simulations of uneven sizes batched in one context, against serial runs of a sample")
+                         "simulations of uneven sizes batched in one context, against serial runs of a sample; "
+                         "adjoint: gradients of a mean-field calibration loss on the national geography")
     ap.add_argument("--sims", type=int, default=512, help="replicates: simulations in the batch")
     ap.add_argument("--serial-sample", type=int, default=16, help="replicates: simulations also run one at a time")
     ap.add_argument("--workers", type=int, default=16, help="replicates: host processes initialising simulations")
@@ -141,6 +142,9 @@ def main():
 
     if args.workload == "replicates":
         replicates(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_workload(args)
         return
 
     import torch
@@ -366,6 +370,122 @@ def replicates(args):
         "serial_sample": {"sims": len(times), "seconds": round(float(np.sum(times)), 4),
                           "fit_us_per_day": round(float(coef[0]) * 1e6, 2),
                           "fit_ps_per_agent_day": round(float(coef[1]) * 1e12, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_workload(args):
+    """Gradients of the mean-field calibration loss on the national geography at `--scale`
+    (0.1 when left at 1.0: 774 nodes, about 2.3e7 agents): node compartments aggregated from
+    the agent population at day 0, 180 days rolled forward, Poisson negative log-likelihood
+    of weekly per-node incidence against the weekly transmission incidence of the stochastic
+    ABM run on the GPU at the known parameters; gradient at perturbed parameters (r0_scalars
+    x 0.9, gravity k x 0.8, amplitude x 1.2) with respect to the 774 r0_scalars, gravity
+    k, a, b, c and seasonal amplitude and peak.  Reported: seconds per loss + gradient
+    (HIP forward and adjoint, 30-day checkpoints), relative error against autograd of the
+    plain float32 PyTorch rollout (per parameter group) and against float64 central
+    differences (h = 1e-4 relative) on scalars and a sample of r0_scalars."""
+    import torch
+
+    __graft_entry__.load_package()
+    from laser_polio_amd import adjoint as adj
+    from laser_polio_amd import configs
+    import calendar
+
+    scale = 0.1 if args.scale == 1.0 else args.scale
+    T = 180
+    pars = configs.nigeria_national_pars(scale=scale, dur=T)
+    sim = configs.build(pars)
+    agg = adj.aggregate(sim)
+    tx = sim._instance("Transmission_ABM")
+    rp = sim.pars
+    doy = np.array([sim.datevec[t].timetuple().tm_yday for t in range(1, T + 1)], dtype=np.float64)
+    diy = np.array([366.0 if calendar.isleap(sim.datevec[t].year) else 365.0 for t in range(1, T + 1)])
+    dist = np.asarray(rp.distances, dtype=np.float64)
+    log(f"[bench] adjoint: {len(rp.init_pop)} nodes, {int(rp.init_pop.sum())} agents (scale {scale}), {T} days")
+    t0 = time.perf_counter()
+    sim.run()
+    y_daily = (sim.results.new_exposed.astype(np.int64) - sim.results.ri_protected - sim.results.sia_protected)[1:T + 1]
+    log(f"[bench] ABM target: {int(y_daily.sum())} transmission exposures in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda", 0)
+
+    def tensors(dtype, device):
+        t = lambda x: torch.as_tensor(np.asarray(x, dtype=np.float64), dtype=dtype, device=device)  # noqa: E731
+        const = {k: t(agg[k]) for k in ("S", "E", "I", "rho", "inv_pop")}
+        params = {"r0s": t(np.asarray(tx.r0_scalars, dtype=np.float64) * 0.9), "k": t(rp.gravity_k * 0.8),
+                  "a": t(rp.gravity_a), "b": t(rp.gravity_b), "c": t(rp.gravity_c),
+                  "amp": t(rp.seasonal_amplitude * 1.2), "peak": t(float(rp.seasonal_peak_doy))}
+        return const, params, t(doy), t(diy), t(rp.init_pop), t(dist)
+
+    def loss(params, const, doy_t, diy_t, pop_t, dist_t, y, fn):
+        net = adj.gravity_network(pop_t, dist_t, params["k"], params["a"], params["b"], params["c"],
+                                  float(rp.max_migr_frac), float(rp.gravity_k_exponent))
+        season = adj.seasonality(params["amp"], params["peak"], doy_t, diy_t)
+        inc = fn(params["r0s"], net, season, const["S"], const["E"], const["I"], const["rho"], const["inv_pop"],
+                 agg["iota"], agg["inv_tau_e"], agg["inv_tau_i"])
+        return adj.poisson_nll(adj.weekly(inc), y)
+
+    c32, p32, doy32, diy32, pop32, dist32 = tensors(torch.float32, dev)
+    y32 = adj.weekly(torch.as_tensor(y_daily, dtype=torch.float32, device=dev))
+
+    def grad_hip():
+        p = {k: v.clone().requires_grad_(True) for k, v in p32.items()}
+        L = loss(p, c32, doy32, diy32, pop32, dist32, y32, adj.rollout)
+        L.backward()
+        return float(L), {k: v.grad.detach() for k, v in p.items()}
+
+    grad_hip()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        L_hip, g_hip = grad_hip()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    p = {k: v.clone().requires_grad_(True) for k, v in p32.items()}
+    t0 = time.perf_counter()
+    L_ag = loss(p, c32, doy32, diy32, pop32, dist32, y32, adj.rollout_reference)
+    L_ag.backward()
+    torch.cuda.synchronize()
+    t_ag = time.perf_counter() - t0
+    rel_ag = {k: float((g_hip[k].double() - p[k].grad.double()).norm() / p[k].grad.double().norm().clamp_min(1e-30))
+              for k in p32}
+    c64, p64, doy64, diy64, pop64, dist64 = tensors(torch.float64, "cpu")
+    y64 = y32.double().cpu()
+    f64 = lambda q: float(loss(q, c64, doy64, diy64, pop64, dist64, y64, adj.rollout_reference))  # noqa: E731
+    fd_err = {}
+    for name, idx in [("k", 0), ("a", 0), ("b", 0), ("c", 0), ("amp", 0), ("peak", 0)] + [("r0s", i) for i in (0, 100, 400)]:
+        base = {k: v.clone() for k, v in p64.items()}
+        x0 = float(base[name].reshape(-1)[idx])
+        h = 1e-4 * max(abs(x0), 1.0)
+        plus = {k: v.clone() for k, v in base.items()}
+        minus = {k: v.clone() for k, v in base.items()}
+        plus[name].reshape(-1)[idx] += h
+        minus[name].reshape(-1)[idx] -= h
+        fd = (f64(plus) - f64(minus)) / (2 * h)
+        g = float(g_hip[name].reshape(-1)[idx])
+        fd_err[f"{name}[{idx}]" if name == "r0s" else name] = abs(g - fd) / max(abs(fd), 1e-12)
+    ms = float(np.median(times)) * 1e3
+    log(f"[bench] loss {L_hip:.6e}; HIP forward+adjoint {ms:.2f} ms (autograd of plain float32 rollout {t_ag * 1e3:.1f} ms)")
+    log(f"[bench] relative error vs autograd: {max(rel_ag.values()):.2e}; vs central differences: {max(fd_err.values()):.2e}")
+    out = {
+        "metric": "mean-field adjoint: loss + gradient time (774 nodes, 180 days, 30-day checkpoints)",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 1,
+        "ms_per_step": round(ms / T, 4),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"adjoint: national geography at scale {scale}, {len(rp.init_pop)} nodes, "
+                               f"{int(rp.init_pop.sum())} agents aggregated at day 0, {T} days mean-field, ABM target"},
+        "autograd_ms": round(t_ag * 1e3, 2),
+        "rel_err_vs_autograd": {k: float(f"{v:.3e}") for k, v in rel_ag.items()},
+        "rel_err_vs_central_fd": {k: float(f"{v:.3e}") for k, v in fd_err.items()},
     }
     print(json.dumps(out), flush=True)
 
