@@ -386,6 +386,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.potp_run, N))) return rc;
   if ((rc = alloc(c, &d.p_run, N))) return rc;
   if ((rc = alloc(c, &d.node_carry, (size_t)N * 3))) return rc;
+  if ((rc = alloc(c, &d.chunk_min_dod, nch))) return rc;
   if ((rc = alloc(c, &d.p_dbg, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.expected_dbg, N))) return rc;

@@ -126,6 +126,11 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) { const int32_t u = __shfl_xor(v, o, 64); v = u < v ? u : v; }
+  return v;
+}
 // Inclusive prefix sum across the wave (Hillis-Steele on 64 lanes).
 __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane) {
 #pragma unroll

@@ -79,6 +79,7 @@ struct Dev {
   int64_t* nb_begin; int32_t* births;
   int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
   int64_t* node_carry;   // [nodes][3] carried counts: S, S with positive weight, R
+  int32_t* chunk_min_dod;  // [chunks] earliest date_of_death among the chunk's live agents
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -216,6 +217,12 @@ __device__ __forceinline__ bool is_plain_day(int32_t flags) {
 __device__ __forceinline__ bool has_ei_byte(uint32_t w) {
   const uint32_t x1 = w ^ 0x01010101u, x2 = w ^ 0x02020202u;
   return ((((x1 - 0x01010101u) & ~x1) | ((x2 - 0x01010101u) & ~x2)) & 0x80808080u) != 0;
+}
+
+// True if any byte of the packed state word equals v (same zero-byte test).
+__device__ __forceinline__ bool has_byte(uint32_t w, uint32_t v) {
+  const uint32_t x = w ^ (v * 0x01010101u);
+  return (((x - 0x01010101u) & ~x) & 0x80808080u) != 0;
 }
 
 // Position of the j-th (0-based) set bit of m; m has more than j set bits.
@@ -383,7 +390,7 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
     if (su < 0) continue;
     const int qt = wave * 64 + src;
     const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
-    const uint2 xe = *reinterpret_cast<const uint2*>(d.exp_timer + i0);
+    const uint2 xe = has_byte(s_st[su][qt], 1) ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
     const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
     const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
     const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
@@ -441,10 +448,15 @@ enum : int { FULL_VD = 1, FULL_RI = 2, FULL_SIA = 4 };
 
 template <int F>
 __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
-  constexpr int G = 4;
+  // tiles per group: RI days carry ri_timer and chronically_missed per tile, SIA days date_of_birth too
+  constexpr int G = (F & FULL_SIA) ? 2 : (F & FULL_RI) ? 4 : 8;
   constexpr bool VD = F & FULL_VD, RI = F & FULL_RI, SIA = F & FULL_SIA;
   const int4 span0 = d.spans[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);   // the same value, known uniform
+  // earliest death date of each of the span's chunks (<= 64): lane l holds chunk l
+  const int32_t cm_all = (VD && lane < span0.z * (LP_TILE / LP_CHUNK))
+                             ? d.chunk_min_dod[(int64_t)span0.y * (LP_TILE / LP_CHUNK) + lane] : INT_MAX;
   const int32_t t = d.day[0];
   const int node = span0.x, sim = span0.w;
   const int64_t live_hi = d.live_end[node];
@@ -490,15 +502,17 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
     const int tile0 = span0.y + g0;
     const int ng = span0.z - g0 < G ? span0.z - g0 : G;
     // ---- dense loads ----
+    // date_of_death is read only in chunks whose earliest death date among live agents is due
     uint32_t st[G], mw[G];
-    int4 dd[G], db[G];
+    int4 db[G];
     uint2 rt[G];
+    int32_t cmin[G];
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
       const bool v = (u < ng) && (i0 < live_hi);
       st[u] = (u < ng) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
-      if (VD) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+      cmin[u] = (VD && u < ng) ? __builtin_amdgcn_readlane(cm_all, (g0 + u) * (LP_TILE / LP_CHUNK) + wave_u) : INT_MAX;
       if (camp) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
       if (RI) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
       if (SIA) db[u] = (v && sia_here) ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
@@ -513,10 +527,51 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
       const uint32_t w0 = st[u];
       uint32_t w1 = w0, w2 = w0;
+      // death dates only in the (rare) chunks with a death due; the branch is uniform per wave
+      int4 ddu = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+      if (VD && cmin[u] <= t && u < ng && i0 < live_hi) ddu = *reinterpret_cast<const int4*>(d.dod + i0);
       uint2 r2 = RI ? rt[u] : make_uint2(0u, 0u);
       int64_t cw = 0;
+      int32_t dmin = INT_MAX;   // earliest death date among the quad's agents alive after today
       bool q = false;
-      if (u < ng && i0 < live_hi) {
+      // Fast path for the common quad: all four slots in the live range, no newborn, no
+      // death due, no SIA here today and no agent entering its RI window; the only change
+      // is then RI's timer decrement.  Anything else takes the per-agent path below.
+      bool fast = false;
+      if (u < ng && i0 + 3 < live_hi && !(VD && i0 + 3 >= nb_lo) && !(SIA && sia_here)) {
+        const uint32_t alive = ~w0 & 0x80808080u;        // bit 8a+7 set <=> agent a has state >= 0
+        bool ok = true;
+        int32_t qmin = INT_MAX;
+        if (VD && cmin[u] <= t) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int32_t dod = lane_of(ddu, a);
+            if ((alive >> (8 * a + 7)) & 1u) { ok = ok && dod > t; qmin = dod < qmin ? dod : qmin; }
+          }
+        }
+        uint2 rn = r2;
+        if (RI && ok) {
+          const int32_t step = sp.ri_step;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            if (!((alive >> (8 * a + 7)) & 1u) || byte_of(mw[u], a) == 1) continue;
+            const int sh16 = 16 * (a & 1);
+            const uint32_t rw = a < 2 ? rn.x : rn.y;
+            const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - step);
+            const bool eligible = (t == step) ? (timer <= 0 && timer >= -step) : (t > step && timer <= 0 && timer > -step);
+            ok = ok && !eligible;
+            const uint32_t nw = (rw & ~(0xffffu << sh16)) | ((uint32_t)(uint16_t)timer << sh16);
+            if (a < 2) rn.x = nw; else rn.y = nw;
+          }
+        }
+        if (ok) {
+          fast = true;
+          dmin = qmin;
+          if (RI) r2 = rn;
+          q = has_ei_byte(w0);
+        }
+      }
+      if (!fast && u < ng && i0 < live_hi) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int64_t i = i0 + a;
@@ -527,8 +582,10 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
           const uint32_t m16 = 0xffffu << sh16;
           if (VD) {
             if (i >= nb_lo) {                             // newborn (model.py:1719-1732)
+              const int32_t dod = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
               d.dob[i] = t;
-              d.dod[i] = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
+              d.dod[i] = dod;
+              dmin = dod < dmin ? dod : dmin;
               d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
               if (sp.newborn_ri_timer >= 0) {
                 d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
@@ -541,7 +598,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               const int64_t w = weight_q24(d.risk[i]);     // a newborn enters S
               c_s += 1; c_spos += (w > 0); c_sw += w; cw += w;
               s = 0;
-            } else if (s >= 0 && lane_of(dd[u], a) <= t) { // get_deaths (model.py:1775-1777)
+            } else if (s >= 0 && lane_of(ddu, a) <= t) { // get_deaths (model.py:1775-1777)
               atomicAdd(&lc[L_DEATHS], 1);
               if (d.pp[i] == 1) atomicAdd(&lc[L_DPP], 1);
               if (d.paralyzed[i] == 1) atomicAdd(&lc[L_DP], 1);
@@ -552,6 +609,9 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                 c_r -= 1;
               }
               s = -1;
+            } else if (s >= 0) {
+              const int32_t dod = lane_of(ddu, a);
+              dmin = dod < dmin ? dod : dmin;
             }
             w1 = (w1 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
           }
@@ -616,6 +676,12 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         if (RI && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
       }
       s_w0[u][tid] = w0; s_w1[u][tid] = w1; s_w2[u][tid] = w2;
+      if (VD && u < ng) {
+        // the chunk's earliest death date: exact when its dates were read, else lowered by newborns
+        const int32_t wm = wave_min_i32(dmin);
+        const int32_t nm = cmin[u] <= t ? wm : (wm < cmin[u] ? wm : cmin[u]);
+        if (lane == 0 && nm != cmin[u]) d.chunk_min_dod[((int64_t)tile0 + u) * (LP_TILE / LP_CHUNK) + wave_u] = nm;
+      }
       // chunk weight delta (uniform condition: wave_sum needs every lane)
       if (u < ng && ((int64_t)tile0 + u) * LP_TILE < live_hi) {
         const int64_t cs = wave_sum_i64(cw);
@@ -638,7 +704,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (su < 0) continue;
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-      const uint2 xe = *reinterpret_cast<const uint2*>(d.exp_timer + i0);
+      const uint2 xe = has_byte(s_w1[su][qt], 1) ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
       const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
       const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
       const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
@@ -760,7 +826,7 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
       st[u] = (u < span.z) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
       st0[u] = st[u];
       if (RECOUNT) rk[u] = v ? *reinterpret_cast<const float4*>(d.risk + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-      if (do_deaths) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+      if (do_deaths || RECOUNT) dd[u] = v ? *reinterpret_cast<const int4*>(d.dod + i0) : make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
       if (do_ri || do_sia) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
       if (do_ri) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
       if (do_sia) db[u] = v ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
@@ -827,6 +893,27 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
       }
       ei[u] = any && need_ei;
     });
+    if (RECOUNT) {
+      // earliest death date among each chunk's live agents (date_of_death of newborns just written)
+      static_for<G>([&](auto uc) {
+        constexpr int u = decltype(uc)::value;
+        const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+        int32_t dmin = INT_MAX;
+        if (tv[u]) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int64_t i = i0 + a;
+            if (i >= live_hi || byte_of(st[u], a) < 0) continue;
+            const int32_t dod = i >= nb_lo ? d.dod[i] : lane_of(dd[u], a);
+            dmin = dod < dmin ? dod : dmin;
+          }
+        }
+        if (u < span.z) {
+          const int32_t wm = wave_min_i32(dmin);
+          if (lane == 0) d.chunk_min_dod[((int64_t)span.y + u) * (LP_TILE / LP_CHUNK) + wave] = wm;
+        }
+      });
+    }
 
     // ---- phase C: E/I columns of quads holding an E or I agent ----
     uint2 xe[G], xi[G], xp[G];
