@@ -1497,8 +1497,9 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
 }
 
 // S -> E of a selected agent with its strain drawn by cumulative strain
-// probability (model.py:1125-1147); counted in the node's new exposures.
-__device__ __forceinline__ void expose_agent(const Dev& d, int node, const lp_sim_params& sp, int64_t slot, int32_t t) {
+// probability (model.py:1125-1147); returns the strain (the caller counts it).
+__device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, const lp_sim_params& sp, int64_t slot,
+                                                    int32_t t) {
   const uint4 o = draw(sp.seed, (uint64_t)(slot - sp.slot_base), t, ST_STRAIN);
   int k = 0;
   for (int s = 0; s < d.K; ++s) {
@@ -1506,8 +1507,9 @@ __device__ __forceinline__ void expose_agent(const Dev& d, int node, const lp_si
   }
   d.state[slot] = 1;
   d.strain[slot] = (int8_t)k;
-  atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+  return k;
 }
+
 
 // Sampling round 1, grid-wide: one wavefront per probe resolves it and claims
 // the agent.  The probe offsets are staged in LDS when they fit (dynamic shared
@@ -1529,8 +1531,19 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int32_t t = d.day[0];
   const int lane = threadIdx.x & 63;
   const int64_t base = d.round_base[1];
-  for (int64_t p = first + (threadIdx.x >> 6); p < M; p += nw) {
+  // each wave takes a contiguous range of probes (probes are ordered by node), so its
+  // claims are counted locally and added to the node's total once per node change
+  const int64_t per = (M + nw - 1) / nw;
+  const int64_t p0 = (first + (threadIdx.x >> 6)) * per;
+  const int64_t p1 = p0 + per < M ? p0 + per : M;
+  int cur_node = -1, claimed = 0;
+  for (int64_t p = p0; p < p1; ++p) {
     const int node = (int)find_node(off, N, p);
+    if (node != cur_node) {
+      if (lane == 0 && claimed) atomicAdd(d.n_uniq + cur_node, claimed);
+      cur_node = node;
+      claimed = 0;
+    }
     const int64_t j = p - off[node];
     const lp_sim_params& sp = d.sims[d.node_sim[node]];
     const uint4 o = philox((uint32_t)j, (uint32_t)(node - sp.node_begin), (uint32_t)t, ST_PROBE | (1u << 8), sp.seed);
@@ -1541,10 +1554,11 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
     if (lane == 0) {
       uint64_t e = SEL_INVALID;
       if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-      else if (claim(d.state, slot)) { atomicAdd(d.n_uniq + node, 1); e = (uint64_t)slot | ((uint64_t)node << 40); }
+      else if (claim(d.state, slot)) { claimed += 1; e = (uint64_t)slot | ((uint64_t)node << 40); }
       d.sel_list[base + p] = e;
     }
   }
+  if (lane == 0 && claimed) atomicAdd(d.n_uniq + cur_node, claimed);
 }
 
 // Grid-wide, after round 1: every agent claimed in round 1 loses its weight and
@@ -1553,14 +1567,38 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   const int32_t t = d.day[0];
   const int64_t M = d.probe_off[d.n_nodes];
   const int64_t base = d.round_base[1];
-  for (int64_t p = (int64_t)blockIdx.x * WG + threadIdx.x; p < M; p += (int64_t)gridDim.x * WG) {
-    const uint64_t e = d.sel_list[base + p];
-    if (e == SEL_INVALID) continue;
-    const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-    const int node = (int)(e >> 40);
-    if (node >= d.n_nodes || slot >= d.live_end[node] || slot < d.seg_begin[node]) { atomicOr(d.err, ERR_SAMPLING); continue; }
-    unweight(d, node, slot);
-    expose_agent(d, node, d.sims[d.node_sim[node]], slot, t);
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * WG;
+  for (int64_t p0 = (int64_t)blockIdx.x * WG + (threadIdx.x & ~63); p0 < M; p0 += stride) {
+    const int64_t p = p0 + lane;
+    int node = -1, k = -1;
+    if (p < M) {
+      const uint64_t e = d.sel_list[base + p];
+      if (e != SEL_INVALID) {
+        const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+        const int nd = (int)(e >> 40);
+        if (nd >= d.n_nodes || slot >= d.live_end[nd] || slot < d.seg_begin[nd]) {
+          atomicOr(d.err, ERR_SAMPLING);
+        } else {
+          node = nd;
+          unweight(d, node, slot);
+          k = expose_agent_nocount(d, node, d.sims[d.node_sim[node]], slot, t);
+        }
+      }
+    }
+    // new exposures by strain: one atomic per strain when the wave's entries share a node
+    const int n0 = __shfl(node, __ffsll((unsigned long long)__ballot(node >= 0)) - 1, 64);
+    const bool same = __ballot(node >= 0 && node != n0) == 0;
+    if (same) {
+      if (__ballot(node >= 0)) {
+        for (int s = 0; s < d.K; ++s) {
+          const int c = __popcll(__ballot(k == s));
+          if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
+        }
+      }
+    } else if (node >= 0) {
+      atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+    }
   }
 }
 
@@ -1573,71 +1611,80 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int64_t sh_base;
   __shared__ int64_t sh_W;
+  __shared__ int32_t s_ne[LP_MAX_STRAINS];   // the node's new exposures by strain (one global add each)
+  __shared__ int32_t s_claimed;
   if (d.size[n] == 0) return;
   const lp_sim_params& sp = d.sims[d.node_sim[n]];
+  if (tid < LP_MAX_STRAINS) s_ne[tid] = 0;
+  if (tid == 0) s_claimed = 0;
+  __syncthreads();
   if (d.take_all[n]) {
     const int64_t lo = d.seg_begin[n], hi = d.live_end[n];
     for (int64_t i = lo + tid; i < hi; i += WG) {
       if (d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
-        expose_agent(d, n, sp, i, t);
+        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, i, t)], 1);
       }
     }
-    return;
+  } else {
+    const int32_t uniq0 = d.n_uniq[n];
+    const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
+    for (int round = 2;; ++round) {
+      if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); break; }
+      const int32_t need = d.size[n] - (uniq0 + s_claimed);
+      if (need <= 0) break;
+      const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
+      if (W <= 0) break;
+      if (tid == 0) {
+        const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
+        if (b + need > d.sel_cap) {
+          atomicOr(d.err, ERR_OVERFLOW);
+          sh_base = -1;
+        } else sh_base = b;
+        sh_W = W;
+      }
+      __syncthreads();
+      const int64_t base = sh_base;
+      if (base < 0) break;
+      for (int j = wave; j < need; j += WG / 64) {
+        const uint4 o = philox((uint32_t)j, (uint32_t)(n - sp.node_begin), (uint32_t)t,
+                               ST_PROBE | ((uint32_t)round << 8), sp.seed);
+        const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+        const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
+        const int64_t slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
+        if (lane == 0) {
+          const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
+          if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
+          __hip_atomic_store(d.sel_list + base + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __threadfence();
+      __syncthreads();
+      for (int j = tid; j < need; j += WG) {
+        const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e == SEL_INVALID) continue;
+        const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+        if (claim(d.state, slot)) {
+          unweight(d, n, slot);
+          atomicAdd(&s_claimed, 1);
+        } else {
+          __hip_atomic_store(d.sel_list + base + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __threadfence();
+      __syncthreads();
+      for (int j = tid; j < need; j += WG) {
+        const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e == SEL_INVALID) continue;
+        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, (int64_t)(e & ((1ull << 40) - 1)), t)], 1);
+      }
+      __threadfence();
+      __syncthreads();
+    }
   }
-  const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
-  for (int round = 2;; ++round) {
-    if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); return; }
-    const int32_t need = d.size[n] - __hip_atomic_load(d.n_uniq + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (need <= 0) return;
-    const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
-    if (W <= 0) return;
-    if (tid == 0) {
-      const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
-      if (b + need > d.sel_cap) {
-        atomicOr(d.err, ERR_OVERFLOW);
-        sh_base = -1;
-      } else sh_base = b;
-      sh_W = W;
-    }
-    __syncthreads();
-    const int64_t base = sh_base;
-    if (base < 0) return;
-    for (int j = wave; j < need; j += WG / 64) {
-      const uint4 o = philox((uint32_t)j, (uint32_t)(n - sp.node_begin), (uint32_t)t,
-                             ST_PROBE | ((uint32_t)round << 8), sp.seed);
-      const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
-      const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
-      const int64_t slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
-      if (lane == 0) {
-        const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
-        if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-        __hip_atomic_store(d.sel_list + base + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    __threadfence();
-    __syncthreads();
-    for (int j = tid; j < need; j += WG) {
-      const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (e == SEL_INVALID) continue;
-      const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-      if (claim(d.state, slot)) {
-        unweight(d, n, slot);
-        atomicAdd(d.n_uniq + n, 1);
-      } else {
-        __hip_atomic_store(d.sel_list + base + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    __threadfence();
-    __syncthreads();
-    for (int j = tid; j < need; j += WG) {
-      const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (e == SEL_INVALID) continue;
-      expose_agent(d, n, sp, (int64_t)(e & ((1ull << 40) - 1)), t);
-    }
-    __threadfence();
-    __syncthreads();
-  }
+  __syncthreads();
+  if (tid < LP_MAX_STRAINS && s_ne[tid]) atomicAdd(d.newexp + n * LP_MAX_STRAINS + tid, s_ne[tid]);
+  if (tid == 0 && s_claimed) atomicAdd(d.n_uniq + n, s_claimed);
 }
 
 __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_adjust) {
