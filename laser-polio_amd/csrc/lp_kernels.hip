@@ -287,6 +287,62 @@ __device__ __forceinline__ bool progress_agent(int a, int8_t& s, int strain, uin
   return to_r;
 }
 
+// One compacted quad holding E or I agents: loads the columns its agents need,
+// runs their disease progression (model.py:419-452) and tallies them as E / I;
+// returns the quad's state word.  `wei`: state bytes that identify today's E and
+// I agents; `w`: the word to update and write back if it differs from `w_orig`.
+// The paralysis columns are read only when a strain-0 agent is or becomes
+// infectious today (the second, dependent load).
+__device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
+                                        uint32_t w_orig, bool do_ds, bool do_tally, const lp_sim_params& sp,
+                                        int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r) {
+  const int64_t sbase = sp.slot_base;
+  const bool has_e = has_byte(wei, 1);
+  const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
+  const uint2 xe = has_e ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
+  const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
+  const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+  bool par = false;
+  if (do_ds) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int8_t sa = byte_of(wei, a);
+      const uint32_t eh = ((a < 2 ? xe.x : xe.y) >> (16 * (a & 1))) & 0xffffu;
+      par = par || (byte_of(xs, a) == 0 && (sa == 2 || (sa == 1 && eh == 0)) && i0 + a < live_hi);
+    }
+  }
+  const uint2 xp = par ? *reinterpret_cast<const uint2*>(d.par_timer + i0) : make_uint2(0u, 0u);
+  const uint32_t xpp = par ? reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2] : 0u;
+  const uint32_t xipv = par ? reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2] : 0u;
+  uint2 e2 = xe, i2 = xi, p2 = xp;
+  uint32_t ppw = xpp;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + a;
+    if (i >= live_hi) continue;
+    int8_t sa = byte_of(wei, a);
+    if (sa != 1 && sa != 2) continue;
+    const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
+    if (do_ds && progress_agent(a, sa, byte_of(xs, a), e2, i2, p2, ppw, xipv, sp.seed, (uint64_t)(i - sbase), t,
+                                sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
+      c_r += 1;
+    if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
+      if (sa == 1) {
+        atomicAdd(&lc[L_E0 + k], 1);
+      } else if (sa == 2) {
+        atomicAdd(&lc[L_I0 + k], 1);
+        atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
+      }
+    }
+    w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
+  }
+  if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+  if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
+  if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
+  if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
+  if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+}
+
 // Writes the span partial: carried-count deltas from red[][], tallies from LDS.
 __device__ __forceinline__ void write_partial(const Dev& d, int tid, const int64_t (*red)[4], const int32_t* lc,
                                               const unsigned long long* lib, bool recount, bool sw_delta) {
@@ -390,42 +446,8 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
     if (su < 0) continue;
     const int qt = wave * 64 + src;
     const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
-    const uint2 xe = has_byte(s_st[su][qt], 1) ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
-    const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
-    const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
-    const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
-    const uint32_t xpp = reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2];
-    const uint32_t xipv = reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2];
-    const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t w0 = s_st[su][qt];
-    uint32_t w = w0;
-    uint2 e2 = xe, i2 = xi, p2 = xp;
-    uint32_t ppw = xpp;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int64_t i = i0 + a;
-      if (i >= live_hi) continue;
-      int8_t s = byte_of(w, a);
-      if (s != 1 && s != 2) continue;
-      const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
-      if (do_ds && progress_agent(a, s, byte_of(xs, a), e2, i2, p2, ppw, xipv, seed, (uint64_t)(i - sbase), t,
-                                  sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
-        c_r += 1;
-      if (do_tally) {                                   // tx_step_prep / count_SEIRP tally of E and I
-        if (s == 1) {
-          atomicAdd(&lc[L_E0 + k], 1);
-        } else if (s == 2) {
-          atomicAdd(&lc[L_I0 + k], 1);
-          atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
-        }
-      }
-      w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
-    }
-    if (w != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
-    if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
-    if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
-    if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
-    if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+    ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, sp, t, lc, lib, c_r);
   }
   {
     const int32_t a2 = wave_sum_i32(c_r);
@@ -704,42 +726,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (su < 0) continue;
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-      const uint2 xe = has_byte(s_w1[su][qt], 1) ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
-      const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
-      const uint2 xp = *reinterpret_cast<const uint2*>(d.par_timer + i0);
-      const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
-      const uint32_t xpp = reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2];
-      const uint32_t xipv = reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2];
-      const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-      const uint32_t w0 = s_w0[su][qt], w1 = s_w1[su][qt];
-      uint32_t w = s_w2[su][qt];
-      uint2 e2 = xe, i2 = xi, p2 = xp;
-      uint32_t ppw = xpp;
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int64_t i = i0 + a;
-        if (i >= live_hi) continue;
-        int8_t s = byte_of(w1, a);
-        if (s != 1 && s != 2) continue;
-        const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
-        if (do_ds && progress_agent(a, s, byte_of(xs, a), e2, i2, p2, ppw, xipv, seed, (uint64_t)(i - sbase), t,
-                                    sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
-          c_r += 1;
-        if (do_tally) {
-          if (s == 1) {
-            atomicAdd(&lc[L_E0 + k], 1);
-          } else if (s == 2) {
-            atomicAdd(&lc[L_I0 + k], 1);
-            atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
-          }
-        }
-        w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
-      }
-      if (w != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
-      if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
-      if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
-      if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
-      if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+      ei_quad(d, i0, live_hi, s_w1[su][qt], s_w2[su][qt], s_w0[su][qt], do_ds, do_tally, sp, t, lc, lib, c_r);
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
