@@ -498,7 +498,9 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const int32_t sia_b = SIA ? d.day_sia_begin[t] : 0;
   const int32_t sia_n = SIA ? d.day_sia_count[t] : 0;
 
-  __shared__ uint32_t s_w0[G][WG], s_w1[G][WG], s_w2[G][WG];   // state words: original, after VD, after RI / SIA
+  // state words: original, after VD, after RI / SIA (the last only on campaign days)
+  constexpr int GW2 = (F & (FULL_RI | FULL_SIA)) ? G : 1;
+  __shared__ uint32_t s_w0[G][WG], s_w1[G][WG], s_w2[GW2][WG];
   __shared__ int32_t lc[L_N];
   __shared__ unsigned long long lib[LP_MAX_STRAINS];
   __shared__ int64_t red[WG / 64][4];
@@ -697,7 +699,8 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         if (!in_ei && w2 != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w2;
         if (RI && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
       }
-      s_w0[u][tid] = w0; s_w1[u][tid] = w1; s_w2[u][tid] = w2;
+      s_w0[u][tid] = w0; s_w1[u][tid] = w1;
+      if constexpr (GW2 == G) s_w2[u][tid] = w2;
       if (VD && u < ng) {
         // the chunk's earliest death date: exact when its dates were read, else lowered by newborns
         const int32_t wm = wave_min_i32(dmin);
@@ -726,7 +729,8 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (su < 0) continue;
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-      ei_quad(d, i0, live_hi, s_w1[su][qt], s_w2[su][qt], s_w0[su][qt], do_ds, do_tally, sp, t, lc, lib, c_r);
+      const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
+      ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, sp, t, lc, lib, c_r);
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
