@@ -44,27 +44,31 @@ def log(*a):
 def agent_pass_bytes(res, flags, t0, t1):
     """Algorithmic bytes of the agent pass over ticks [t0, t1).
 
-    Every day: N_live * 1 (state) + N_EI * 17 (E/I timers read and written 2*(2+2),
-    strain 1, infectivity 4, paralysis fields 4).  VD step days add the per-chunk
-    earliest death dates (4 B per 256 slots), 6 B per death (date_of_death,
-    potentially_paralyzed, paralyzed) and 17 B per birth (date_of_birth, date_of_death,
-    node_id, ri_timer and state written, risk read); RI days N_live * 4 (ri_timer read
-    and written); SIA days N_live * 4 (date_of_birth); RI or SIA days N_live * 1
-    (chronically_missed).  The susceptible statistics and the earliest death dates are
-    carried across days, so no day reads the risk or date_of_death column of every
-    agent.  N_live = live agents that day (results.pop), N_EI = E+I at the start of the
-    day.
+    Every day: N_live * 1 (state); per E agent 5 (strain 1, exposure timer read and
+    written 2+2); per I agent 9 (strain 1, infection timer read and written 2+2,
+    infectivity 4); per strain-0 I agent 3 more (paralysis timer 2 and
+    potentially_paralyzed 1, which DiseaseState reads for strain-0 infections only).
+    VD step days add the per-chunk earliest death dates (4 B per 256 slots), 6 B per
+    death (date_of_death, potentially_paralyzed, paralyzed) and 17 B per birth
+    (date_of_birth, date_of_death, node_id, ri_timer and state written, risk read); RI
+    days N_live * 4 (ri_timer read and written); SIA days N_live * 4 (date_of_birth);
+    RI or SIA days N_live * 1 (chronically_missed).  The susceptible statistics and the
+    earliest death dates are carried across days, so no day reads the risk or
+    date_of_death column of every agent.  N_live = live agents that day (results.pop);
+    E, I and strain-0 I are the counts at the start of the day (the previous row).
     """
     from laser_polio_amd import engine as em
 
     pop = res["pop"].astype(np.int64).sum(axis=1)
-    ei = (res["E"].astype(np.int64) + res["I"].astype(np.int64)).sum(axis=1)
+    n_e = res["E"].astype(np.int64).sum(axis=1)
+    n_i = res["I"].astype(np.int64).sum(axis=1)
+    n_i0 = res["I_by_strain"][:, :, 0].astype(np.int64).sum(axis=1)
     births = res["births"].astype(np.int64).sum(axis=1) if "births" in res else np.zeros_like(pop)
     deaths = res["deaths"].astype(np.int64).sum(axis=1) if "deaths" in res else np.zeros_like(pop)
     total = 0
     for t in range(t0, t1):
-        n_live, n_ei, f = int(pop[t]), int(ei[t - 1]), int(flags[t])
-        b = 1 * n_live + 17 * n_ei
+        n_live, f = int(pop[t]), int(flags[t])
+        b = 1 * n_live + 5 * int(n_e[t - 1]) + 9 * int(n_i[t - 1]) + 3 * int(n_i0[t - 1])
         if f & em.DAY_VD_STEP:
             b += 4 * (n_live // 256) + 6 * int(deaths[t]) + 17 * int(births[t])
         if f & em.DAY_RI:
@@ -77,7 +81,7 @@ def agent_pass_bytes(res, flags, t0, t1):
     return total
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_agents_national_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "agent_pass_national_traffic.md")
 
 
 def workload_key(scale, warmup, steps, tpw):
@@ -96,16 +100,18 @@ def measured_traffic(key, path=TRAFFIC_PROFILE):
 
 
 def dump_days(path, res, flags, t0, t1):
-    """Per-day flags, N_live, N_EI (start of day), N_draw (agents of nodes with transmission
-    draws), births and deaths."""
+    """Per-day flags, N_live, E, I and strain-0 I (start of day), N_draw (agents of nodes
+    with transmission draws), births and deaths."""
     pop = res["pop"].astype(np.int64)
-    ei = res["E"].astype(np.int64) + res["I"].astype(np.int64)
+    n_e, n_i = res["E"].astype(np.int64).sum(axis=1), res["I"].astype(np.int64).sum(axis=1)
+    n_i0 = res["I_by_strain"][:, :, 0].astype(np.int64).sum(axis=1)
     tx_new = res["new_exposed"].astype(np.int64) - res["ri_protected"] - res["sia_protected"]
     with open(path, "w") as f:
-        f.write("t,flags,n_live,n_ei,n_draw,births,deaths\n")
+        f.write("t,flags,n_live,n_e,n_i,n_i0,n_draw,births,deaths\n")
         for t in range(t0, t1):
             n_draw = int(pop[t][tx_new[t] > 0].sum())
-            f.write(f"{t},{int(flags[t])},{int(pop[t].sum())},{int(ei[t - 1].sum())},{n_draw},"
+            f.write(f"{t},{int(flags[t])},{int(pop[t].sum())},{int(n_e[t - 1])},{int(n_i[t - 1])},"
+                    f"{int(n_i0[t - 1])},{n_draw},"
                     f"{int(res['births'][t].sum())},{int(res['deaths'][t].sum())}\n")
 
 
@@ -194,8 +200,8 @@ def main():
     elapsed = time.perf_counter() - t0
     launches, kernel_ms = (0, 0.0) if args.graph else eng.agent_pass_timing()
     sim._mark_device_newer()
-    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "new_exposed", "ri_protected", "sia_protected",
-                                                 "births", "deaths")}
+    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "I_by_strain", "new_exposed", "ri_protected",
+                                                 "sia_protected", "births", "deaths")}
     agent_days = int(res["pop"][t_begin:t_end].astype(np.int64).sum())
     value = agent_days / elapsed
     log(f"[bench] {args.steps} days in {elapsed:.3f}s -> {value:.4e} agent-days/s; "

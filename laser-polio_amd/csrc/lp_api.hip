@@ -104,6 +104,7 @@ struct lp_ctx {
   hipGraphExec_t graph[8] = {};                    // one per day kind: plain, FULL_* combinations
   bool graph_ready[8] = {};
   bool recount_pending = true;                     // next agent pass recounts carried statistics
+  bool rec_dirty = false;                          // agent records newer than the bound columns
   std::vector<int32_t> day_flags_h;
   std::string err;
   bool timing = false;
@@ -197,6 +198,13 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
   }
 }
 
+// Grid of the record pack / unpack kernels (grid-stride over quads of slots).
+int rec_grid(const lp_ctx* c) {
+  const int64_t nq = (c->cfg.capacity + 3) / 4;
+  const int64_t g = (nq + lp::WG - 1) / lp::WG;
+  return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
 // The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
 // recounts every carried statistic and the paralysis counts); mask: stages;
 // tx_adjust: apply today's transmission exposures to the log; advance: move
@@ -206,6 +214,8 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
+  // a RECOUNT pass reads the bound columns: bring them up to date first
+  if (recount && c->rec_dirty) hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
   hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
   const bool ev = c->timing;
   if (ev) {
@@ -225,6 +235,7 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
     default: launch_agents_u<16>(c, s, kind, mask); break;
   }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
+  if (recount) hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
@@ -236,6 +247,7 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   }
   hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust, advance ? 1 : 0);
   HIPCHK(c, hipGetLastError());
+  c->rec_dirty = true;
   return LP_OK;
 }
 
@@ -387,6 +399,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.p_run, N))) return rc;
   if ((rc = alloc(c, &d.node_carry, (size_t)N * 3))) return rc;
   if ((rc = alloc(c, &d.chunk_min_dod, nch))) return rc;
+  if ((rc = alloc(c, &d.rec, (size_t)g.capacity))) return rc;
   if ((rc = alloc(c, &d.p_dbg, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.expected_dbg, N))) return rc;
@@ -468,6 +481,8 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
 
 int lp_invalidate(lp_ctx* c) {
   if (!c) return LP_ERR_ARG;
+  if (c->rec_dirty)
+    return fail(c, LP_ERR_STATE, "lp_invalidate with unsynced device state: call lp_sync before writing bound buffers");
   c->recount_pending = true;
   return LP_OK;
 }
@@ -481,6 +496,11 @@ int lp_log(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_LOG); }
 
 int lp_sync(lp_ctx* c) {
   if (!c) return LP_ERR_ARG;
+  if (c->prepared && c->rec_dirty) {
+    hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, c->stream, c->d);
+    HIPCHK(c, hipGetLastError());
+    c->rec_dirty = false;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!c->prepared) return LP_OK;
   int32_t e = 0;

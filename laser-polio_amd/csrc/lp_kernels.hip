@@ -58,6 +58,7 @@ struct Dev {
   uint16_t* exp_timer; uint16_t* inf_timer; uint16_t* par_timer;
   int8_t* pp; int8_t* paralyzed; int8_t* ipv; uint8_t* missed;
   float* risk; float* infect; int32_t* dob; int32_t* dod; int16_t* ri_timer;
+  uint4* rec;   // [capacity] agent records: the DiseaseState / tally fields of each slot in 16 B
   // node tables
   const int64_t* seg_begin; int64_t* live_end; const int64_t* seg_end; const int32_t* node_sim;
   const float* r0s; const float* keep; const float* birth_rate_step;
@@ -287,60 +288,150 @@ __device__ __forceinline__ bool progress_agent(int a, int8_t& s, int strain, uin
   return to_r;
 }
 
-// One compacted quad holding E or I agents: loads the columns its agents need,
-// runs their disease progression (model.py:419-452) and tallies them as E / I;
-// returns the quad's state word.  `wei`: state bytes that identify today's E and
-// I agents; `w`: the word to update and write back if it differs from `w_orig`.
-// The paralysis columns are read only when a strain-0 agent is or becomes
-// infectious today (the second, dependent load).
-__device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
-                                        uint32_t w_orig, bool do_ds, bool do_tally, const lp_sim_params& sp,
-                                        int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r) {
-  const int64_t sbase = sp.slot_base;
-  const bool has_e = has_byte(wei, 1);
-  const uint32_t xs = reinterpret_cast<const uint32_t*>(d.strain)[i0 >> 2];
-  const uint2 xe = has_e ? *reinterpret_cast<const uint2*>(d.exp_timer + i0) : make_uint2(0u, 0u);
-  const uint2 xi = *reinterpret_cast<const uint2*>(d.inf_timer + i0);
-  const float4 xin = do_tally ? *reinterpret_cast<const float4*>(d.infect + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-  bool par = false;
-  if (do_ds) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int8_t sa = byte_of(wei, a);
-      const uint32_t eh = ((a < 2 ? xe.x : xe.y) >> (16 * (a & 1))) & 0xffffu;
-      par = par || (byte_of(xs, a) == 0 && (sa == 2 || (sa == 1 && eh == 0)) && i0 + a < live_hi);
+// Agent record (Dev::rec, 16 B per slot): the mutable fields of DiseaseState
+// and the E / I tally in one aligned line segment, so that an E or I agent costs
+// one 16-byte access instead of one cache line from each of seven columns.
+//   x: exposure_timer (bits 0-15) | infection_timer (16-31)
+//   y: paralysis_timer (0-15) | strain (16-23) | potentially_paralyzed (24-31)
+//   z: daily_infectivity (float bits)
+//   w: ipv_protected (0-7) | paralyzed (8-15)
+// k_pack fills the records from the bound columns after a RECOUNT pass and
+// k_unpack writes them back (lp_sync); in between the records are the state of
+// these fields.  Single fields are written with byte stores (RB_*), so writers
+// of different fields of one record never overwrite each other.
+enum : int { RB_STRAIN = 6, RB_PP = 7, RB_IPV = 12, RB_PARALYZED = 13 };
+
+__device__ __forceinline__ int8_t* rec_byte(const Dev& d, int64_t i, int b) {
+  return reinterpret_cast<int8_t*>(d.rec + i) + b;
+}
+
+// DiseaseState step of one E or I agent on its record (model.py:419-452): s its
+// state (1 or 2, updated), r its record (x and y updated).  Sets `to_par` when
+// the agent is paralysed today (the caller stores the paralyzed byte).  Returns
+// true on I -> R.
+__device__ __forceinline__ bool progress_rec(int8_t& s, uint4& r, uint64_t seed, uint64_t slot, int32_t t,
+                                             uint64_t thr_paralysis, bool& to_par, int32_t* lc_newpp,
+                                             int32_t* lc_newp) {
+  bool to_r = false;
+  if (s == 1) {
+    const uint32_t et = r.x & 0xffffu;
+    if (et == 0) s = 2;
+    else r.x = (r.x & 0xffff0000u) | (et - 1);
+  }
+  if (s == 2) {
+    const uint32_t it = r.x >> 16;
+    if (it == 0) { s = 3; to_r = true; }
+    else r.x = (r.x & 0xffffu) | ((it - 1) << 16);
+    if (((r.y >> 16) & 0xffu) == 0) {                  // strain 0
+      const uint32_t pt = r.y & 0xffffu;
+      if (pt == 0 && (int8_t)(r.y >> 24) == -1) {
+        if ((r.w & 0xffu) == 0) {                      // not ipv_protected
+          r.y = (r.y & 0x00ffffffu) | (1u << 24);
+          atomicAdd(lc_newpp, 1);
+          const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
+          if ((uint64_t)o.x < thr_paralysis) { to_par = true; atomicAdd(lc_newp, 1); }
+        } else {
+          r.y = r.y & 0x00ffffffu;
+        }
+      }
+      if (pt > 0) r.y = (r.y & 0xffff0000u) | (pt - 1);
     }
   }
-  const uint2 xp = par ? *reinterpret_cast<const uint2*>(d.par_timer + i0) : make_uint2(0u, 0u);
-  const uint32_t xpp = par ? reinterpret_cast<const uint32_t*>(d.pp)[i0 >> 2] : 0u;
-  const uint32_t xipv = par ? reinterpret_cast<const uint32_t*>(d.ipv)[i0 >> 2] : 0u;
-  uint2 e2 = xe, i2 = xi, p2 = xp;
-  uint32_t ppw = xpp;
+  return to_r;
+}
+
+// Per-simulation constants of the E / I stage, copied into registers once per
+// workgroup (read through `sp` they would be reloaded after every record store).
+struct EIConst {
+  uint64_t seed, thr_paralysis;
+  int64_t slot_base;
+  float r0k[LP_MAX_STRAINS];
+};
+
+__device__ __forceinline__ EIConst ei_const(const lp_sim_params& sp) {
+  EIConst c;
+  c.seed = sp.seed;
+  c.thr_paralysis = sp.thr_paralysis;
+  c.slot_base = sp.slot_base;
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) c.r0k[k] = sp.strain_r0[k];
+  return c;
+}
+
+// One compacted quad holding E or I agents: loads the quad's four records (one
+// 64-byte segment, four loads in flight together), runs the disease progression
+// of its E and I agents (model.py:419-452) and tallies them as E / I.  `wei`:
+// state bytes that identify today's E and I agents; `w`: the quad's state word
+// to update and write back if it differs from `w_orig`.
+__device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
+                                        uint32_t w_orig, bool do_ds, bool do_tally, const EIConst& ec,
+                                        int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r) {
+  const uint4* rq = d.rec + i0;
+  uint4 r[4] = {rq[0], rq[1], rq[2], rq[3]};
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int64_t i = i0 + a;
-    if (i >= live_hi) continue;
     int8_t sa = byte_of(wei, a);
-    if (sa != 1 && sa != 2) continue;
-    const int k = byte_of(xs, a) & (LP_MAX_STRAINS - 1);
-    if (do_ds && progress_agent(a, sa, byte_of(xs, a), e2, i2, p2, ppw, xipv, sp.seed, (uint64_t)(i - sbase), t,
-                                sp.thr_paralysis, d.paralyzed + i, &lc[L_NEWPP], &lc[L_NEWP]))
+    if ((sa != 1 && sa != 2) || i >= live_hi) continue;
+    const uint4 r0 = r[a];
+    const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
+    bool to_par = false;
+    if (do_ds && progress_rec(sa, r[a], ec.seed, (uint64_t)(i - ec.slot_base), t, ec.thr_paralysis, to_par,
+                              &lc[L_NEWPP], &lc[L_NEWP]))
       c_r += 1;
     if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
       if (sa == 1) {
         atomicAdd(&lc[L_E0 + k], 1);
       } else if (sa == 2) {
         atomicAdd(&lc[L_I0 + k], 1);
-        atomicAdd(&lib[k], (unsigned long long)weight_q24(lane_of(xin, a) * sp.strain_r0[k]));
+        const float r0k = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
+        atomicAdd(&lib[k], (unsigned long long)weight_q24(__uint_as_float(r0.z) * r0k));
       }
     }
+    uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + i);
+    if (r[a].x != r0.x) rw[0] = r[a].x;
+    if (r[a].y != r0.y) rw[1] = r[a].y;
+    if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
     w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
   }
   if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
-  if (e2.x != xe.x || e2.y != xe.y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
-  if (i2.x != xi.x || i2.y != xi.y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
-  if (p2.x != xp.x || p2.y != xp.y) *reinterpret_cast<uint2*>(d.par_timer + i0) = p2;
-  if (ppw != xpp) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
+}
+
+// Agent records <- bound columns, slots [0, capacity) (one quad per thread).
+__global__ void __launch_bounds__(WG) k_pack(Dev d) {
+  const int64_t nq = (d.capacity + 3) >> 2;
+  for (int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x; q < nq; q += (int64_t)gridDim.x * WG) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = q * 4 + a;
+      if (i >= d.capacity) break;
+      d.rec[i] = make_uint4((uint32_t)d.exp_timer[i] | ((uint32_t)d.inf_timer[i] << 16),
+                            (uint32_t)d.par_timer[i] | ((uint32_t)(uint8_t)d.strain[i] << 16) |
+                                ((uint32_t)(uint8_t)d.pp[i] << 24),
+                            __float_as_uint(d.infect[i]),
+                            (uint32_t)(uint8_t)d.ipv[i] | ((uint32_t)(uint8_t)d.paralyzed[i] << 8));
+    }
+  }
+}
+
+// Bound columns <- agent records (daily_infectivity is never written, so not copied back).
+__global__ void __launch_bounds__(WG) k_unpack(Dev d) {
+  const int64_t nq = (d.capacity + 3) >> 2;
+  for (int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x; q < nq; q += (int64_t)gridDim.x * WG) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = q * 4 + a;
+      if (i >= d.capacity) break;
+      const uint4 r = d.rec[i];
+      d.exp_timer[i] = (uint16_t)(r.x & 0xffffu);
+      d.inf_timer[i] = (uint16_t)(r.x >> 16);
+      d.par_timer[i] = (uint16_t)(r.y & 0xffffu);
+      d.strain[i] = (int8_t)(r.y >> 16);
+      d.pp[i] = (int8_t)(r.y >> 24);
+      d.ipv[i] = (int8_t)(r.w & 0xffu);
+      d.paralyzed[i] = (int8_t)(r.w >> 8);
+    }
+  }
 }
 
 // Writes the span partial: carried-count deltas from red[][], tallies from LDS.
@@ -431,8 +522,7 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
   __syncthreads();
 
   int32_t c_r = 0;
-  const uint64_t seed = sp.seed;
-  const int64_t sbase = sp.slot_base;
+  const EIConst ec = ei_const(sp);
   for (int base = 0; base < n_ei; base += 64) {
     int j = base + lane, su = -1, src = 0;
     static_for<U>([&](auto uc) {
@@ -447,7 +537,7 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
     const int qt = wave * 64 + src;
     const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
     const uint32_t w0 = s_st[su][qt];
-    ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, sp, t, lc, lib, c_r);
+    ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, ec, t, lc, lib, c_r);
   }
   {
     const int32_t a2 = wave_sum_i32(c_r);
@@ -471,7 +561,7 @@ enum : int { FULL_VD = 1, FULL_RI = 2, FULL_SIA = 4 };
 template <int F>
 __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   // tiles per group: RI days carry ri_timer and chronically_missed per tile, SIA days date_of_birth too
-  constexpr int G = (F & FULL_SIA) ? 2 : (F & FULL_RI) ? 4 : 8;
+  constexpr int G = (F & FULL_SIA) && (F & FULL_RI) ? 1 : (F & (FULL_SIA | FULL_RI)) ? 2 : 8;
   constexpr bool VD = F & FULL_VD, RI = F & FULL_RI, SIA = F & FULL_SIA;
   const int4 span0 = d.spans[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -509,6 +599,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const int64_t nb_lo = VD ? d.nb_begin[node] : live_hi;
   const uint64_t seed = sp.seed;
   const int64_t sbase = sp.slot_base;
+  const EIConst ec = ei_const(sp);
   const uint64_t thr_ri = RI ? d.thr_ri[node] : 0, thr_ipv = RI ? d.thr_ipv[node] : 0;
   // does any of today's campaigns target this node?  (uniform over the workgroup)
   bool sia_here = false;
@@ -624,8 +715,9 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               s = 0;
             } else if (s >= 0 && lane_of(ddu, a) <= t) { // get_deaths (model.py:1775-1777)
               atomicAdd(&lc[L_DEATHS], 1);
-              if (d.pp[i] == 1) atomicAdd(&lc[L_DPP], 1);
-              if (d.paralyzed[i] == 1) atomicAdd(&lc[L_DP], 1);
+              const uint4 rd = d.rec[i];
+              if ((int8_t)(rd.y >> 24) == 1) atomicAdd(&lc[L_DPP], 1);
+              if ((int8_t)(rd.w >> 8) == 1) atomicAdd(&lc[L_DP], 1);
               if (s == 0) {
                 const int64_t w = weight_q24(d.risk[i]);
                 c_s -= 1; c_spos -= (w > 0); c_sw -= w; cw -= w;
@@ -656,7 +748,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                 const uint4 o = draw(seed, slot, t, ST_RI);
                 if ((uint64_t)o.x < thr_ri) {
                   atomicAdd(&lc[L_RIV], 1);
-                  if (s == 0) { s = 1; k = sp.ri_strain; d.strain[i] = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
+                  if (s == 0) { s = 1; k = sp.ri_strain; *rec_byte(d, i, RB_STRAIN) = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
                 }
                 if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); ipv_new |= 1u << (4 * u + a); }
               }
@@ -675,7 +767,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                   if (s == 0 && (uint64_t)o.x < d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1]) {
                     s = 1;
                     k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
-                    d.strain[i] = (int8_t)k;
+                    *rec_byte(d, i, RB_STRAIN) = (int8_t)k;
                     atomicAdd(&lc[L_SIAPK0 + k], 1);
                     if (last) atomicAdd(&lc[L_SIAP], 1);
                   }
@@ -730,7 +822,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
       const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
-      ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, sp, t, lc, lib, c_r);
+      ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
@@ -739,7 +831,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-          if ((ipv_new >> (4 * u + a)) & 1u) d.ipv[i0 + a] = 1;
+          if ((ipv_new >> (4 * u + a)) & 1u) *rec_byte(d, i0 + a, RB_IPV) = 1;
       });
     }
     __syncthreads();
@@ -1497,7 +1589,7 @@ __device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, cons
     if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
   }
   d.state[slot] = 1;
-  d.strain[slot] = (int8_t)k;
+  *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
   return k;
 }
 
