@@ -79,7 +79,7 @@ enum {
 };
 
 typedef struct {
-  int64_t capacity;        /* agent slots over all sims; multiple of LP_TILE */
+  int64_t capacity;        /* agent slots over all sims; multiple of LP_TILE, below 2^32 */
   int32_t n_nodes;         /* nodes over all sims (node ids are global) */
   int32_t n_sims;          /* independent simulations in the batch */
   int32_t n_strains;       /* 1..LP_MAX_STRAINS */

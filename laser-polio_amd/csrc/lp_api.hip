@@ -101,9 +101,10 @@ struct lp_ctx {
   int32_t tpw = 4;
   int32_t agent_grid = 1;
   std::vector<void*> allocs;
-  hipGraphExec_t graph[8] = {};                    // one per day kind: plain, FULL_* combinations
-  bool graph_ready[8] = {};
+  hipGraphExec_t graph[lp::N_KINDS] = {};          // one per day kind: plain, FULL_* combinations, plain from the list
+  bool graph_ready[lp::N_KINDS] = {};
   bool recount_pending = true;                     // next agent pass recounts carried statistics
+  int32_t list_tick = -1;                          // the tick whose E / I list is complete (-1: none)
   bool rec_dirty = false;                          // agent records newer than the bound columns
   std::vector<int32_t> day_flags_h;
   std::string err;
@@ -160,6 +161,9 @@ int alloc(lp_ctx* c, T** p, size_t n) {
 
 int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
+// Workgroups of k_ei (4 wavefronts each); each wavefront loops over list pieces.
+constexpr int kEiGrid = 2048;
+
 
 // Host-side mirror of the device day classification (k_agents::is_plain_day).
 int32_t host_stage_flags(int32_t flags, int mask) {
@@ -187,6 +191,7 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
   const dim3 g(c->n_spans), b(lp::WG);
   switch (kind) {
     case lp::KIND_PLAIN: hipLaunchKernelGGL((lp::k_plain<U>), g, b, 0, s, c->d, mask); break;
+    case lp::KIND_PLAIN_LIST: hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask); break;
     case 1: hipLaunchKernelGGL((lp::k_full<1>), g, b, 0, s, c->d, mask); break;
     case 2: hipLaunchKernelGGL((lp::k_full<2>), g, b, 0, s, c->d, mask); break;
     case 3: hipLaunchKernelGGL((lp::k_full<3>), g, b, 0, s, c->d, mask); break;
@@ -236,7 +241,8 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
   if (recount) hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
-  hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount);
+  hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
+                     kind == lp::KIND_PLAIN_LIST ? 1 : 0);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
     hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
@@ -270,7 +276,7 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
   *out = nullptr;
   if (cfg->capacity <= 0 || cfg->capacity % LP_TILE != 0 || cfg->n_nodes <= 0 || cfg->n_sims <= 0 ||
       cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->n_sia_events < 0 ||
-      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 40 || cfg->mix_mode != 0)
+      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode != 0)
     return LP_ERR_ARG;
   lp_ctx* c = new lp_ctx();
   c->cfg = *cfg;
@@ -400,6 +406,13 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.node_carry, (size_t)N * 3))) return rc;
   if ((rc = alloc(c, &d.chunk_min_dod, nch))) return rc;
   if ((rc = alloc(c, &d.rec, (size_t)g.capacity))) return rc;
+  if ((rc = alloc(c, &d.ei_list, 2 * (size_t)g.capacity))) return rc;
+  if ((rc = alloc(c, &d.ei_cnt, 2 * (size_t)N))) return rc;
+  if ((rc = alloc(c, &d.ei_piece_off, (size_t)N + 1))) return rc;
+  if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.ei_cnt, 0, 2 * (size_t)N * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(d.ei_piece_off, 0, ((size_t)N + 1) * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(d.node_acc, 0, (size_t)N * lp::NPART * sizeof(int64_t), c->stream));
   if ((rc = alloc(c, &d.p_dbg, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.expected_dbg, N))) return rc;
@@ -437,15 +450,19 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   if (t == 0) {
     if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true))) return rc;
     c->recount_pending = false;
+    c->list_tick = 1;
     t = 1;
   }
   for (; t < t_end; ++t) {
     if (c->recount_pending) {
       if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true))) return rc;
       c->recount_pending = false;
+      c->list_tick = t + 1;
       continue;
     }
-    const int kind = day_kind(c, t, LP_STAGE_ALL, 0);
+    int kind = day_kind(c, t, LP_STAGE_ALL, 0);
+    if (kind == lp::KIND_PLAIN && c->list_tick == t) kind = lp::KIND_PLAIN_LIST;
+    c->list_tick = t + 1;   // every day kind run here completes the next tick's list
     if (use_graph && !c->timing) {
       if (!c->graph_ready[kind]) {
         hipGraph_t graph;
@@ -476,6 +493,7 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
   const int recount = (log_only || c->recount_pending) ? 1 : 0;
   c->recount_pending = false;
+  c->list_tick = -1;   // stage calls do not keep the E / I list
   return enqueue_day(c, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
 }
 
@@ -484,6 +502,7 @@ int lp_invalidate(lp_ctx* c) {
   if (c->rec_dirty)
     return fail(c, LP_ERR_STATE, "lp_invalidate with unsynced device state: call lp_sync before writing bound buffers");
   c->recount_pending = true;
+  c->list_tick = -1;
   return LP_OK;
 }
 
@@ -509,6 +528,7 @@ int lp_sync(lp_ctx* c) {
   if (e & lp::ERR_OVERFLOW) return fail(c, LP_ERR_OVERFLOW, "selected-agent list overflow");
   if (e & lp::ERR_SAMPLING) return fail(c, LP_ERR_STATE, "infection sampling found an inconsistent chunk prefix");
   if (e & lp::ERR_KIND) return fail(c, LP_ERR_STATE, "agent pass launched with the wrong day variant");
+  if (e & lp::ERR_LIST) return fail(c, LP_ERR_STATE, "E/I list holds a slot outside the agent capacity");
   return LP_OK;
 }
 
@@ -558,7 +578,7 @@ const char* lp_last_error(lp_ctx* c) { return c ? c->err.c_str() : "null context
 void lp_destroy(lp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 8; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+  for (int k = 0; k < lp::N_KINDS; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -50,7 +50,7 @@ enum : int {
   P_E0 = 15, P_I0 = 19, P_SIAPK0 = 23, P_SW = 27, P_IB0 = 28
 };
 
-enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4, ERR_KIND = 8 };
+enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4, ERR_KIND = 8, ERR_LIST = 16 };
 
 struct Dev {
   // agent columns
@@ -81,6 +81,10 @@ struct Dev {
   int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
   int64_t* node_carry;   // [nodes][3] carried counts: S, S with positive weight, R
   int32_t* chunk_min_dod;  // [chunks] earliest date_of_death among the chunk's live agents
+  uint32_t* ei_list;       // [2][capacity] per-node lists of E / I agents (slots)
+  int32_t* ei_cnt;         // [2][nodes] list lengths
+  int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
+  int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -142,6 +146,8 @@ __global__ void k_day_begin(Dev d, int mask) {
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
   for (int k = 0; k < LP_MAX_STRAINS; ++k) d.newexp[n * LP_MAX_STRAINS + k] = 0;
   d.n_uniq[n] = 0; d.size[n] = 0; d.take_all[n] = 0; d.m_round[n] = 0;
+  d.ei_cnt[(size_t)((t + 1) & 1) * d.n_nodes + n] = 0;
+  for (int f = 0; f < NPART; ++f) d.node_acc[(size_t)n * NPART + f] = 0;
   d.births[n] = 0;
   d.nb_begin[n] = d.live_end[n];
   if (sim_stopped(d, sim, t)) return;
@@ -206,7 +212,8 @@ __device__ __forceinline__ void static_for(F&& f) {
 //   FULL     VD step / RI / SIA days: k_agents adds the campaign and death columns
 //   RECOUNT  the first pass and any pass after the host changed bound buffers:
 //            k_agents recounts every carried statistic from the agent columns
-enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 8 };   // FULL days use kinds 1..7 (FULL_* bits)
+// FULL days use kinds 1..7 (FULL_* bits); KIND_PLAIN_LIST: a plain day from the E / I list (k_ei)
+enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 8, KIND_PLAIN_LIST = 9, N_KINDS = 10 };
 
 __device__ __forceinline__ bool is_plain_day(int32_t flags) {
   return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA));
@@ -306,12 +313,11 @@ __device__ __forceinline__ int8_t* rec_byte(const Dev& d, int64_t i, int b) {
 }
 
 // DiseaseState step of one E or I agent on its record (model.py:419-452): s its
-// state (1 or 2, updated), r its record (x and y updated).  Sets `to_par` when
-// the agent is paralysed today (the caller stores the paralyzed byte).  Returns
-// true on I -> R.
+// state (1 or 2, updated), r its record (x and y updated).  Sets `to_pp` when the
+// agent becomes potentially paralysed today and `to_par` when it is paralysed
+// (the caller counts both and stores the paralyzed byte).  Returns true on I -> R.
 __device__ __forceinline__ bool progress_rec(int8_t& s, uint4& r, uint64_t seed, uint64_t slot, int32_t t,
-                                             uint64_t thr_paralysis, bool& to_par, int32_t* lc_newpp,
-                                             int32_t* lc_newp) {
+                                             uint64_t thr_paralysis, bool& to_pp, bool& to_par) {
   bool to_r = false;
   if (s == 1) {
     const uint32_t et = r.x & 0xffffu;
@@ -327,9 +333,9 @@ __device__ __forceinline__ bool progress_rec(int8_t& s, uint4& r, uint64_t seed,
       if (pt == 0 && (int8_t)(r.y >> 24) == -1) {
         if ((r.w & 0xffu) == 0) {                      // not ipv_protected
           r.y = (r.y & 0x00ffffffu) | (1u << 24);
-          atomicAdd(lc_newpp, 1);
+          to_pp = true;
           const uint4 o = draw(seed, slot, t, ST_PARALYSIS);
-          if ((uint64_t)o.x < thr_paralysis) { to_par = true; atomicAdd(lc_newp, 1); }
+          if ((uint64_t)o.x < thr_paralysis) to_par = true;
         } else {
           r.y = r.y & 0x00ffffffu;
         }
@@ -362,12 +368,14 @@ __device__ __forceinline__ EIConst ei_const(const lp_sim_params& sp) {
 // 64-byte segment, four loads in flight together), runs the disease progression
 // of its E and I agents (model.py:419-452) and tallies them as E / I.  `wei`:
 // state bytes that identify today's E and I agents; `w`: the quad's state word
-// to update and write back if it differs from `w_orig`.
-__device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
+// to update and write back if it differs from `w_orig`.  Returns the agents
+// (bit a) that are still E or I afterwards.
+__device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
                                         uint32_t w_orig, bool do_ds, bool do_tally, const EIConst& ec,
                                         int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r) {
   const uint4* rq = d.rec + i0;
   uint4 r[4] = {rq[0], rq[1], rq[2], rq[3]};
+  uint32_t surv = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int64_t i = i0 + a;
@@ -375,10 +383,11 @@ __device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_h
     if ((sa != 1 && sa != 2) || i >= live_hi) continue;
     const uint4 r0 = r[a];
     const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
-    bool to_par = false;
-    if (do_ds && progress_rec(sa, r[a], ec.seed, (uint64_t)(i - ec.slot_base), t, ec.thr_paralysis, to_par,
-                              &lc[L_NEWPP], &lc[L_NEWP]))
+    bool to_pp = false, to_par = false;
+    if (do_ds && progress_rec(sa, r[a], ec.seed, (uint64_t)(i - ec.slot_base), t, ec.thr_paralysis, to_pp, to_par))
       c_r += 1;
+    if (to_pp) atomicAdd(&lc[L_NEWPP], 1);
+    if (to_par) atomicAdd(&lc[L_NEWP], 1);
     if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
       if (sa == 1) {
         atomicAdd(&lc[L_E0 + k], 1);
@@ -393,8 +402,61 @@ __device__ __forceinline__ void ei_quad(const Dev& d, int64_t i0, int64_t live_h
     if (r[a].y != r0.y) rw[1] = r[a].y;
     if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
     w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
+    if (sa == 1 || sa == 2) surv |= 1u << a;
   }
   if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+  return surv;
+}
+
+// ---------------------------------------------------------------------------
+// E/I lists.  ei_list[b] holds, per node n, the slots of the node's E and I
+// agents at ei_list + b * capacity + seg_begin[n] (room for the whole segment),
+// ei_cnt[b * n_nodes + n] of them, in no particular order.  Day t reads list
+// t & 1 (the agents that are E or I when the day starts) and every pass of the
+// day appends to list (t + 1) & 1: the agent passes their agents that are E or I
+// afterwards (survivors of the progression, RI / SIA exposures), the sampler the
+// agents it exposes.  Plain days then process the list alone (k_ei) instead of
+// scanning every agent; the order of a list does not affect any result.
+
+// Pieces of EI_PIECE list entries: one wavefront of k_ei takes one piece at a time.
+constexpr int EI_PIECE = 256;
+
+__device__ __forceinline__ uint32_t* ei_list_of(const Dev& d, int b, int node) {
+  return d.ei_list + (size_t)b * (size_t)d.capacity + d.seg_begin[node];
+}
+
+// Appends slots i0 + a (bit a of `bits`) of every lane to node `node`'s list b:
+// one atomic per wave, ranks from per-position ballots.  Called by all 64 lanes
+// of the wave together.
+__device__ __forceinline__ void wave_append_quad(const Dev& d, int b, int node, int64_t i0, uint32_t bits, int lane) {
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint64_t mk[4];
+  int tot = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) { mk[a] = __ballot((bits >> a) & 1u); tot += __popcll(mk[a]); }
+  if (tot == 0) return;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, tot);
+  base = __shfl(base, 0, 64);
+  uint32_t* L = ei_list_of(d, b, node);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if ((bits >> a) & 1u) L[base + __popcll(mk[a] & lt)] = (uint32_t)(i0 + a);
+    base += __popcll(mk[a]);
+  }
+}
+
+// Appends `slot` of each active lane with `flag` to node `node`'s list b (every
+// active lane passes the same node): one atomic per call.
+__device__ __forceinline__ void wave_append_slot(const Dev& d, int b, int node, int64_t slot, bool flag, int lane) {
+  const uint64_t mk = __ballot(flag);
+  if (mk == 0) return;
+  const int first = __ffsll((unsigned long long)mk) - 1;
+  const int rank = __popcll(mk & ((1ull << lane) - 1ull));
+  int base = 0;
+  if (lane == first) base = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, __popcll(mk));
+  base = __shfl(base, first, 64);
+  if (flag) ei_list_of(d, b, node)[base + rank] = (uint32_t)slot;
 }
 
 // Agent records <- bound columns, slots [0, capacity) (one quad per thread).
@@ -523,6 +585,7 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 
   int32_t c_r = 0;
   const EIConst ec = ei_const(sp);
+  const int nxt = (t + 1) & 1;
   for (int base = 0; base < n_ei; base += 64) {
     int j = base + lane, su = -1, src = 0;
     static_for<U>([&](auto uc) {
@@ -533,11 +596,15 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
         else j -= c;
       }
     });
-    if (su < 0) continue;
-    const int qt = wave * 64 + src;
-    const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
-    const uint32_t w0 = s_st[su][qt];
-    ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, ec, t, lc, lib, c_r);
+    uint32_t surv = 0;
+    int64_t i0 = 0;
+    if (su >= 0) {
+      const int qt = wave * 64 + src;
+      i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
+      const uint32_t w0 = s_st[su][qt];
+      surv = ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, ec, t, lc, lib, c_r);
+    }
+    wave_append_quad(d, nxt, node, i0, surv, lane);
   }
   {
     const int32_t a2 = wave_sum_i32(c_r);
@@ -545,6 +612,145 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
   }
   __syncthreads();
   write_partial(d, tid, red, lc, lib, false, false);
+}
+
+// ---------------------------------------------------------------------------
+// Plain days from the E / I list: DiseaseState and the E / I tally of the
+// agents in list t & 1 only (the rest of the population does not change on a
+// plain day), survivors appended to list (t + 1) & 1.  Each wavefront takes
+// pieces of EI_PIECE entries of one node (ei_piece_off) in turn and adds the
+// piece's tallies to node_acc with one atomic per field; k_node_reduce reads
+// them in place of the span partials.  Cost: proportional to the E and I
+// agents, not to the population.
+
+// Largest n in [0, N) with off[n] <= p (off[0] = 0 <= p): 64-ary search, all lanes.
+__device__ __forceinline__ int find_piece_node(const int32_t* off, int N, int32_t p, int lane) {
+  int lo = 0, hi = N;
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) / 64;
+    const int c = lo + lane * step;
+    const int32_t v = c < hi ? off[c] : INT_MAX;
+    const uint64_t ok = __ballot(v <= p);
+    lo += (63 - __clzll((unsigned long long)ok)) * step;
+    hi = lo + step < hi ? lo + step : hi;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int cur = t & 1, nxt = cur ^ 1;
+  const int N = d.n_nodes, K = d.K;
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * (WG / 64) + (threadIdx.x >> 6), nw = gridDim.x * (WG / 64);
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if (!is_plain_day(flags)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(d.err, ERR_KIND);
+    return;
+  }
+  const bool do_ds = flags & LP_DAY_DS;
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const int32_t total = d.ei_piece_off[N];
+  for (int32_t p = gw; p < total; p += nw) {
+    const int n = find_piece_node(d.ei_piece_off, N, p, lane);
+    const int sim = d.node_sim[n];
+    if (sim_stopped(d, sim, t)) continue;
+    const EIConst ec = ei_const(d.sims[sim]);
+    const int32_t cnt = d.ei_cnt[(size_t)cur * N + n];
+    const int32_t j0 = (p - d.ei_piece_off[n]) * EI_PIECE;
+    const uint32_t* L = ei_list_of(d, cur, n);
+    // four entries per lane; lanes past the end repeat entry j0 (unconditional loads)
+    uint32_t sl[4];
+    bool v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t j = j0 + q * 64 + lane;
+      v[q] = j < cnt;
+      const uint32_t raw = L[v[q] ? j : j0];
+      if ((int64_t)raw >= d.capacity) { atomicOr(d.err, ERR_LIST); v[q] = false; }
+      sl[q] = (int64_t)raw < d.capacity ? raw : 0u;
+    }
+    int8_t s[4];
+    uint4 r[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { s[q] = d.state[sl[q]]; r[q] = d.rec[sl[q]]; }
+    // tallies: E and I counts per strain in 16-bit fields, I->R / new potentially
+    // paralysed / new paralysed in 16-bit fields, infectious weight per strain
+    int64_t e_pk = 0, i_pk = 0, misc = 0, ib0 = 0, ib1 = 0, ib2 = 0, ib3 = 0;
+    uint32_t surv = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int8_t sa = s[q];
+      if (!v[q] || (sa != 1 && sa != 2)) continue;
+      const uint32_t slot = sl[q];
+      const uint4 r0 = r[q];
+      const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
+      bool to_pp = false, to_par = false;
+      if (do_ds && progress_rec(sa, r[q], ec.seed, (uint64_t)((int64_t)slot - ec.slot_base), t, ec.thr_paralysis,
+                                to_pp, to_par))
+        misc += 1;
+      if (to_pp) misc += 1ll << 16;
+      if (to_par) misc += 1ll << 32;
+      if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
+        if (sa == 1) {
+          e_pk += 1ll << (16 * k);
+        } else if (sa == 2) {
+          i_pk += 1ll << (16 * k);
+          const float r0k = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
+          const int64_t wv = weight_q24(__uint_as_float(r0.z) * r0k);
+          ib0 += k == 0 ? wv : 0; ib1 += k == 1 ? wv : 0; ib2 += k == 2 ? wv : 0; ib3 += k == 3 ? wv : 0;
+        }
+      }
+      uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + slot);
+      if (r[q].x != r0.x) rw[0] = r[q].x;
+      if (r[q].y != r0.y) rw[1] = r[q].y;
+      if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
+      if (sa != s[q]) d.state[slot] = sa;
+      if (sa == 1 || sa == 2) surv |= 1u << q;
+    }
+    // survivors -> next day's list: one atomic per piece
+    {
+      uint64_t mk[4];
+      int rank[4], tot = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        mk[q] = __ballot((surv >> q) & 1u);
+        rank[q] = tot + __popcll(mk[q] & ((1ull << lane) - 1ull));
+        tot += __popcll(mk[q]);
+      }
+      if (tot) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)nxt * N + n, tot);
+        base = __shfl(base, 0, 64);
+        uint32_t* Ln = ei_list_of(d, nxt, n);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((surv >> q) & 1u) Ln[base + rank[q]] = sl[q];
+      }
+    }
+    // the piece's tallies -> node_acc (lanes 0..14 one field each)
+    e_pk = wave_sum_i64(e_pk); i_pk = wave_sum_i64(i_pk); misc = wave_sum_i64(misc);
+    ib0 = wave_sum_i64(ib0); ib1 = wave_sum_i64(ib1); ib2 = wave_sum_i64(ib2); ib3 = wave_sum_i64(ib3);
+    int f = -1;
+    int64_t val = 0;
+    if (lane < LP_MAX_STRAINS) {
+      if (lane < K) { f = P_E0 + lane; val = (e_pk >> (16 * lane)) & 0xffff; }
+    } else if (lane < 2 * LP_MAX_STRAINS) {
+      const int k = lane - LP_MAX_STRAINS;
+      if (k < K) { f = P_I0 + k; val = (i_pk >> (16 * k)) & 0xffff; }
+    } else if (lane < 3 * LP_MAX_STRAINS) {
+      const int k = lane - 2 * LP_MAX_STRAINS;
+      if (k < K) { f = P_IB0 + k; val = k == 0 ? ib0 : k == 1 ? ib1 : k == 2 ? ib2 : ib3; }
+    } else if (lane == 12) {
+      f = P_R; val = misc & 0xffff;
+    } else if (lane == 13) {
+      f = P_NEWPP; val = (misc >> 16) & 0xffff;
+    } else if (lane == 14) {
+      f = P_NEWP; val = (misc >> 32) & 0xffff;
+    }
+    if (f >= 0 && val != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.node_acc + (size_t)n * NPART + f), (unsigned long long)val);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -601,6 +807,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const int64_t sbase = sp.slot_base;
   const EIConst ec = ei_const(sp);
   const uint64_t thr_ri = RI ? d.thr_ri[node] : 0, thr_ipv = RI ? d.thr_ipv[node] : 0;
+  const int nxt = (t + 1) & 1;
   // does any of today's campaigns target this node?  (uniform over the workgroup)
   bool sia_here = false;
   if (SIA)
@@ -648,6 +855,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       uint2 r2 = RI ? rt[u] : make_uint2(0u, 0u);
       int64_t cw = 0;
       int32_t dmin = INT_MAX;   // earliest death date among the quad's agents alive after today
+      uint32_t cnew = 0;        // agents exposed by RI / SIA today (bit a)
       bool q = false;
       // Fast path for the common quad: all four slots in the live range, no newborn, no
       // death due, no SIA here today and no agent entering its RI window; the only change
@@ -775,6 +983,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               }
             }
             if (s_before == 0 && s != 0) {                // vaccinated out of S, counted as E today
+              cnew |= 1u << a;
               const int64_t wt = weight_q24(d.risk[i]);
               c_s -= 1; c_spos -= (wt > 0); c_sw -= wt; cw -= wt;
               if (do_tally) atomicAdd(&lc[L_E0 + k], 1);
@@ -784,6 +993,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         }
       }
       if (!VD) w1 = w0;
+      if (camp) wave_append_quad(d, nxt, node, i0, cnew, lane);
       m[u] = __ballot(q && need_ei);
       n_ei += __popcll(m[u]);
       const bool in_ei = (m[u] >> lane) & 1ull;
@@ -818,11 +1028,15 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
           else j -= c;
         }
       });
-      if (su < 0) continue;
-      const int qt = wave * 64 + src;
-      const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-      const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
-      ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
+      uint32_t surv = 0;
+      int64_t i0 = 0;
+      if (su >= 0) {
+        const int qt = wave * 64 + src;
+        i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
+        const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
+        surv = ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
+      }
+      wave_append_quad(d, nxt, node, i0, surv, lane);
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
@@ -1024,6 +1238,7 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+      uint32_t wf = st[u];   // the quad's state word after the pass
       if (tv[u] && !ei[u] && !do_ri && !do_sia) {
         // no E/I in the quad and no campaign today: only a recount looks at it
         if (RECOUNT) {
@@ -1129,6 +1344,7 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
           w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
         }
         if (w != st0[u]) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+        wf = w;
         if (ei[u]) {
           if (e2.x != xe[u].x || e2.y != xe[u].y) *reinterpret_cast<uint2*>(d.exp_timer + i0) = e2;
           if (i2.x != xi[u].x || i2.y != xi[u].y) *reinterpret_cast<uint2*>(d.inf_timer + i0) = i2;
@@ -1136,6 +1352,15 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
           if (ppw != xpp[u]) reinterpret_cast<uint32_t*>(d.pp)[i0 >> 2] = ppw;
         }
         if (do_ri && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
+      }
+      if (u < span.z) {                                   // E / I list of the next day (uniform condition)
+        uint32_t bits = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int8_t s = byte_of(wf, a);
+          if (tv[u] && i0 + a < live_hi && (s == 1 || s == 2)) bits |= 1u << a;
+        }
+        wave_append_quad(d, (t + 1) & 1, node, i0, bits, lane);
       }
       // chunk weight sums: rewritten when recounting, adjusted by their delta otherwise
       // (the condition is uniform over the workgroup: wave_sum needs every lane)
@@ -1194,7 +1419,7 @@ __device__ int64_t node_span_scan(int64_t* partials, int64_t* prefix, int s0, in
 }
 
 // Per node: reduce span partials, span prefix, results rows of VD / DS / RI / SIA.
-__global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount) {
+__global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount, int list) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x;
   const int sim = d.node_sim[n];
@@ -1224,7 +1449,8 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
     int64_t acc = 0;
 #pragma unroll
     for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
-    tally[tid] = acc;
+    // after k_ei only the carried S weights come from the span partials
+    tally[tid] = (list && tid != P_SW) ? d.node_acc[(size_t)n * NPART + tid] : acc;
   }
   __syncthreads();
   if (tid < 3) {
@@ -1648,6 +1874,7 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
 // is exposed.
 __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   const int32_t t = d.day[0];
+  const int nxt = (t + 1) & 1;
   const int64_t M = d.probe_off[d.n_nodes];
   const int64_t base = d.round_base[1];
   const int lane = threadIdx.x & 63;
@@ -1655,10 +1882,11 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   for (int64_t p0 = (int64_t)blockIdx.x * WG + (threadIdx.x & ~63); p0 < M; p0 += stride) {
     const int64_t p = p0 + lane;
     int node = -1, k = -1;
+    int64_t slot = 0;
     if (p < M) {
       const uint64_t e = d.sel_list[base + p];
       if (e != SEL_INVALID) {
-        const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+        slot = (int64_t)(e & ((1ull << 40) - 1));
         const int nd = (int)(e >> 40);
         if (nd >= d.n_nodes || slot >= d.live_end[nd] || slot < d.seg_begin[nd]) {
           atomicOr(d.err, ERR_SAMPLING);
@@ -1678,9 +1906,12 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           const int c = __popcll(__ballot(k == s));
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
+        wave_append_slot(d, nxt, n0, slot, node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+      const int pos = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, 1);
+      ei_list_of(d, nxt, node)[pos] = (uint32_t)slot;
     }
   }
 }
@@ -1701,13 +1932,18 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   if (tid < LP_MAX_STRAINS) s_ne[tid] = 0;
   if (tid == 0) s_claimed = 0;
   __syncthreads();
+  const int nxt = (t + 1) & 1;
   if (d.take_all[n]) {
     const int64_t lo = d.seg_begin[n], hi = d.live_end[n];
-    for (int64_t i = lo + tid; i < hi; i += WG) {
-      if (d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
+    for (int64_t i0 = lo; i0 < hi; i0 += WG) {         // uniform trip count: whole waves append
+      const int64_t i = i0 + tid;
+      bool ex = false;
+      if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
         atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, i, t)], 1);
+        ex = true;
       }
+      wave_append_slot(d, nxt, n, i, ex, lane);
     }
   } else {
     const int32_t uniq0 = d.n_uniq[n];
@@ -1756,10 +1992,14 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       }
       __threadfence();
       __syncthreads();
-      for (int j = tid; j < need; j += WG) {
-        const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (e == SEL_INVALID) continue;
-        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, (int64_t)(e & ((1ull << 40) - 1)), t)], 1);
+      for (int j0 = 0; j0 < need; j0 += WG) {          // uniform trip count: whole waves append
+        const int j = j0 + tid;
+        const uint64_t e = j < need ? __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : SEL_INVALID;
+        const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
+        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, slot, t)], 1);
+        wave_append_slot(d, nxt, n, slot, e != SEL_INVALID, lane);
       }
       __threadfence();
       __syncthreads();
@@ -1810,11 +2050,39 @@ __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_ad
 
 // Per node: transmission incidence and the log rows; with `advance`, the last
 // workgroup moves the day counter to the next tick.
+// One workgroup of WG threads: ei_piece_off = exclusive prefix over nodes of the
+// pieces of list b (the last entry is the total).
+__device__ void ei_plan(const Dev& d, int b) {
+  __shared__ int32_t sh[WG];
+  const int tid = threadIdx.x, N = d.n_nodes;
+  const int per = (N + WG - 1) / WG;
+  const int n0 = tid * per, n1 = n0 + per < N ? n0 + per : N;
+  const int32_t* cnt = d.ei_cnt + (size_t)b * N;
+  int32_t acc = 0;
+  for (int n = n0; n < n1; ++n) acc += (cnt[n] + EI_PIECE - 1) / EI_PIECE;
+  sh[tid] = acc;
+  __syncthreads();
+  for (int o = 1; o < WG; o <<= 1) {
+    const int32_t v = tid >= o ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int32_t off = sh[tid] - acc;
+  for (int n = n0; n < n1; ++n) { d.ei_piece_off[n] = off; off += (cnt[n] + EI_PIECE - 1) / EI_PIECE; }
+  if (tid == WG - 1) d.ei_piece_off[N] = sh[WG - 1];
+}
+
+// Per node: transmission incidence and the log rows; with `advance`, the last
+// workgroup plans the next day's E / I list pieces and moves the day counter.
 __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, int advance) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < d.n_nodes) day_end_node(d, n, t, mask, tx_adjust);
-  if (advance && last_block(d.ticket + 2) && threadIdx.x == 0) d.day[0] = t + 1;
+  if (advance && last_block(d.ticket + 2)) {
+    ei_plan(d, (t + 1) & 1);
+    if (threadIdx.x == 0) d.day[0] = t + 1;
+  }
 }
 
 
