@@ -192,13 +192,27 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
   switch (kind) {
     case lp::KIND_PLAIN: hipLaunchKernelGGL((lp::k_plain<U>), g, b, 0, s, c->d, mask); break;
     case lp::KIND_PLAIN_LIST: hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask); break;
-    case 1: hipLaunchKernelGGL((lp::k_full<1>), g, b, 0, s, c->d, mask); break;
-    case 2: hipLaunchKernelGGL((lp::k_full<2>), g, b, 0, s, c->d, mask); break;
-    case 3: hipLaunchKernelGGL((lp::k_full<3>), g, b, 0, s, c->d, mask); break;
-    case 4: hipLaunchKernelGGL((lp::k_full<4>), g, b, 0, s, c->d, mask); break;
-    case 5: hipLaunchKernelGGL((lp::k_full<5>), g, b, 0, s, c->d, mask); break;
-    case 6: hipLaunchKernelGGL((lp::k_full<6>), g, b, 0, s, c->d, mask); break;
-    case 7: hipLaunchKernelGGL((lp::k_full<7>), g, b, 0, s, c->d, mask); break;
+    case 1: hipLaunchKernelGGL((lp::k_full<1, true>), g, b, 0, s, c->d, mask); break;
+    case 2: hipLaunchKernelGGL((lp::k_full<2, true>), g, b, 0, s, c->d, mask); break;
+    case 3: hipLaunchKernelGGL((lp::k_full<3, true>), g, b, 0, s, c->d, mask); break;
+    case 4: hipLaunchKernelGGL((lp::k_full<4, true>), g, b, 0, s, c->d, mask); break;
+    case 5: hipLaunchKernelGGL((lp::k_full<5, true>), g, b, 0, s, c->d, mask); break;
+    case 6: hipLaunchKernelGGL((lp::k_full<6, true>), g, b, 0, s, c->d, mask); break;
+    case 7: hipLaunchKernelGGL((lp::k_full<7, true>), g, b, 0, s, c->d, mask); break;
+    case lp::KIND_FULL_LIST + 0: case lp::KIND_FULL_LIST + 1: case lp::KIND_FULL_LIST + 2:
+    case lp::KIND_FULL_LIST + 3: case lp::KIND_FULL_LIST + 4: case lp::KIND_FULL_LIST + 5:
+    case lp::KIND_FULL_LIST + 6:
+      hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask);
+      switch (kind - lp::KIND_FULL_LIST + 1) {
+        case 1: hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, mask); break;
+        case 2: hipLaunchKernelGGL((lp::k_full<2, false>), g, b, 0, s, c->d, mask); break;
+        case 3: hipLaunchKernelGGL((lp::k_full<3, false>), g, b, 0, s, c->d, mask); break;
+        case 4: hipLaunchKernelGGL((lp::k_full<4, false>), g, b, 0, s, c->d, mask); break;
+        case 5: hipLaunchKernelGGL((lp::k_full<5, false>), g, b, 0, s, c->d, mask); break;
+        case 6: hipLaunchKernelGGL((lp::k_full<6, false>), g, b, 0, s, c->d, mask); break;
+        default: hipLaunchKernelGGL((lp::k_full<7, false>), g, b, 0, s, c->d, mask); break;
+      }
+      break;
     default: hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), g, b, 0, s, c->d, mask); break;
   }
 }
@@ -242,7 +256,7 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
   if (recount) hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
-                     kind == lp::KIND_PLAIN_LIST ? 1 : 0);
+                     kind == lp::KIND_PLAIN_LIST ? 1 : kind >= lp::KIND_FULL_LIST ? 2 : 0);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
     hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
@@ -461,7 +475,10 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       continue;
     }
     int kind = day_kind(c, t, LP_STAGE_ALL, 0);
-    if (kind == lp::KIND_PLAIN && c->list_tick == t) kind = lp::KIND_PLAIN_LIST;
+    if (c->list_tick == t) {
+      if (kind == lp::KIND_PLAIN) kind = lp::KIND_PLAIN_LIST;
+      else if (kind >= lp::KIND_FULL && kind < lp::KIND_RECOUNT) kind = lp::KIND_FULL_LIST + kind - lp::KIND_FULL;
+    }
     c->list_tick = t + 1;   // every day kind run here completes the next tick's list
     if (use_graph && !c->timing) {
       if (!c->graph_ready[kind]) {

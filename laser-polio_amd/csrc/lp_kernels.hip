@@ -4,16 +4,18 @@
 // VitalDynamics -> DiseaseState -> RI -> SIA -> Transmission -> log):
 //   k_day_begin    per node: births (count + newborn slot range), per-day resets,
 //                  early-stop rule                     model.py:1711-1719, 788-795
-//   agent pass     ONE pass over the agent slots, node-segmented, one workgroup
-//                  per span of tiles, one partial per span (no global atomics):
-//                  k_plain on days without births/deaths/RI/SIA (state column +
-//                  E/I columns of the quads holding E/I), k_full<VD|RI|SIA> on
-//                  the other days (adds the death / campaign columns), k_agents
-//                  when every carried statistic is recounted
-//                                     model.py:1767-1781, 393-454, 1784-1855,
+//   agent pass     k_ei: DiseaseState and the E / I tally over the per-node lists
+//                  of E and I agents (every day the lists are complete; plain days
+//                  need nothing else); k_full<VD|RI|SIA> on the other days: births,
+//                  deaths (only chunks with a death due), RI and SIA over the
+//                  node-segmented agent slots, one workgroup per span of tiles;
+//                  k_plain / k_full<F, true> run DiseaseState inline instead when
+//                  no list is available, k_agents when every carried statistic is
+//                  recounted               model.py:1767-1781, 393-454, 1784-1855,
 //                                              1994-2060, 980-1007, 902-929
-//   k_node_reduce  per node: reduce span partials, carried S / R counts, exclusive
-//                  prefix of the span weights, results rows of VD/DS/RI/SIA
+//   k_node_reduce  per node: reduce span partials (and k_ei's accumulators), carried
+//                  S / R counts, exclusive prefix of the span weights, result rows
+//                  of VD/DS/RI/SIA
 //   k_mix_foi      per node: network mix (fixed-order tree), seasonality, r0 scalars,
 //                  rate -> probability, expected exposures, Poisson / zero-inflated NB
 //                  count draw (chunks in parallel across the wave)  model.py:1327-1407
@@ -22,11 +24,13 @@
 //                  replacement probes + unique (model.py:1089-1122): every probe
 //                  searches its node's span prefix, then a span's chunks, then one
 //                  chunk, one wavefront per probe; round 1 grid-wide (k_commit
-//                  exposes its agents), later rounds and the take-all nodes per
-//                  node in k_finish; each exposed agent's strain is drawn as it
-//                  is exposed (S -> E)                          model.py:1125-1147
+//                  claims and exposes its agents), later rounds and the take-all
+//                  nodes per node in k_finish; each exposed agent's strain is drawn
+//                  as it is exposed (S -> E) and it joins the next day's E / I list
+//                                                               model.py:1125-1147
 //   k_day_end      per node: log rows + transmission incidence; its last
-//                  workgroup advances the day counter      model.py:1436-1441, 1461-1483
+//                  workgroup plans the next day's list pieces and advances the day
+//                  counter                             model.py:1436-1441, 1461-1483
 // A day is replayed as one hipGraph (one graph per day kind).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,6 +42,7 @@
 namespace lp {
 
 constexpr int WG = 256;
+constexpr int MAX_TPW = 16;         // tiles per span at most
 constexpr int NPART = 32;           // fields of a tally partial
 constexpr int8_t SEL_MARK = 0x7F;   // state byte of an agent selected for exposure today
 constexpr uint64_t SEL_INVALID = ~0ull;
@@ -212,8 +217,10 @@ __device__ __forceinline__ void static_for(F&& f) {
 //   FULL     VD step / RI / SIA days: k_agents adds the campaign and death columns
 //   RECOUNT  the first pass and any pass after the host changed bound buffers:
 //            k_agents recounts every carried statistic from the agent columns
-// FULL days use kinds 1..7 (FULL_* bits); KIND_PLAIN_LIST: a plain day from the E / I list (k_ei)
-enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 8, KIND_PLAIN_LIST = 9, N_KINDS = 10 };
+// FULL days use kinds 1..7 (FULL_* bits); KIND_PLAIN_LIST: a plain day from the E / I list (k_ei);
+// KIND_FULL_LIST + F - 1: FULL day F with DiseaseState from the list (k_ei, then k_full<F, false>)
+enum : int { KIND_PLAIN = 0, KIND_FULL = 1, KIND_RECOUNT = 8, KIND_PLAIN_LIST = 9, KIND_FULL_LIST = 10,
+             N_KINDS = 17 };
 
 __device__ __forceinline__ bool is_plain_day(int32_t flags) {
   return !(flags & (LP_DAY_VD_STEP | LP_DAY_RI | LP_DAY_SIA));
@@ -615,9 +622,12 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-// Plain days from the E / I list: DiseaseState and the E / I tally of the
-// agents in list t & 1 only (the rest of the population does not change on a
-// plain day), survivors appended to list (t + 1) & 1.  Each wavefront takes
+// DiseaseState and the E / I tally from the E / I list: the agents in list
+// t & 1 only (no other agent is E or I), survivors appended to list (t + 1) & 1.
+// On plain days this is the whole agent pass; on FULL days it runs first and
+// k_full<F, false> then does births, deaths, RI and SIA.  On VD step days the
+// agents due to die today are skipped: the reference removes them before
+// DiseaseState (model.py:226-241), so neither step sees them.  Each wavefront takes
 // pieces of EI_PIECE entries of one node (ei_piece_off) in turn and adds the
 // piece's tallies to node_acc with one atomic per field; k_node_reduce reads
 // them in place of the span partials.  Cost: proportional to the E and I
@@ -644,12 +654,9 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * (WG / 64) + (threadIdx.x >> 6), nw = gridDim.x * (WG / 64);
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  if (!is_plain_day(flags)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(d.err, ERR_KIND);
-    return;
-  }
   const bool do_ds = flags & LP_DAY_DS;
   const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const bool vd = flags & LP_DAY_VD_STEP;   // agents due to die today leave before DiseaseState
   const int32_t total = d.ei_piece_off[N];
   for (int32_t p = gw; p < total; p += nw) {
     const int n = find_piece_node(d.ei_piece_off, N, p, lane);
@@ -672,8 +679,13 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
     }
     int8_t s[4];
     uint4 r[4];
+    int32_t dd[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
 #pragma unroll
     for (int q = 0; q < 4; ++q) { s[q] = d.state[sl[q]]; r[q] = d.rec[sl[q]]; }
+    if (vd) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dd[q] = d.dod[sl[q]];
+    }
     // tallies: E and I counts per strain in 16-bit fields, I->R / new potentially
     // paralysed / new paralysed in 16-bit fields, infectious weight per strain
     int64_t e_pk = 0, i_pk = 0, misc = 0, ib0 = 0, ib1 = 0, ib2 = 0, ib3 = 0;
@@ -681,7 +693,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int8_t sa = s[q];
-      if (!v[q] || (sa != 1 && sa != 2)) continue;
+      if (!v[q] || (sa != 1 && sa != 2) || dd[q] <= t) continue;   // dying today: the VD pass takes it
       const uint32_t slot = sl[q];
       const uint4 r0 = r[q];
       const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
@@ -754,7 +766,10 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-// FULL days (VD step, RI and / or SIA; F = OR of the FULL_* bits).  The span's
+// FULL days (VD step, RI and / or SIA; F = OR of the FULL_* bits).  EI: the
+// pass also runs DiseaseState and the E / I tally of its E and I agents (when no
+// E / I list is available); otherwise k_ei has done that already and, on VD-only
+// days, only chunks with a death due or newborn slots are visited.  The span's
 // tiles go through in groups of G: one round of dense loads (state and the
 // day's campaign / death columns), births, deaths, RI and SIA per agent in
 // registers, then the group's quads holding an agent that was E or I after
@@ -764,11 +779,12 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
 // SIA in the reference day, model.py:226-241).
 enum : int { FULL_VD = 1, FULL_RI = 2, FULL_SIA = 4 };
 
-template <int F>
+template <int F, bool EI>
 __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   // tiles per group: RI days carry ri_timer and chronically_missed per tile, SIA days date_of_birth too
   constexpr int G = (F & FULL_SIA) && (F & FULL_RI) ? 1 : (F & (FULL_SIA | FULL_RI)) ? 2 : 8;
   constexpr bool VD = F & FULL_VD, RI = F & FULL_RI, SIA = F & FULL_SIA;
+  constexpr bool SPARSE_VD = F == FULL_VD && !EI;
   const int4 span0 = d.spans[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);   // the same value, known uniform
@@ -790,13 +806,16 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   }
   const bool do_ds = flags & LP_DAY_DS;
   const bool do_tally = (flags & LP_DAY_TX) != 0;
-  const bool need_ei = do_ds || do_tally;
+  const bool need_ei = EI && (do_ds || do_tally);
   const int32_t sia_b = SIA ? d.day_sia_begin[t] : 0;
   const int32_t sia_n = SIA ? d.day_sia_count[t] : 0;
 
   // state words: original, after VD, after RI / SIA (the last only on campaign days)
   constexpr int GW2 = (F & (FULL_RI | FULL_SIA)) ? G : 1;
   __shared__ uint32_t s_w0[G][WG], s_w1[G][WG], s_w2[GW2][WG];
+  // per quad of the span: its agents that are E or I after today's pass (bit a),
+  // appended to the next day's list once at the end
+  __shared__ uint8_t s_sv[MAX_TPW][WG];
   __shared__ int32_t lc[L_N];
   __shared__ unsigned long long lib[LP_MAX_STRAINS];
   __shared__ int64_t red[WG / 64][4];
@@ -833,8 +852,15 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
       const bool v = (u < ng) && (i0 < live_hi);
-      st[u] = (u < ng) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
       cmin[u] = (VD && u < ng) ? __builtin_amdgcn_readlane(cm_all, (g0 + u) * (LP_TILE / LP_CHUNK) + wave_u) : INT_MAX;
+      // VD-only days after k_ei: a chunk with no death due and no newborn slot has nothing
+      // to do (wave-uniform test); its state is not read
+      bool act = u < ng;
+      if (SPARSE_VD) {
+        const int64_t c0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)wave_u * LP_CHUNK;
+        act = act && (cmin[u] <= t || (c0 + LP_CHUNK > nb_lo && c0 < live_hi));
+      }
+      st[u] = act ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
       if (camp) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
       if (RI) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
       if (SIA) db[u] = (v && sia_here) ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
@@ -993,7 +1019,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         }
       }
       if (!VD) w1 = w0;
-      if (camp) wave_append_quad(d, nxt, node, i0, cnew, lane);
+      if (u < ng) s_sv[g0 + u][tid] = (uint8_t)cnew;
       m[u] = __ballot(q && need_ei);
       n_ei += __popcll(m[u]);
       const bool in_ei = (m[u] >> lane) & 1ull;
@@ -1028,15 +1054,13 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
           else j -= c;
         }
       });
-      uint32_t surv = 0;
-      int64_t i0 = 0;
-      if (su >= 0) {
-        const int qt = wave * 64 + src;
-        i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-        const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
-        surv = ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
-      }
-      wave_append_quad(d, nxt, node, i0, surv, lane);
+      if (su < 0) continue;
+      const int qt = wave * 64 + src;
+      const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
+      const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
+      const uint32_t surv =
+          ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
+      if (surv) s_sv[g0 + su][qt] |= (uint8_t)surv;
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
@@ -1051,6 +1075,26 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
     __syncthreads();
   }
 
+  {
+    // the span's E / I agents after today -> next day's list: one atomic per wave
+    int c = 0;
+    for (int u = 0; u < span0.z; ++u) c += __popc(s_sv[u][tid]);
+    const int incl = (int)wave_incl_scan_i64(c, lane);
+    const int tot = __shfl(incl, 63, 64);
+    if (tot) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, tot);
+      int pos = __shfl(base, 0, 64) + incl - c;
+      uint32_t* L = ei_list_of(d, nxt, node);
+      for (int u = 0; u < span0.z; ++u) {
+        const uint32_t bits = s_sv[u][tid];
+        const int64_t i0 = ((int64_t)span0.y + u) * LP_TILE + (int64_t)tid * 4;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          if ((bits >> a) & 1u) L[pos++] = (uint32_t)(i0 + a);
+      }
+    }
+  }
   {
     const int32_t a0 = wave_sum_i32(c_s), a1 = wave_sum_i32(c_spos), a2 = wave_sum_i32(c_r);
     const int64_t a3 = wave_sum_i64(c_sw);
@@ -1449,8 +1493,10 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
     int64_t acc = 0;
 #pragma unroll
     for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
-    // after k_ei only the carried S weights come from the span partials
-    tally[tid] = (list && tid != P_SW) ? d.node_acc[(size_t)n * NPART + tid] : acc;
+    // list 1: k_ei alone (only the carried S weights come from the span partials);
+    // list 2: k_ei and a FULL-day pass (both add up)
+    const int64_t ea = list ? d.node_acc[(size_t)n * NPART + tid] : 0;
+    tally[tid] = list == 1 ? (tid == P_SW ? acc : ea) : acc + ea;
   }
   __syncthreads();
   if (tid < 3) {
@@ -1722,29 +1768,12 @@ __device__ __forceinline__ int64_t ld64(const int64_t* p) {
   return ATOMIC_IO ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
 }
 
-// One wavefront per probe: target T in [0, W); the span with prefix <= T by a
-// 64-ary search over the node's span prefix, the chunk inside the span by a wave
-// scan of its chunk weights, the agent inside the chunk by a wave scan of the
-// agents' weights.  MARKED_AS_S: an agent already claimed in this round
-// (SEL_MARK) keeps its weight, so that all probes of a round see the weights of
-// the round's start.  Returns the slot, or -1 if the weights were inconsistent.
+// The agent hit by target T inside span s whose prefix (weight before it) is
+// sp_base: the chunk by a wave scan of the span's chunk weights, the agent by a
+// wave scan of the chunk's agent weights (MARKED_AS_S as for resolve_probe).
+// Returns the slot, or -1 if the weights were inconsistent.
 template <bool ATOMIC_IO, bool MARKED_AS_S>
-__device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
-  int64_t lo = d.node_span_begin[node];
-  int64_t hi = d.node_span_begin[node + 1];   // search spans [lo, hi)
-  while (hi - lo > 1) {
-    const int64_t range = hi - lo;
-    const int64_t step = (range + 63) / 64;
-    const int64_t c = lo + (int64_t)lane * step;
-    const int64_t v = (c < hi) ? ld64<ATOMIC_IO>(prefix + c) : INT64_MAX;
-    const uint64_t ok = __ballot(v <= T);        // monotone: lane 0 always true
-    const int L = 63 - __clzll(ok);
-    lo = lo + (int64_t)L * step;
-    const int64_t nhi = lo + step;
-    hi = nhi < hi ? nhi : hi;
-  }
-  const int64_t s = lo;
-  const int64_t sp_base = ld64<ATOMIC_IO>(prefix + s);
+__device__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane) {
   const int4 span = d.spans[s];
   const int64_t c0 = (int64_t)span.y * (LP_TILE / LP_CHUNK);
   const int nc = span.z * (LP_TILE / LP_CHUNK);
@@ -1779,6 +1808,32 @@ __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_
   const int L = __ffsll((unsigned long long)hb) - 1;
   const int ha = __shfl(hit, L, 64);
   return c * LP_CHUNK + (int64_t)L * 4 + ha;
+}
+
+
+// One wavefront per probe: target T in [0, W); the span with prefix <= T by a
+// 64-ary search over the node's span prefix, the chunk inside the span by a wave
+// scan of its chunk weights, the agent inside the chunk by a wave scan of the
+// agents' weights.  MARKED_AS_S: an agent already claimed in this round
+// (SEL_MARK) keeps its weight, so that all probes of a round see the weights of
+// the round's start.  Returns the slot, or -1 if the weights were inconsistent.
+template <bool ATOMIC_IO, bool MARKED_AS_S>
+__device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
+  int64_t lo = d.node_span_begin[node];
+  int64_t hi = d.node_span_begin[node + 1];   // search spans [lo, hi)
+  while (hi - lo > 1) {
+    const int64_t range = hi - lo;
+    const int64_t step = (range + 63) / 64;
+    const int64_t c = lo + (int64_t)lane * step;
+    const int64_t v = (c < hi) ? ld64<ATOMIC_IO>(prefix + c) : INT64_MAX;
+    const uint64_t ok = __ballot(v <= T);        // monotone: lane 0 always true
+    const int L = 63 - __clzll(ok);
+    lo = lo + (int64_t)L * step;
+    const int64_t nhi = lo + step;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const int64_t s = lo;
+  return resolve_in_span<ATOMIC_IO, MARKED_AS_S>(d, s, ld64<ATOMIC_IO>(prefix + s), T, lane);
 }
 
 // Claim a hit: state byte 0 -> SEL_MARK with a word CAS; false if the agent was
@@ -1820,9 +1875,9 @@ __device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, cons
 }
 
 
-// Sampling round 1, grid-wide: one wavefront per probe resolves it and claims
-// the agent.  The probe offsets are staged in LDS when they fit (dynamic shared
-// memory of (n_nodes + 1) * 8 bytes, else none).
+// Sampling round 1, grid-wide: one wavefront per probe resolves it to an agent
+// (k_commit claims the hits).  The probe offsets are staged in LDS when they fit
+// (dynamic shared memory of (n_nodes + 1) * 8 bytes, else none).
 constexpr int LDS_OFF_MAX = 8192;
 __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   extern __shared__ int64_t s_off[];
@@ -1840,38 +1895,55 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int32_t t = d.day[0];
   const int lane = threadIdx.x & 63;
   const int64_t base = d.round_base[1];
-  // each wave takes a contiguous range of probes (probes are ordered by node), so its
-  // claims are counted locally and added to the node's total once per node change
+  // each wave takes a contiguous range of probes (probes are ordered by node); the
+  // node's seed, weight and span prefix (one span per lane, up to 64 spans) are
+  // loaded once per node, so a probe costs the chunk-weight and the agent loads
   const int64_t per = (M + nw - 1) / nw;
   const int64_t p0 = (first + (threadIdx.x >> 6)) * per;
   const int64_t p1 = p0 + per < M ? p0 + per : M;
-  int cur_node = -1, claimed = 0;
+  int node = -1, ns = 0;
+  int64_t node_lo = 0, node_hi = 0, W = 0, s0 = 0, spx = INT64_MAX;
+  uint64_t seed = 0;
+  uint32_t lnode = 0;
   for (int64_t p = p0; p < p1; ++p) {
-    const int node = (int)find_node(off, N, p);
-    if (node != cur_node) {
-      if (lane == 0 && claimed) atomicAdd(d.n_uniq + cur_node, claimed);
-      cur_node = node;
-      claimed = 0;
+    if (node < 0 || p >= node_hi) {
+      node = (int)find_node(off, N, p);
+      node_lo = off[node];
+      node_hi = off[node + 1];
+      const lp_sim_params& sp = d.sims[d.node_sim[node]];
+      seed = sp.seed;
+      lnode = (uint32_t)(node - sp.node_begin);
+      W = d.W_round[node];
+      s0 = d.node_span_begin[node];
+      ns = (int)(d.node_span_begin[node + 1] - s0);
+      spx = (ns <= 64 && lane < ns) ? d.span_prefix[s0 + lane] : INT64_MAX;
     }
-    const int64_t j = p - off[node];
-    const lp_sim_params& sp = d.sims[d.node_sim[node]];
-    const uint4 o = philox((uint32_t)j, (uint32_t)(node - sp.node_begin), (uint32_t)t, ST_PROBE | (1u << 8), sp.seed);
+    const int64_t j = p - node_lo;
+    const uint4 o = philox((uint32_t)j, lnode, (uint32_t)t, ST_PROBE | (1u << 8), seed);
     const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
-    const int64_t W = d.W_round[node];
     const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
-    const int64_t slot = W > 0 ? resolve_probe<false, true>(d, node, T, d.span_prefix, lane) : -1;
+    int64_t slot = -1;
+    if (W > 0) {
+      if (ns <= 64) {
+        const uint64_t ok = __ballot(spx <= T);      // monotone over the node's spans
+        if (ok) {
+          const int L = 63 - __clzll((unsigned long long)ok);
+          slot = resolve_in_span<false, true>(d, s0 + L, __shfl(spx, L, 64), T, lane);
+        }
+      } else {
+        slot = resolve_probe<false, true>(d, node, T, d.span_prefix, lane);
+      }
+    }
     if (lane == 0) {
-      uint64_t e = SEL_INVALID;
       if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-      else if (claim(d.state, slot)) { claimed += 1; e = (uint64_t)slot | ((uint64_t)node << 40); }
-      d.sel_list[base + p] = e;
+      d.sel_list[base + p] = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)node << 40));
     }
   }
-  if (lane == 0 && claimed) atomicAdd(d.n_uniq + cur_node, claimed);
 }
 
-// Grid-wide, after round 1: every agent claimed in round 1 loses its weight and
-// is exposed.
+// Grid-wide, after round 1: the first probe of each agent hit in round 1 claims
+// it (the round's distinct agents, counted per node in n_uniq); a claimed agent
+// loses its weight and is exposed.
 __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   const int32_t t = d.day[0];
   const int nxt = (t + 1) & 1;
@@ -1890,7 +1962,7 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
         const int nd = (int)(e >> 40);
         if (nd >= d.n_nodes || slot >= d.live_end[nd] || slot < d.seg_begin[nd]) {
           atomicOr(d.err, ERR_SAMPLING);
-        } else {
+        } else if (claim(d.state, slot)) {           // the first probe of an agent claims it
           node = nd;
           unweight(d, node, slot);
           k = expose_agent_nocount(d, node, d.sims[d.node_sim[node]], slot, t);
@@ -1901,15 +1973,18 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
     const int n0 = __shfl(node, __ffsll((unsigned long long)__ballot(node >= 0)) - 1, 64);
     const bool same = __ballot(node >= 0 && node != n0) == 0;
     if (same) {
-      if (__ballot(node >= 0)) {
+      const uint64_t any = __ballot(node >= 0);
+      if (any) {
         for (int s = 0; s < d.K; ++s) {
           const int c = __popcll(__ballot(k == s));
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
+        if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
         wave_append_slot(d, nxt, n0, slot, node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
+      atomicAdd(d.n_uniq + node, 1);
       const int pos = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, 1);
       ei_list_of(d, nxt, node)[pos] = (uint32_t)slot;
     }
