@@ -258,7 +258,7 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
                      kind == lp::KIND_PLAIN_LIST ? 1 : kind >= lp::KIND_FULL_LIST ? 2 : 0);
   if (mask & LP_STAGE_TRANSMISSION) {
-    hipLaunchKernelGGL(lp::k_mix_foi, dim3(N), dim3(lp::WG), 0, s, d, mask);
+    hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
     hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
     hipLaunchKernelGGL(lp::k_sample, dim3(2048), dim3(lp::WG), off_lds, s, d);
@@ -424,6 +424,8 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.ei_cnt, 2 * (size_t)N))) return rc;
   if ((rc = alloc(c, &d.ei_piece_off, (size_t)N + 1))) return rc;
   if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
+  if ((rc = alloc(c, &d.beta_f, (size_t)N * LP_MAX_STRAINS))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.beta_f, 0, (size_t)N * LP_MAX_STRAINS * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(d.ei_cnt, 0, 2 * (size_t)N * sizeof(int32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(d.ei_piece_off, 0, ((size_t)N + 1) * sizeof(int32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(d.node_acc, 0, (size_t)N * lp::NPART * sizeof(int64_t), c->stream));

@@ -90,6 +90,7 @@ struct Dev {
   int32_t* ei_cnt;         // [2][nodes] list lengths
   int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
+  float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -848,6 +849,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
     int4 db[G];
     uint2 rt[G];
     int32_t cmin[G];
+    bool actv[G];   // SPARSE_VD: this wave's chunk of tile u has work today (wave-uniform)
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
@@ -860,6 +862,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         const int64_t c0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)wave_u * LP_CHUNK;
         act = act && (cmin[u] <= t || (c0 + LP_CHUNK > nb_lo && c0 < live_hi));
       }
+      actv[u] = act;
       st[u] = act ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
       if (camp) mw[u] = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
       if (RI) rt[u] = v ? *reinterpret_cast<const uint2*>(d.ri_timer + i0) : make_uint2(0u, 0u);
@@ -873,6 +876,11 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
     static_for<G>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       const int64_t i0 = ((int64_t)tile0 + u) * LP_TILE + (int64_t)tid * 4;
+      if (SPARSE_VD && !actv[u]) {                        // nothing due in this chunk today
+        m[u] = 0;
+        if (u < ng) s_sv[g0 + u][tid] = 0;
+        return;
+      }
       const uint32_t w0 = st[u];
       uint32_t w1 = w0, w2 = w0;
       // death dates only in the (rare) chunks with a death due; the branch is uniform per wave
@@ -1509,6 +1517,8 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   }
   __syncthreads();
   if (tid < NPART) d.node_tally[(size_t)n * NPART + tid] = tally[tid];
+  if (tid < LP_MAX_STRAINS)   // infectious weight per strain as the network mix reads it
+    d.beta_f[(size_t)n * LP_MAX_STRAINS + tid] = tid < d.K ? (float)tally[P_IB0 + tid] * 5.9604644775390625e-08f : 0.0f;
   if (flags & LP_DAY_TX) {
     const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
     if (tid == 0) d.W_round[n] = W;
@@ -1607,31 +1617,38 @@ __device__ void plan_scan(const Dev& d, int round) {
 }
 
 // ---------------------------------------------------------------------------
-__device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& sp, int32_t t, int tid, int lane) {
+// One wavefront per node.  The network mix sum_j N_ji beta_j is added in the
+// order of a 256-lane strided sum followed by a halving tree over the 256
+// partials: lane l holds the partials of lanes l, l + 64, l + 128, l + 192, folds
+// them in-lane for the first two tree levels and across lanes for the rest.
+__device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& sp, int32_t t, int lane) {
   const int K = d.K;
   const int ns = sp.n_nodes, li = i - sp.node_begin;
   const float* NT = d.network_t + sp.net_off + (size_t)li * ns;
-  __shared__ float part[LP_MAX_STRAINS][WG];
+  const float4* B = reinterpret_cast<const float4*>(d.beta_f) + sp.node_begin;
+  float acc[4][LP_MAX_STRAINS];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+#pragma unroll
+    for (int k = 0; k < LP_MAX_STRAINS; ++k) acc[v][k] = 0.0f;
+    for (int j = lane + 64 * v; j < ns; j += 256) {
+      const float nt = NT[j];
+      const float4 b = B[j];
+      acc[v][0] = __builtin_fmaf(nt, b.x, acc[v][0]);
+      acc[v][1] = __builtin_fmaf(nt, b.y, acc[v][1]);
+      acc[v][2] = __builtin_fmaf(nt, b.z, acc[v][2]);
+      acc[v][3] = __builtin_fmaf(nt, b.w, acc[v][3]);
+    }
+  }
+  float part[LP_MAX_STRAINS];
 #pragma unroll
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
-    if (k >= K) break;
-    float acc = 0.0f;
-    for (int j = tid; j < ns; j += WG) {
-      const float bj = (float)d.node_tally[(size_t)(sp.node_begin + j) * NPART + P_IB0 + k] * 5.9604644775390625e-08f;
-      acc = __builtin_fmaf(NT[j], bj, acc);
-    }
-    part[k][tid] = acc;
-  }
-  __syncthreads();
-  for (int w = WG / 2; w >= 1; w >>= 1) {
-    if (tid < w) {
+    float x = (acc[0][k] + acc[2][k]) + (acc[1][k] + acc[3][k]);   // tree levels 128 and 64
 #pragma unroll
-      for (int k = 0; k < LP_MAX_STRAINS; ++k) if (k < K) part[k][tid] = part[k][tid] + part[k][tid + w];
-    }
-    __syncthreads();
+    for (int w = 32; w >= 1; w >>= 1) x = x + __shfl_down(x, w, 64);
+    part[k] = __shfl(x, 0, 64);
   }
-  if (tid >= 64) return;
-  // Wave 0: node-level map, then the count draw with chunks spread over lanes.
+  // node-level map, then the count draw with chunks spread over lanes
   const int64_t* tl = d.node_tally + (size_t)i * NPART;
   const int32_t pop_t = d.r_pop[(size_t)t * d.n_nodes + i];
   const float popc = (float)(pop_t > 1 ? pop_t : 1);
@@ -1643,7 +1660,7 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
     if (k >= K) break;
     const float bi = (float)tl[P_IB0 + k] * 5.9604644775390625e-08f;
-    const float bm = __builtin_fmaf(bi, d.keep[i], part[k][0]);
+    const float bm = __builtin_fmaf(bi, d.keep[i], part[k]);
     const float b2 = (bm * season) * d.r0s[i];
     const float rate = b2 / popc;
     float p = -expm1f_lp(-rate);
@@ -1731,18 +1748,19 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   }
 }
 
-// Per node: network mix, FOI, count draw.  Workgroup per node.
+// Per node: network mix, FOI, count draw.  One wavefront per node.
 __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
   const int32_t t = d.day[0];
-  const int i = blockIdx.x;
+  const int i = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= d.n_nodes) return;
   const int sim = d.node_sim[i];
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  const int tid = threadIdx.x, lane = tid & 63;
   if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
-    if (tid == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
+    if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
   } else {
-    mix_foi_node(d, i, sim, sp, t, tid, lane);
+    mix_foi_node(d, i, sim, sp, t, lane);
   }
 }
 
@@ -1851,13 +1869,22 @@ __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
   }
 }
 
+// Remove an exposed agent's weight w from its chunk and from span s.
+__device__ __forceinline__ void unweight_in(const Dev& d, int64_t s, int64_t slot, int64_t w) {
+  const unsigned long long dw = (unsigned long long)(-w);
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
+}
+
+// The span of node `node` holding `slot`.
+__device__ __forceinline__ int64_t span_of(const Dev& d, int node, int64_t slot) {
+  const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
+  return d.node_span_begin[node] + (tile - first) / d.tpw;
+}
+
 // Remove an exposed agent's weight from its chunk and its span.
 __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
-  const unsigned long long dw = (unsigned long long)(-weight_q24(d.risk[slot]));
-  atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
-  const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
-  const int64_t s = d.node_span_begin[node] + (tile - first) / d.tpw;
-  atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
+  unweight_in(d, span_of(d, node, slot), slot, weight_q24(d.risk[slot]));
 }
 
 // S -> E of a selected agent with its strain drawn by cumulative strain
@@ -1960,12 +1987,21 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
       if (e != SEL_INVALID) {
         slot = (int64_t)(e & ((1ull << 40) - 1));
         const int nd = (int)(e >> 40);
-        if (nd >= d.n_nodes || slot >= d.live_end[nd] || slot < d.seg_begin[nd]) {
+        if (nd >= d.n_nodes || slot >= d.capacity) {
           atomicOr(d.err, ERR_SAMPLING);
-        } else if (claim(d.state, slot)) {           // the first probe of an agent claims it
-          node = nd;
-          unweight(d, node, slot);
-          k = expose_agent_nocount(d, node, d.sims[d.node_sim[node]], slot, t);
+        } else {
+          // everything the exposure reads, loaded ahead of the claim
+          const int64_t lo = d.seg_begin[nd], hi = d.live_end[nd];
+          const int64_t w = weight_q24(d.risk[slot]);
+          const int64_t sp_i = span_of(d, nd, slot);
+          const lp_sim_params& sp = d.sims[d.node_sim[nd]];
+          if (slot >= hi || slot < lo) {
+            atomicOr(d.err, ERR_SAMPLING);
+          } else if (claim(d.state, slot)) {         // the first probe of an agent claims it
+            node = nd;
+            unweight_in(d, sp_i, slot, w);
+            k = expose_agent_nocount(d, node, sp, slot, t);
+          }
         }
       }
     }
