@@ -309,12 +309,13 @@ __device__ __forceinline__ bool progress_agent(int a, int8_t& s, int strain, uin
 //   x: exposure_timer (bits 0-15) | infection_timer (16-31)
 //   y: paralysis_timer (0-15) | strain (16-23) | potentially_paralyzed (24-31)
 //   z: daily_infectivity (float bits)
-//   w: ipv_protected (0-7) | paralyzed (8-15)
+//   w: ipv_protected (0-7) | paralyzed (8-15) | disease_state (16-23, a copy
+//      kept exact for E and I agents: every writer of their state writes both)
 // k_pack fills the records from the bound columns after a RECOUNT pass and
 // k_unpack writes them back (lp_sync); in between the records are the state of
 // these fields.  Single fields are written with byte stores (RB_*), so writers
 // of different fields of one record never overwrite each other.
-enum : int { RB_STRAIN = 6, RB_PP = 7, RB_IPV = 12, RB_PARALYZED = 13 };
+enum : int { RB_STRAIN = 6, RB_PP = 7, RB_IPV = 12, RB_PARALYZED = 13, RB_STATE = 14 };
 
 __device__ __forceinline__ int8_t* rec_byte(const Dev& d, int64_t i, int b) {
   return reinterpret_cast<int8_t*>(d.rec + i) + b;
@@ -409,6 +410,7 @@ __device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t li
     if (r[a].x != r0.x) rw[0] = r[a].x;
     if (r[a].y != r0.y) rw[1] = r[a].y;
     if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
+    if (sa != byte_of(wei, a)) *rec_byte(d, i, RB_STATE) = sa;
     w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
     if (sa == 1 || sa == 2) surv |= 1u << a;
   }
@@ -479,7 +481,8 @@ __global__ void __launch_bounds__(WG) k_pack(Dev d) {
                             (uint32_t)d.par_timer[i] | ((uint32_t)(uint8_t)d.strain[i] << 16) |
                                 ((uint32_t)(uint8_t)d.pp[i] << 24),
                             __float_as_uint(d.infect[i]),
-                            (uint32_t)(uint8_t)d.ipv[i] | ((uint32_t)(uint8_t)d.paralyzed[i] << 8));
+                            (uint32_t)(uint8_t)d.ipv[i] | ((uint32_t)(uint8_t)d.paralyzed[i] << 8) |
+                                ((uint32_t)(uint8_t)d.state[i] << 16));
     }
   }
 }
@@ -682,7 +685,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
     uint4 r[4];
     int32_t dd[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { s[q] = d.state[sl[q]]; r[q] = d.rec[sl[q]]; }
+    for (int q = 0; q < 4; ++q) { r[q] = d.rec[sl[q]]; s[q] = (int8_t)(r[q].w >> 16); }
     if (vd) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) dd[q] = d.dod[sl[q]];
@@ -718,7 +721,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
       if (r[q].x != r0.x) rw[0] = r[q].x;
       if (r[q].y != r0.y) rw[1] = r[q].y;
       if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
-      if (sa != s[q]) d.state[slot] = sa;
+      if (sa != s[q]) { d.state[slot] = sa; *rec_byte(d, slot, RB_STATE) = sa; }
       if (sa == 1 || sa == 2) surv |= 1u << q;
     }
     // survivors -> next day's list: one atomic per piece
@@ -990,7 +993,11 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                 const uint4 o = draw(seed, slot, t, ST_RI);
                 if ((uint64_t)o.x < thr_ri) {
                   atomicAdd(&lc[L_RIV], 1);
-                  if (s == 0) { s = 1; k = sp.ri_strain; *rec_byte(d, i, RB_STRAIN) = (int8_t)k; atomicAdd(&lc[L_RIP], 1); }
+                  if (s == 0) {
+                    s = 1; k = sp.ri_strain;
+                    *rec_byte(d, i, RB_STRAIN) = (int8_t)k; *rec_byte(d, i, RB_STATE) = 1;
+                    atomicAdd(&lc[L_RIP], 1);
+                  }
                 }
                 if ((uint64_t)o.y < thr_ipv) { atomicAdd(&lc[L_IPV], 1); ipv_new |= 1u << (4 * u + a); }
               }
@@ -1010,6 +1017,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                     s = 1;
                     k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
                     *rec_byte(d, i, RB_STRAIN) = (int8_t)k;
+                    *rec_byte(d, i, RB_STATE) = 1;
                     atomicAdd(&lc[L_SIAPK0 + k], 1);
                     if (last) atomicAdd(&lc[L_SIAP], 1);
                   }
@@ -1898,6 +1906,7 @@ __device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, cons
   }
   d.state[slot] = 1;
   *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
+  *rec_byte(d, slot, RB_STATE) = 1;
   return k;
 }
 
