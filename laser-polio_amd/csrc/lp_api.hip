@@ -107,6 +107,7 @@ struct lp_ctx {
   int32_t list_tick = -1;                          // the tick whose E / I list is complete (-1: none)
   bool rec_dirty = false;                          // agent records newer than the bound columns
   std::vector<int32_t> day_flags_h;
+  std::vector<int32_t> ri_cnt_h;
   std::string err;
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -224,17 +225,24 @@ int rec_grid(const lp_ctx* c) {
   return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
 }
 
+// Bound columns <- device-held state: agent records and actual ri_timer values.
+void enqueue_sync_columns(lp_ctx* c) {
+  hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, c->stream, c->d);
+  hipLaunchKernelGGL(lp::k_ri_sync, dim3(c->n_spans), dim3(lp::WG), 0, c->stream, c->d);
+  hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, c->stream, c->d, -2);
+}
+
 // The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
 // recounts every carried statistic and the paralysis counts); mask: stages;
 // tx_adjust: apply today's transmission exposures to the log; advance: move
 // the day counter at the end.
-int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
+int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool advance) {
   const int recount = kind == lp::KIND_RECOUNT ? 1 : 0;
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
   // a RECOUNT pass reads the bound columns: bring them up to date first
-  if (recount && c->rec_dirty) hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
+  if (recount && c->rec_dirty) enqueue_sync_columns(c);
   hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
   const bool ev = c->timing;
   if (ev) {
@@ -254,7 +262,13 @@ int enqueue_day(lp_ctx* c, int kind, int mask, int tx_adjust, bool advance) {
     default: launch_agents_u<16>(c, s, kind, mask); break;
   }
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
-  if (recount) hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
+  if (recount) {
+    hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
+    // the pass left ri_timer actual as of today if it ran RI; otherwise it is actual as
+    // of the latest applied RI step, as before the pass
+    const bool ri = host_stage_flags(c->day_flags_h[t], mask) & LP_DAY_RI;
+    hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, s, d, ri ? t : -2);
+  }
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
                      kind == lp::KIND_PLAIN_LIST ? 1 : kind >= lp::KIND_FULL_LIST ? 2 : 0);
   if (mask & LP_STAGE_TRANSMISSION) {
@@ -448,6 +462,16 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.sel_list, d.sel_cap))) return rc;
   c->day_flags_h.resize(g.nt);
   HIPCHK(c, hipMemcpy(c->day_flags_h.data(), c->d.day_flags, g.nt * 4, hipMemcpyDeviceToHost));
+  {
+    // RI days among ticks 0..t, for the arithmetic on the reference-tick ri_timer values
+    c->ri_cnt_h.assign((size_t)g.nt + 1, 0);
+    for (int32_t t = 0; t < g.nt; ++t) c->ri_cnt_h[t + 1] = c->ri_cnt_h[t] + ((c->day_flags_h[t] & LP_DAY_RI) ? 1 : 0);
+    int32_t* rc_d = nullptr;
+    if ((rc = alloc(c, &rc_d, (size_t)g.nt + 1))) return rc;
+    HIPCHK(c, hipMemcpyAsync(rc_d, c->ri_cnt_h.data(), ((size_t)g.nt + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    d.ri_cnt = rc_d;
+    if ((rc = alloc(c, &d.ri_ref, 2))) return rc;
+  }
   std::vector<int32_t> stop(S, INT_MAX);
   HIPCHK(c, hipMemcpyAsync(d.stop_day, stop.data(), S * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -464,14 +488,14 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   int rc;
   int32_t t = t_begin;
   if (t == 0) {
-    if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true))) return rc;
+    if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true))) return rc;
     c->recount_pending = false;
     c->list_tick = 1;
     t = 1;
   }
   for (; t < t_end; ++t) {
     if (c->recount_pending) {
-      if ((rc = enqueue_day(c, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true))) return rc;
+      if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true))) return rc;
       c->recount_pending = false;
       c->list_tick = t + 1;
       continue;
@@ -486,7 +510,7 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       if (!c->graph_ready[kind]) {
         hipGraph_t graph;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_day(c, kind, LP_STAGE_ALL, 1, true);
+        rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true);
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (rc) return rc;
         if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -496,7 +520,7 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       }
       HIPCHK(c, hipGraphLaunch(c->graph[kind], c->stream));
     } else {
-      if ((rc = enqueue_day(c, kind, LP_STAGE_ALL, 1, true))) return rc;
+      if ((rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true))) return rc;
     }
   }
   return LP_OK;
@@ -513,7 +537,7 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   const int recount = (log_only || c->recount_pending) ? 1 : 0;
   c->recount_pending = false;
   c->list_tick = -1;   // stage calls do not keep the E / I list
-  return enqueue_day(c, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+  return enqueue_day(c, t, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
 }
 
 int lp_invalidate(lp_ctx* c) {
@@ -535,7 +559,7 @@ int lp_log(lp_ctx* c, int32_t t) { return lp_step_stage(c, t, LP_STAGE_LOG); }
 int lp_sync(lp_ctx* c) {
   if (!c) return LP_ERR_ARG;
   if (c->prepared && c->rec_dirty) {
-    hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, c->stream, c->d);
+    enqueue_sync_columns(c);
     HIPCHK(c, hipGetLastError());
     c->rec_dirty = false;
   }

@@ -91,6 +91,8 @@ struct Dev {
   int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
   float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
+  const int32_t* ri_cnt;   // [nt + 1] ri_cnt[t + 1] = RI days among ticks 0..t
+  int32_t* ri_ref;         // [2] ri_timer reference tick; latest tick whose RI step is applied
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -416,6 +418,43 @@ __device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t li
   }
   if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
   return surv;
+}
+
+// ri_timer <- actual values: for live agents that are not chronically missed,
+// minus RI's decrements from the reference tick through the latest applied RI
+// step (one workgroup per span); k_set_ri_ref then moves the reference there.
+__global__ void __launch_bounds__(WG) k_ri_sync(Dev d) {
+  const int4 span = d.spans[blockIdx.x];
+  const int32_t dc = d.ri_cnt[d.ri_ref[1] + 1] - d.ri_cnt[d.ri_ref[0] + 1];
+  const int32_t shift = d.sims[span.w].ri_step * dc;
+  if (shift == 0) return;
+  for (int u = 0; u < span.z; ++u) {
+    const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)threadIdx.x * 4;
+    const uint32_t w = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
+    if ((w & 0x80808080u) == 0x80808080u) continue;   // no live agent in the quad
+    const uint32_t mw = reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2];
+    const uint2 r = *reinterpret_cast<const uint2*>(d.ri_timer + i0);
+    uint2 n = r;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (byte_of(w, a) < 0 || byte_of(mw, a) == 1) continue;
+      const int sh16 = 16 * (a & 1);
+      uint32_t& h = a < 2 ? n.x : n.y;
+      const int16_t v = (int16_t)((int16_t)((h >> sh16) & 0xffffu) - shift);
+      h = (h & ~(0xffffu << sh16)) | ((uint32_t)(uint16_t)v << sh16);
+    }
+    if (n.x != r.x || n.y != r.y) *reinterpret_cast<uint2*>(d.ri_timer + i0) = n;
+  }
+}
+
+// through >= -1: ri_timer is actual as of tick `through` (after a RECOUNT pass);
+// through < -1: as of the latest applied RI step (after k_ri_sync).
+__global__ void k_set_ri_ref(Dev d, int32_t through) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int32_t v = through >= -1 ? through : d.ri_ref[1];
+    d.ri_ref[0] = v;
+    d.ri_ref[1] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -831,6 +870,13 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const EIConst ec = ei_const(sp);
   const uint64_t thr_ri = RI ? d.thr_ri[node] : 0, thr_ipv = RI ? d.thr_ipv[node] : 0;
   const int nxt = (t + 1) & 1;
+  // ri_timer holds, for live agents not chronically missed, the timer as of tick
+  // ri_ref[0]: RI's decrements since then are applied by arithmetic (sh_prev: RI days
+  // before today, for births and deaths; sh_today: through today, for RI)
+  const int32_t cnt_ref = d.ri_cnt[d.ri_ref[0] + 1];
+  const int32_t sh_prev = sp.ri_step * (d.ri_cnt[t] - cnt_ref);
+  const int32_t sh_today = sp.ri_step * (d.ri_cnt[t + 1] - cnt_ref);
+  if (RI && blockIdx.x == 0 && tid == 0) d.ri_ref[1] = t;
   // does any of today's campaigns target this node?  (uniform over the workgroup)
   bool sia_here = false;
   if (SIA)
@@ -909,25 +955,20 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
             if ((alive >> (8 * a + 7)) & 1u) { ok = ok && dod > t; qmin = dod < qmin ? dod : qmin; }
           }
         }
-        uint2 rn = r2;
         if (RI && ok) {
           const int32_t step = sp.ri_step;
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
             if (!((alive >> (8 * a + 7)) & 1u) || byte_of(mw[u], a) == 1) continue;
-            const int sh16 = 16 * (a & 1);
-            const uint32_t rw = a < 2 ? rn.x : rn.y;
-            const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - step);
+            const uint32_t rw = a < 2 ? r2.x : r2.y;
+            const int16_t timer = (int16_t)((int16_t)((rw >> (16 * (a & 1))) & 0xffffu) - sh_today);
             const bool eligible = (t == step) ? (timer <= 0 && timer >= -step) : (t > step && timer <= 0 && timer > -step);
             ok = ok && !eligible;
-            const uint32_t nw = (rw & ~(0xffffu << sh16)) | ((uint32_t)(uint16_t)timer << sh16);
-            if (a < 2) rn.x = nw; else rn.y = nw;
           }
         }
         if (ok) {
           fast = true;
           dmin = qmin;
-          if (RI) r2 = rn;
           q = has_ei_byte(w0);
         }
       }
@@ -947,10 +988,14 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               d.dod[i] = dod;
               dmin = dod < dmin ? dod : dmin;
               d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
-              if (sp.newborn_ri_timer >= 0) {
-                d.ri_timer[i] = (int16_t)sp.newborn_ri_timer;
+              {                                           // its ri_timer, as of the reference tick
+                const int16_t base = sp.newborn_ri_timer >= 0
+                                         ? (int16_t)sp.newborn_ri_timer
+                                         : (RI ? (int16_t)(((a < 2 ? r2.x : r2.y) >> sh16) & 0xffffu) : d.ri_timer[i]);
+                const int16_t col = (int16_t)(base + sh_prev);
+                if (sp.newborn_ri_timer >= 0 || col != base) d.ri_timer[i] = col;
                 if (RI) {
-                  const uint32_t hv = (uint32_t)(uint16_t)sp.newborn_ri_timer << sh16;
+                  const uint32_t hv = (uint32_t)(uint16_t)col << sh16;
                   if (a < 2) r2.x = (r2.x & ~m16) | hv; else r2.y = (r2.y & ~m16) | hv;
                 }
               }
@@ -963,6 +1008,12 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               const uint4 rd = d.rec[i];
               if ((int8_t)(rd.y >> 24) == 1) atomicAdd(&lc[L_DPP], 1);
               if ((int8_t)(rd.w >> 8) == 1) atomicAdd(&lc[L_DP], 1);
+              // its ri_timer stops here: store the actual value (chronically missed agents
+              // are never decremented)
+              if (sh_prev != 0 && (camp ? byte_of(mw[u], a) : (int8_t)d.missed[i]) != 1) {
+                const int16_t c = RI ? (int16_t)(((a < 2 ? r2.x : r2.y) >> sh16) & 0xffffu) : d.ri_timer[i];
+                d.ri_timer[i] = (int16_t)(c - sh_prev);
+              }
               if (s == 0) {
                 const int64_t w = weight_q24(d.risk[i]);
                 c_s -= 1; c_spos -= (w > 0); c_sw -= w; cw -= w;
@@ -983,9 +1034,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
             if (RI) {                                     // fast_ri (model.py:1826-1854)
               const int32_t step = sp.ri_step;
               const uint32_t rw = a < 2 ? r2.x : r2.y;
-              const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - step);
-              const uint32_t nw = (rw & ~m16) | ((uint32_t)(uint16_t)timer << sh16);
-              if (a < 2) r2.x = nw; else r2.y = nw;
+              const int16_t timer = (int16_t)((int16_t)((rw >> sh16) & 0xffffu) - sh_today);
               bool eligible = false;
               if (t == step) eligible = timer <= 0 && timer >= -step;
               else if (t > step) eligible = timer <= 0 && timer > -step;
@@ -1041,7 +1090,6 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       const bool in_ei = (m[u] >> lane) & 1ull;
       if (u < ng) {
         if (!in_ei && w2 != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w2;
-        if (RI && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
       }
       s_w0[u][tid] = w0; s_w1[u][tid] = w1;
       if constexpr (GW2 == G) s_w2[u][tid] = w2;
