@@ -44,18 +44,18 @@ def log(*a):
 def agent_pass_bytes(res, flags, t0, t1):
     """Algorithmic bytes of the agent pass over ticks [t0, t1).
 
-    Every day: N_live * 1 (state); per E agent 5 (strain 1, exposure timer read and
-    written 2+2); per I agent 9 (strain 1, infection timer read and written 2+2,
-    infectivity 4); per strain-0 I agent 3 more (paralysis timer 2 and
-    potentially_paralyzed 1, which DiseaseState reads for strain-0 infections only).
-    VD step days add the per-chunk earliest death dates (4 B per 256 slots), 6 B per
-    death (date_of_death, potentially_paralyzed, paralyzed) and 17 B per birth
-    (date_of_birth, date_of_death, node_id, ri_timer and state written, risk read); RI
-    days N_live * 4 (ri_timer read and written); SIA days N_live * 4 (date_of_birth);
-    RI or SIA days N_live * 1 (chronically_missed).  The susceptible statistics and the
-    earliest death dates are carried across days, so no day reads the risk or
-    date_of_death column of every agent.  N_live = live agents that day (results.pop);
-    E, I and strain-0 I are the counts at the start of the day (the previous row).
+    DiseaseState and the E / I tally run over the per-node E / I lists: per E or I agent
+    its list entry read and written back (4 + 4) and its state (1); per E agent 5 more
+    (strain, exposure timer read and written), per I agent 9 (strain, infection timer
+    read and written, infectivity), per strain-0 I agent 3 more (paralysis timer and
+    potentially_paralyzed, read by DiseaseState for strain-0 infections only).  VD step
+    days add the per-chunk earliest death dates (4 B per 256 slots), 6 B per death
+    (date_of_death, potentially_paralyzed, paralyzed) and 17 B per birth (date_of_birth,
+    date_of_death, node_id, ri_timer and state written, risk read).  RI and SIA days scan
+    every agent's state and chronically_missed (2 B); RI days add ri_timer (2 B, read:
+    the decrement is applied by arithmetic), SIA days date_of_birth (4 B).  N_live = live
+    agents that day (results.pop); E, I and strain-0 I are the counts at the start of the
+    day (the previous row).
     """
     from laser_polio_amd import engine as em
 
@@ -68,15 +68,15 @@ def agent_pass_bytes(res, flags, t0, t1):
     total = 0
     for t in range(t0, t1):
         n_live, f = int(pop[t]), int(flags[t])
-        b = 1 * n_live + 5 * int(n_e[t - 1]) + 9 * int(n_i[t - 1]) + 3 * int(n_i0[t - 1])
+        b = 14 * int(n_e[t - 1]) + 18 * int(n_i[t - 1]) + 3 * int(n_i0[t - 1])
         if f & em.DAY_VD_STEP:
             b += 4 * (n_live // 256) + 6 * int(deaths[t]) + 17 * int(births[t])
+        if f & (em.DAY_RI | em.DAY_SIA):
+            b += 2 * n_live
         if f & em.DAY_RI:
-            b += 4 * n_live
+            b += 2 * n_live
         if f & em.DAY_SIA:
             b += 4 * n_live
-        if f & (em.DAY_RI | em.DAY_SIA):
-            b += 1 * n_live
         total += b
     return total
 
