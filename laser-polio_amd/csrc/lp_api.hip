@@ -164,6 +164,8 @@ int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
 // Workgroups of k_ei (4 wavefronts each); each wavefront loops over list pieces.
 constexpr int kEiGrid = 2048;
+// Workgroups per node of k_ri_list (each takes every kRiSplit-th batch of 1024 candidates).
+constexpr int kRiSplit = 8;
 
 
 // Host-side mirror of the device day classification (k_agents::is_plain_day).
@@ -185,6 +187,13 @@ int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
          ((f & LP_DAY_SIA) ? lp::FULL_SIA : 0);
 }
 
+// FULL-from-list day kinds whose RI runs from the candidate lists (RI without SIA).
+bool sparse_ri(int kind) {
+  if (kind < lp::KIND_FULL_LIST) return false;
+  const int f = kind - lp::KIND_FULL_LIST + 1;
+  return (f & lp::FULL_RI) && !(f & lp::FULL_SIA);
+}
+
 // Agent pass of one day: k_plain over U = tiles per span on plain days,
 // k_full<kind> on FULL days, k_agents in groups of 2 tiles when recounting.
 template <int U>
@@ -204,6 +213,13 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
     case lp::KIND_FULL_LIST + 3: case lp::KIND_FULL_LIST + 4: case lp::KIND_FULL_LIST + 5:
     case lp::KIND_FULL_LIST + 6:
       hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask);
+      if (sparse_ri(kind)) {
+        // RI without SIA: the VD step alone (RI masked out), then RI from the candidate lists
+        if ((kind - lp::KIND_FULL_LIST + 1) & lp::FULL_VD)
+          hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, mask & ~LP_STAGE_RI);
+        hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);
+        break;
+      }
       switch (kind - lp::KIND_FULL_LIST + 1) {
         case 1: hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, mask); break;
         case 2: hipLaunchKernelGGL((lp::k_full<2, false>), g, b, 0, s, c->d, mask); break;
@@ -241,8 +257,10 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
-  // a RECOUNT pass reads the bound columns: bring them up to date first
+  // a RECOUNT pass reads the bound columns: bring them up to date first; it rebuilds the
+  // RI candidate lists
   if (recount && c->rec_dirty) enqueue_sync_columns(c);
+  if (recount) HIPCHK(c, hipMemsetAsync(d.ri_lcnt, 0, 2 * (size_t)N * sizeof(int32_t), s));
   hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
   const bool ev = c->timing;
   if (ev) {
@@ -269,8 +287,10 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
     const bool ri = host_stage_flags(c->day_flags_h[t], mask) & LP_DAY_RI;
     hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, s, d, ri ? t : -2);
   }
+  // span partials: none written on list-only days (plain, RI-only from the lists)
+  const bool spans_written = kind != lp::KIND_PLAIN_LIST && kind != lp::KIND_FULL_LIST + lp::FULL_RI - 1;
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
-                     kind == lp::KIND_PLAIN_LIST ? 1 : kind >= lp::KIND_FULL_LIST ? 2 : 0);
+                     kind < lp::KIND_PLAIN_LIST ? 0 : spans_written ? 2 : 1);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
     hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
@@ -439,6 +459,8 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.ei_piece_off, (size_t)N + 1))) return rc;
   if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.beta_f, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.ri_list, 2 * (size_t)g.capacity))) return rc;
+  if ((rc = alloc(c, &d.ri_lcnt, 2 * (size_t)N))) return rc;
   HIPCHK(c, hipMemsetAsync(d.beta_f, 0, (size_t)N * LP_MAX_STRAINS * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(d.ei_cnt, 0, 2 * (size_t)N * sizeof(int32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(d.ei_piece_off, 0, ((size_t)N + 1) * sizeof(int32_t), c->stream));

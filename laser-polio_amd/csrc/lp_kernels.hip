@@ -93,6 +93,8 @@ struct Dev {
   float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
   const int32_t* ri_cnt;   // [nt + 1] ri_cnt[t + 1] = RI days among ticks 0..t
   int32_t* ri_ref;         // [2] ri_timer reference tick; latest tick whose RI step is applied
+  uint32_t* ri_list;       // [2][capacity] per-node lists of RI candidates (slots)
+  int32_t* ri_lcnt;        // [2][nodes] their lengths
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
@@ -477,7 +479,7 @@ __device__ __forceinline__ uint32_t* ei_list_of(const Dev& d, int b, int node) {
 // Appends slots i0 + a (bit a of `bits`) of every lane to node `node`'s list b:
 // one atomic per wave, ranks from per-position ballots.  Called by all 64 lanes
 // of the wave together.
-__device__ __forceinline__ void wave_append_quad(const Dev& d, int b, int node, int64_t i0, uint32_t bits, int lane) {
+__device__ __forceinline__ void wave_append_quad_to(uint32_t* L, int32_t* cnt, int64_t i0, uint32_t bits, int lane) {
   const uint64_t lt = (1ull << lane) - 1ull;
   uint64_t mk[4];
   int tot = 0;
@@ -485,14 +487,42 @@ __device__ __forceinline__ void wave_append_quad(const Dev& d, int b, int node, 
   for (int a = 0; a < 4; ++a) { mk[a] = __ballot((bits >> a) & 1u); tot += __popcll(mk[a]); }
   if (tot == 0) return;
   int base = 0;
-  if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, tot);
+  if (lane == 0) base = atomicAdd(cnt, tot);
   base = __shfl(base, 0, 64);
-  uint32_t* L = ei_list_of(d, b, node);
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     if ((bits >> a) & 1u) L[base + __popcll(mk[a] & lt)] = (uint32_t)(i0 + a);
     base += __popcll(mk[a]);
   }
+}
+
+__device__ __forceinline__ void wave_append_quad(const Dev& d, int b, int node, int64_t i0, uint32_t bits, int lane) {
+  wave_append_quad_to(ei_list_of(d, b, node), d.ei_cnt + (size_t)b * d.n_nodes + node, i0, bits, lane);
+}
+
+// Remove an exposed agent's weight w from its chunk and from span s.
+__device__ __forceinline__ void unweight_in(const Dev& d, int64_t s, int64_t slot, int64_t w) {
+  const unsigned long long dw = (unsigned long long)(-w);
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
+  atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
+}
+
+// The span of node `node` holding `slot`.
+__device__ __forceinline__ int64_t span_of(const Dev& d, int node, int64_t slot) {
+  const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
+  return d.node_span_begin[node] + (tile - first) / d.tpw;
+}
+
+// RI candidate lists: ri_list[b] holds, per node, the slots of live agents that are
+// not chronically missed and whose RI window can still open (at ri_list + b *
+// capacity + seg_begin[n], ri_lcnt[b * n_nodes + n] of them).  Buffer b = (RI days
+// before the next RI day) & 1: births append to it, the next RI day reads it and
+// keeps its candidates in the other buffer; k_node_reduce empties the buffer read.
+__device__ __forceinline__ uint32_t* ri_list_of(const Dev& d, int b, int node) {
+  return d.ri_list + (size_t)b * (size_t)d.capacity + d.seg_begin[node];
+}
+__device__ __forceinline__ int32_t* ri_cnt_of(const Dev& d, int b, int node) {
+  return d.ri_lcnt + (size_t)b * d.n_nodes + node;
 }
 
 // Appends `slot` of each active lane with `flag` to node `node`'s list b (every
@@ -809,6 +839,139 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
+// RI from the candidate lists (fast_ri, model.py:1826-1854), gridDim.y workgroups
+// per node, four candidates per thread at a time.  ri_timer is a reference-tick value
+// (timer today = col - ri_step * RI days since the reference), so only agents whose
+// window can open are visited: each candidate is checked against today's window,
+// drawn if eligible (vaccinated; susceptibles exposed with the RI strain, IPV
+// doses), and kept for later RI days while its window can still open.  Deltas go
+// to node_acc (k_node_reduce adds them to the node tallies) and to the chunk /
+// span weights.  Runs after k_ei (DiseaseState reads ipv_protected before RI).
+__global__ void __launch_bounds__(WG) k_ri_list(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int sim = d.node_sim[n];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if (sim_stopped(d, sim, t) || !(flags & LP_DAY_RI)) return;
+  if (n == 0 && blockIdx.y == 0 && tid == 0) d.ri_ref[1] = t;
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t step = sp.ri_step;
+  const int32_t sh_today = step * (d.ri_cnt[t + 1] - d.ri_cnt[d.ri_ref[0] + 1]);
+  const uint64_t thr_ri = d.thr_ri[n], thr_ipv = d.thr_ipv[n];
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  const int rk = sp.ri_strain & (LP_MAX_STRAINS - 1);
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const int A = d.ri_cnt[t] & 1, B = A ^ 1, nxt = (t + 1) & 1;
+  const int32_t cnt = *ri_cnt_of(d, A, n);
+  const uint32_t* L = ri_list_of(d, A, n);
+  uint32_t* LB = ri_list_of(d, B, n);
+  __shared__ int32_t s_c[8];   // vaccinated, exposed, ipv, S, S with positive weight
+  if (tid < 8) s_c[tid] = 0;
+  __syncthreads();
+  int32_t c_riv = 0, c_rip = 0, c_ipv = 0, c_spos = 0;
+  // blockIdx.y of gridDim.y workgroups of the node takes every gridDim.y-th batch
+  for (int32_t j0 = (int32_t)blockIdx.y * 4 * WG; j0 < cnt; j0 += (int32_t)gridDim.y * 4 * WG) {
+    uint32_t sl[4];
+    bool v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t j = j0 + q * WG + tid;
+      v[q] = j < cnt;
+      const uint32_t raw = v[q] ? L[j] : 0u;
+      if (v[q] && (int64_t)raw >= d.capacity) { atomicOr(d.err, ERR_LIST); v[q] = false; }
+      sl[q] = v[q] ? raw : 0u;
+    }
+    int8_t st[4];
+    uint8_t ms[4];
+    int16_t col[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      st[q] = v[q] ? d.state[sl[q]] : (int8_t)-1;
+      ms[q] = v[q] ? d.missed[sl[q]] : (uint8_t)1;
+      col[q] = v[q] ? d.ri_timer[sl[q]] : (int16_t)0;
+    }
+    uint32_t keep = 0, expo = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int8_t s = st[q];
+      if (s < 0 || ms[q] == 1) continue;                // dead or never reached by RI
+      const int16_t timer = (int16_t)(col[q] - sh_today);
+      if (timer > -step) keep |= 1u << q;
+      bool eligible = false;
+      if (t == step) eligible = timer <= 0 && timer >= -step;
+      else if (t > step) eligible = timer <= 0 && timer > -step;
+      if (!eligible) continue;
+      const uint32_t slot = sl[q];
+      const uint4 o = draw(seed, (uint64_t)((int64_t)slot - sbase), t, ST_RI);
+      if ((uint64_t)o.x < thr_ri) {
+        c_riv += 1;
+        if (s == 0) {                                     // exposed with the RI strain, counted as E today
+          const int64_t w = weight_q24(d.risk[slot]);
+          unweight_in(d, span_of(d, n, slot), slot, w);
+          d.state[slot] = 1;
+          *rec_byte(d, slot, RB_STRAIN) = (int8_t)rk;
+          *rec_byte(d, slot, RB_STATE) = 1;
+          c_rip += 1;
+          c_spos += (w > 0);
+          expo |= 1u << q;
+        }
+      }
+      if ((uint64_t)o.y < thr_ipv) { c_ipv += 1; *rec_byte(d, slot, RB_IPV) = 1; }
+    }
+    // survivors -> the other candidate buffer (one atomic per wave and batch),
+    // exposures -> tomorrow's E / I list
+    {
+      uint64_t mk[4];
+      int rank[4], tot = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        mk[q] = __ballot((keep >> q) & 1u);
+        rank[q] = tot + __popcll(mk[q] & ((1ull << lane) - 1ull));
+        tot += __popcll(mk[q]);
+      }
+      if (tot) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(ri_cnt_of(d, B, n), tot);
+        base = __shfl(base, 0, 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((keep >> q) & 1u) LB[base + rank[q]] = sl[q];
+      }
+    }
+    if (__ballot(expo != 0)) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) wave_append_slot(d, nxt, n, (int64_t)sl[q], (expo >> q) & 1u, lane);
+    }
+  }
+  const int32_t a0 = wave_sum_i32(c_riv), a1 = wave_sum_i32(c_rip), a2 = wave_sum_i32(c_ipv),
+                a3 = wave_sum_i32(c_spos);
+  if (lane == 0) {
+    if (a0) atomicAdd(&s_c[0], a0);
+    if (a1) atomicAdd(&s_c[1], a1);
+    if (a2) atomicAdd(&s_c[2], a2);
+    if (a3) atomicAdd(&s_c[3], a3);
+  }
+  __syncthreads();
+  if (tid < 8) {
+    int f = -1;
+    int64_t val = 0;
+    switch (tid) {
+      case 0: f = P_RIV; val = s_c[0]; break;
+      case 1: f = P_RIP; val = s_c[1]; break;
+      case 2: f = P_IPV; val = s_c[2]; break;
+      case 3: f = P_S; val = -(int64_t)s_c[1]; break;
+      case 4: f = P_SPOS; val = -(int64_t)s_c[3]; break;
+      case 5: if (do_tally) { f = P_E0 + rk; val = s_c[1]; } break;
+      default: break;
+    }
+    if (f >= 0 && val != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.node_acc + (size_t)n * NPART + f), (unsigned long long)val);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FULL days (VD step, RI and / or SIA; F = OR of the FULL_* bits).  EI: the
 // pass also runs DiseaseState and the E / I tally of its E and I agents (when no
 // E / I list is available); otherwise k_ei has done that already and, on VD-only
@@ -939,6 +1102,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       int64_t cw = 0;
       int32_t dmin = INT_MAX;   // earliest death date among the quad's agents alive after today
       uint32_t cnew = 0;        // agents exposed by RI / SIA today (bit a)
+      uint32_t rcand = 0;       // RI: agents whose window can still open after today (bit a)
       bool q = false;
       // Fast path for the common quad: all four slots in the live range, no newborn, no
       // death due, no SIA here today and no agent entering its RI window; the only change
@@ -964,6 +1128,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
             const int16_t timer = (int16_t)((int16_t)((rw >> (16 * (a & 1))) & 0xffffu) - sh_today);
             const bool eligible = (t == step) ? (timer <= 0 && timer >= -step) : (t > step && timer <= 0 && timer > -step);
             ok = ok && !eligible;
+            if (timer > -step) rcand |= 1u << a;
           }
         }
         if (ok) {
@@ -973,6 +1138,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         }
       }
       if (!fast && u < ng && i0 < live_hi) {
+        rcand = 0;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int64_t i = i0 + a;
@@ -1080,11 +1246,19 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
               if (do_tally) atomicAdd(&lc[L_E0 + k], 1);
             }
           }
+          if (RI && s >= 0 && byte_of(mw[u], a) != 1) {
+            const int16_t tm = (int16_t)((int16_t)(((a < 2 ? r2.x : r2.y) >> sh16) & 0xffffu) - sh_today);
+            if (tm > -sp.ri_step) rcand |= 1u << a;
+          }
           w2 = (w2 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
         }
       }
       if (!VD) w1 = w0;
       if (u < ng) s_sv[g0 + u][tid] = (uint8_t)cnew;
+      if (RI) {                                           // candidates for later RI days
+        const int b = (d.ri_cnt[t + 1]) & 1;
+        wave_append_quad_to(ri_list_of(d, b, node), ri_cnt_of(d, b, node), i0, u < ng ? rcand : 0u, lane);
+      }
       m[u] = __ballot(q && need_ei);
       n_ei += __popcll(m[u]);
       const bool in_ei = (m[u] >> lane) & 1ull;
@@ -1157,6 +1331,20 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         for (int a = 0; a < 4; ++a)
           if ((bits >> a) & 1u) L[pos++] = (uint32_t)(i0 + a);
       }
+    }
+  }
+  if (VD) {
+    // today's newborns in this span join the RI candidates of the next RI day (today's, if
+    // today is one: births precede RI); a dense RI pass today lists its own candidates
+    const int64_t s_lo = (int64_t)span0.y * LP_TILE, s_hi = s_lo + (int64_t)span0.z * LP_TILE;
+    const int64_t lo = nb_lo > s_lo ? nb_lo : s_lo, hi = live_hi < s_hi ? live_hi : s_hi;
+    if (!RI && hi > lo) {
+      __shared__ int32_t s_base;
+      const int b = d.ri_cnt[t] & 1;
+      if (tid == 0) s_base = atomicAdd(ri_cnt_of(d, b, node), (int32_t)(hi - lo));
+      __syncthreads();
+      uint32_t* L = ri_list_of(d, b, node);
+      for (int64_t x = lo + tid; x < hi; x += WG) L[s_base + (x - lo)] = (uint32_t)x;
     }
   }
   {
@@ -1469,6 +1657,20 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
           if (tv[u] && i0 + a < live_hi && (s == 1 || s == 2)) bits |= 1u << a;
         }
         wave_append_quad(d, (t + 1) & 1, node, i0, bits, lane);
+        // RI candidates: live agents not chronically missed whose window can still open
+        // (ri_timer is actual here; re-read after this pass's own RI step)
+        uint32_t rb = 0;
+        if (tv[u]) {
+          const uint2 rr = *reinterpret_cast<const uint2*>(d.ri_timer + i0);
+          const uint32_t mm = reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int16_t tm = (int16_t)(((a < 2 ? rr.x : rr.y) >> (16 * (a & 1))) & 0xffffu);
+            if (i0 + a < live_hi && byte_of(wf, a) >= 0 && byte_of(mm, a) != 1 && tm > -sp.ri_step) rb |= 1u << a;
+          }
+        }
+        const int rbuf = d.ri_cnt[t + 1] & 1;
+        wave_append_quad_to(ri_list_of(d, rbuf, node), ri_cnt_of(d, rbuf, node), i0, rb, lane);
       }
       // chunk weight sums: rewritten when recounting, adjusted by their delta otherwise
       // (the condition is uniform over the workgroup: wave_sum needs every lane)
@@ -1573,6 +1775,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   }
   __syncthreads();
   if (tid < NPART) d.node_tally[(size_t)n * NPART + tid] = tally[tid];
+  if (tid == 0 && (flags & LP_DAY_RI)) *ri_cnt_of(d, d.ri_cnt[t] & 1, n) = 0;   // consumed by today's RI
   if (tid < LP_MAX_STRAINS)   // infectious weight per strain as the network mix reads it
     d.beta_f[(size_t)n * LP_MAX_STRAINS + tid] = tid < d.K ? (float)tally[P_IB0 + tid] * 5.9604644775390625e-08f : 0.0f;
   if (flags & LP_DAY_TX) {
@@ -1923,19 +2126,6 @@ __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
     if (prev == old) return true;
     old = prev;
   }
-}
-
-// Remove an exposed agent's weight w from its chunk and from span s.
-__device__ __forceinline__ void unweight_in(const Dev& d, int64_t s, int64_t slot, int64_t w) {
-  const unsigned long long dw = (unsigned long long)(-w);
-  atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
-  atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)s * NPART + P_SW), dw);
-}
-
-// The span of node `node` holding `slot`.
-__device__ __forceinline__ int64_t span_of(const Dev& d, int node, int64_t slot) {
-  const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
-  return d.node_span_begin[node] + (tile - first) / d.tpw;
 }
 
 // Remove an exposed agent's weight from its chunk and its span.
