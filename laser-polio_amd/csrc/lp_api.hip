@@ -252,7 +252,7 @@ void enqueue_sync_columns(lp_ctx* c) {
 // recounts every carried statistic and the paralysis counts); mask: stages;
 // tx_adjust: apply today's transmission exposures to the log; advance: move
 // the day counter at the end.
-int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool advance) {
+int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool advance, bool begin) {
   const int recount = kind == lp::KIND_RECOUNT ? 1 : 0;
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
@@ -261,7 +261,7 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   // RI candidate lists
   if (recount && c->rec_dirty) enqueue_sync_columns(c);
   if (recount) HIPCHK(c, hipMemsetAsync(d.ri_lcnt, 0, 2 * (size_t)N * sizeof(int32_t), s));
-  hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
+  if (begin) hipLaunchKernelGGL(lp::k_day_begin, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask);
   const bool ev = c->timing;
   if (ev) {
     if (c->ev_used + 2 > c->ev.size()) {
@@ -459,6 +459,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.ei_piece_off, (size_t)N + 1))) return rc;
   if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.beta_f, (size_t)N * LP_MAX_STRAINS))) return rc;
+  if ((rc = alloc(c, &d.sim_cases, (size_t)S))) return rc;
   if ((rc = alloc(c, &d.ri_list, 2 * (size_t)g.capacity))) return rc;
   if ((rc = alloc(c, &d.ri_lcnt, 2 * (size_t)N))) return rc;
   HIPCHK(c, hipMemsetAsync(d.beta_f, 0, (size_t)N * LP_MAX_STRAINS * sizeof(float), c->stream));
@@ -506,18 +507,22 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   if (!c->prepared) return fail(c, LP_ERR_STATE, "lp_run_days before lp_prepare");
   if (t_begin < 0 || t_end > c->cfg.nt || t_begin > t_end) return fail(c, LP_ERR_ARG, "bad tick range");
   if (t_begin == t_end) return LP_OK;
-  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t_begin);
+  // inside the call each day's k_day_end begins the next day; only the first is begun here
+  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t_begin, t_end);
+  bool first = true;
   int rc;
   int32_t t = t_begin;
   if (t == 0) {
-    if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true))) return rc;
+    if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_LOG, 0, true, first))) return rc;
+    first = false;
     c->recount_pending = false;
     c->list_tick = 1;
     t = 1;
   }
   for (; t < t_end; ++t) {
     if (c->recount_pending) {
-      if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true))) return rc;
+      if ((rc = enqueue_day(c, t, lp::KIND_RECOUNT, LP_STAGE_ALL, 1, true, first))) return rc;
+      first = false;
       c->recount_pending = false;
       c->list_tick = t + 1;
       continue;
@@ -528,11 +533,11 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       else if (kind >= lp::KIND_FULL && kind < lp::KIND_RECOUNT) kind = lp::KIND_FULL_LIST + kind - lp::KIND_FULL;
     }
     c->list_tick = t + 1;   // every day kind run here completes the next tick's list
-    if (use_graph && !c->timing) {
+    if (use_graph && !c->timing && !first) {
       if (!c->graph_ready[kind]) {
         hipGraph_t graph;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true);
+        rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true, false);
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (rc) return rc;
         if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -542,8 +547,9 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       }
       HIPCHK(c, hipGraphLaunch(c->graph[kind], c->stream));
     } else {
-      if ((rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true))) return rc;
+      if ((rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true, first))) return rc;
     }
+    first = false;
   }
   return LP_OK;
 }
@@ -553,13 +559,14 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   if (!c->prepared) return fail(c, LP_ERR_STATE, "step before lp_prepare");
   if (t < 0 || t >= c->cfg.nt) return fail(c, LP_ERR_ARG, "bad tick");
   if (mask & ~LP_STAGE_ALL) return fail(c, LP_ERR_ARG, "bad stage mask");
-  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t);
+  hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t, 0);
   const bool fused_tx_log = (mask & LP_STAGE_TRANSMISSION) && (mask & LP_STAGE_LOG);
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
   const int recount = (log_only || c->recount_pending) ? 1 : 0;
   c->recount_pending = false;
   c->list_tick = -1;   // stage calls do not keep the E / I list
-  return enqueue_day(c, t, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false);
+  return enqueue_day(c, t, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false,
+                     true);
 }
 
 int lp_invalidate(lp_ctx* c) {

@@ -91,6 +91,7 @@ struct Dev {
   int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
   float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
+  int32_t* sim_cases;      // [sims] 1 when a node of the sim logged E or I today
   const int32_t* ri_cnt;   // [nt + 1] ri_cnt[t + 1] = RI days among ticks 0..t
   int32_t* ri_ref;         // [2] ri_timer reference tick; latest tick whose RI step is applied
   uint32_t* ri_list;       // [2][capacity] per-node lists of RI candidates (slots)
@@ -117,7 +118,10 @@ __device__ __forceinline__ int32_t stage_day_flags(int32_t flags, int mask) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_set_day(Dev d, int32_t t) { if (threadIdx.x == 0 && blockIdx.x == 0) d.day[0] = t; }
+// day[0] = current tick; day[1] = end of the current lp_run_days call (0 outside one).
+__global__ void k_set_day(Dev d, int32_t t, int32_t until) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) { d.day[0] = t; d.day[1] = until; }
+}
 
 // Kaplan-Meier style age at death (days) for an agent of `age` days, from a
 // cumulative deaths table by year of age: year of death drawn in proportion to
@@ -145,21 +149,16 @@ __device__ int32_t km_age_at_death(const int64_t* cd, uint64_t seed, uint64_t sl
 }
 
 // ---------------------------------------------------------------------------
-// Per node: births, per-day resets, early-stop rule.
-__global__ void k_day_begin(Dev d, int mask) {
-  const int32_t t = d.day[0];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n == 0) d.sel_count[0] = 0;
-  if (n >= d.n_nodes) return;
+// Per node: births, per-day resets (model.py:1711-1719).
+__device__ void day_begin_node(const Dev& d, int n, int32_t t, int mask) {
   const int sim = d.node_sim[n];
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
   for (int k = 0; k < LP_MAX_STRAINS; ++k) d.newexp[n * LP_MAX_STRAINS + k] = 0;
   d.n_uniq[n] = 0; d.size[n] = 0; d.take_all[n] = 0; d.m_round[n] = 0;
-  d.ei_cnt[(size_t)((t + 1) & 1) * d.n_nodes + n] = 0;
-  for (int f = 0; f < NPART; ++f) d.node_acc[(size_t)n * NPART + f] = 0;
   d.births[n] = 0;
   d.nb_begin[n] = d.live_end[n];
+  d.ei_cnt[(size_t)((t + 1) & 1) * d.n_nodes + n] = 0;
   if (sim_stopped(d, sim, t)) return;
   if ((flags & LP_DAY_VD_STEP) && t >= 1) {
     const int32_t pop_prev = d.r_pop[(size_t)(t - 1) * d.n_nodes + n];
@@ -176,11 +175,36 @@ __global__ void k_day_begin(Dev d, int mask) {
     d.births[n] = b;
     d.live_end[n] += b;
   }
-  if ((flags & LP_DAY_STOPCHK) && t >= 1 && n == sp.node_begin && d.stop_day[sim] == INT_MAX) {
-    int64_t any = 0;
-    const size_t row = (size_t)(t - 1) * d.n_nodes;
-    for (int j = sp.node_begin; j < sp.node_begin + sp.n_nodes; ++j) any += d.r_E[row + j] + d.r_I[row + j];
+}
+
+// Per sim: the early-stop rule (model.py:788-795) on the previous day's E and I;
+// `cases` < 0: sum the previous day's rows, else whether any was positive.
+__device__ void day_stop_check(const Dev& d, int sim, int32_t t, int mask, int cases) {
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  if ((flags & LP_DAY_STOPCHK) && t >= 1 && d.stop_day[sim] == INT_MAX) {
+    int64_t any = cases;
+    if (cases < 0) {
+      any = 0;
+      const size_t row = (size_t)(t - 1) * d.n_nodes;
+      for (int j = sp.node_begin; j < sp.node_begin + sp.n_nodes; ++j) any += d.r_E[row + j] + d.r_I[row + j];
+    }
     if (any == 0 && !(sp.last_seed_day > t)) d.stop_day[sim] = t;
+  }
+}
+
+// Per node: births, per-day resets, early-stop rule.  Inside lp_run_days only the
+// first day launches it: k_day_end of each day begins the next one.
+__global__ void k_day_begin(Dev d, int mask) {
+  const int32_t t = d.day[0];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) d.sel_count[0] = 0;
+  if (n >= d.n_nodes) return;
+  day_begin_node(d, n, t, mask);
+  const int sim = d.node_sim[n];
+  if (n == d.sims[sim].node_begin) {
+    day_stop_check(d, sim, t, mask, -1);
+    d.sim_cases[sim] = 0;
   }
 }
 
@@ -1016,9 +1040,12 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const int32_t sia_b = SIA ? d.day_sia_begin[t] : 0;
   const int32_t sia_n = SIA ? d.day_sia_count[t] : 0;
 
-  // state words: original, after VD, after RI / SIA (the last only on campaign days)
-  constexpr int GW2 = (F & (FULL_RI | FULL_SIA)) ? G : 1;
-  __shared__ uint32_t s_w0[G][WG], s_w1[G][WG], s_w2[GW2][WG];
+  // state words for the inline E / I stage: original, after VD, after RI / SIA (the
+  // last only on campaign days); without it (EI false) nothing is shared between
+  // threads until the partial, so the groups need no barrier
+  constexpr int GW = EI ? G : 1;
+  constexpr int GW2 = EI && (F & (FULL_RI | FULL_SIA)) ? G : 1;
+  __shared__ uint32_t s_w0[GW][WG], s_w1[GW][WG], s_w2[GW2][WG];
   // per quad of the span: its agents that are E or I after today's pass (bit a),
   // appended to the next day's list once at the end
   __shared__ uint8_t s_sv[MAX_TPW][WG];
@@ -1265,8 +1292,10 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (u < ng) {
         if (!in_ei && w2 != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w2;
       }
-      s_w0[u][tid] = w0; s_w1[u][tid] = w1;
-      if constexpr (GW2 == G) s_w2[u][tid] = w2;
+      if constexpr (EI) {
+        s_w0[u][tid] = w0; s_w1[u][tid] = w1;
+        if constexpr (GW2 == G) s_w2[u][tid] = w2;
+      }
       if (VD && u < ng) {
         // the chunk's earliest death date: exact when its dates were read, else lowered by newborns
         const int32_t wm = wave_min_i32(dmin);
@@ -1279,7 +1308,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         if (lane == 0 && cs != 0) d.chunk_sum[(((int64_t)tile0 + u) * LP_TILE >> 8) + wave] += cs;
       }
     });
-    __syncthreads();
+    if constexpr (EI) __syncthreads();
 
     // ---- disease progression and E / I tally of the compacted quads ----
     for (int base = 0; base < n_ei; base += 64) {
@@ -1295,9 +1324,9 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (su < 0) continue;
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
-      const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[su][qt];
-      const uint32_t surv =
-          ei_quad(d, i0, live_hi, s_w1[su][qt], wb, s_w0[su][qt], do_ds, do_tally, ec, t, lc, lib, c_r);
+      const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[GW == G ? su : 0][qt];
+      const uint32_t surv = ei_quad(d, i0, live_hi, s_w1[GW == G ? su : 0][qt], wb, s_w0[GW == G ? su : 0][qt],
+                                    do_ds, do_tally, ec, t, lc, lib, c_r);
       if (surv) s_sv[g0 + su][qt] |= (uint8_t)surv;
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
@@ -1310,7 +1339,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
           if ((ipv_new >> (4 * u + a)) & 1u) *rec_byte(d, i0 + a, RB_IPV) = 1;
       });
     }
-    __syncthreads();
+    if constexpr (EI) __syncthreads();
   }
 
   {
@@ -1760,8 +1789,10 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
 #pragma unroll
     for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
     // list 1: k_ei alone (only the carried S weights come from the span partials);
-    // list 2: k_ei and a FULL-day pass (both add up)
+    // list 2: k_ei and a FULL-day pass (both add up).  The accumulators are cleared
+    // for the next day's k_ei.
     const int64_t ea = list ? d.node_acc[(size_t)n * NPART + tid] : 0;
+    if (list) d.node_acc[(size_t)n * NPART + tid] = 0;
     tally[tid] = list == 1 ? (tid == P_SW ? acc : ea) : acc + ea;
   }
   __syncthreads();
@@ -1842,31 +1873,25 @@ __device__ bool last_block(uint32_t* ticket) {
   return am_last;
 }
 
-// One workgroup: exclusive scan of the probes of sampling round `round` over
-// nodes (probe_off), and reservation of their space in the selected list.
+// Exclusive scan of the probes of sampling round `round` over nodes (probe_off),
+// and reservation of their space in the selected list; run by the first
+// wavefront of one workgroup.
 __device__ void plan_scan(const Dev& d, int round) {
-  __shared__ int64_t sh[WG];
-  const int tid = threadIdx.x;
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
   const int N = d.n_nodes;
-  const int per = (N + WG - 1) / WG;
-  const int a = tid * per, b = (a + per < N) ? a + per : N;
+  const int per = (N + 63) / 64;
+  const int a = lane * per, b = (a + per < N) ? a + per : N;
   int64_t s = 0;
-  for (int n = a; n < b; ++n) s += __hip_atomic_load(d.m_round + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  sh[tid] = s;
-  __syncthreads();
-  for (int o = 1; o < WG; o <<= 1) {
-    const int64_t v = (tid >= o) ? sh[tid - o] : 0;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
-  }
-  int64_t run = sh[tid] - s;
+  for (int n = a; n < b; ++n) s += d.m_round[n];   // written by k_mix_foi (an earlier launch)
+  const int64_t incl = wave_incl_scan_i64(s, lane);
+  int64_t run = incl - s;
   for (int n = a; n < b; ++n) {
     d.probe_off[n] = run;
-    run += __hip_atomic_load(d.m_round + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    run += d.m_round[n];
   }
-  if (tid == WG - 1) {
-    int64_t M = sh[WG - 1];
+  if (lane == 63) {
+    int64_t M = incl;
     const int64_t base = __hip_atomic_load(d.sel_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (base + M > d.sel_cap) { atomicOr(d.err, ERR_OVERFLOW); M = 0; }
     d.probe_off[N] = M;
@@ -2012,16 +2037,20 @@ __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
   const int32_t t = d.day[0];
   const int i = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (i >= d.n_nodes) return;
-  const int sim = d.node_sim[i];
-  const lp_sim_params& sp = d.sims[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
-    if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
-  } else {
-    mix_foi_node(d, i, sim, sp, t, lane);
+  if (i < d.n_nodes) {
+    const int sim = d.node_sim[i];
+    const lp_sim_params& sp = d.sims[sim];
+    const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+    if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
+      if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
+    } else {
+      mix_foi_node(d, i, sim, sp, t, lane);
+    }
   }
 }
+
+// One workgroup: plan sampling round `round` (probe offsets over nodes, list space).
+__global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, round); }
 
 
 // ---------------------------------------------------------------------------
@@ -2030,9 +2059,6 @@ __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
 // hit are exposed.  Round 1 runs grid-wide (k_plan, k_sample, k_commit);
 // k_finish then takes each node on its own for its later rounds and handles the
 // take-all nodes.
-
-// Single workgroup: plan sampling round `round` (probe offsets over nodes, list space).
-__global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, round); }
 
 __device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t p) {
   int a = 0, b = N;   // largest n in [0, N) with off[n] <= p < off[n+1]
@@ -2400,45 +2426,49 @@ __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_ad
     d.r_S[row] = (int32_t)tl[P_S] - ne_tot;
     d.r_E[row] = e;
     d.r_I[row] = in;
+    if (e + in > 0) d.sim_cases[sim] = 1;   // for the next day's early-stop rule
     d.r_R[row] += (int32_t)tl[P_R];
     d.r_potp[row] = d.potp_run[n];
     d.r_p[row] = d.p_run[n];
   }
 }
 
-// Per node: transmission incidence and the log rows; with `advance`, the last
-// workgroup moves the day counter to the next tick.
-// One workgroup of WG threads: ei_piece_off = exclusive prefix over nodes of the
-// pieces of list b (the last entry is the total).
+// ei_piece_off = exclusive prefix over nodes of the pieces of list b (the last
+// entry is the total); run by the first wavefront of one workgroup.
 __device__ void ei_plan(const Dev& d, int b) {
-  __shared__ int32_t sh[WG];
-  const int tid = threadIdx.x, N = d.n_nodes;
-  const int per = (N + WG - 1) / WG;
-  const int n0 = tid * per, n1 = n0 + per < N ? n0 + per : N;
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x, N = d.n_nodes;
+  const int per = (N + 63) / 64;
+  const int n0 = lane * per, n1 = n0 + per < N ? n0 + per : N;
   const int32_t* cnt = d.ei_cnt + (size_t)b * N;
   int32_t acc = 0;
   for (int n = n0; n < n1; ++n) acc += (cnt[n] + EI_PIECE - 1) / EI_PIECE;
-  sh[tid] = acc;
-  __syncthreads();
-  for (int o = 1; o < WG; o <<= 1) {
-    const int32_t v = tid >= o ? sh[tid - o] : 0;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
-  }
-  int32_t off = sh[tid] - acc;
+  const int64_t incl = wave_incl_scan_i64(acc, lane);
+  int32_t off = (int32_t)incl - acc;
   for (int n = n0; n < n1; ++n) { d.ei_piece_off[n] = off; off += (cnt[n] + EI_PIECE - 1) / EI_PIECE; }
-  if (tid == WG - 1) d.ei_piece_off[N] = sh[WG - 1];
+  if (lane == 63) d.ei_piece_off[N] = (int32_t)incl;
 }
 
-// Per node: transmission incidence and the log rows; with `advance`, the last
-// workgroup plans the next day's E / I list pieces and moves the day counter.
+// Per node: transmission incidence and the log rows.  With `advance`: when the
+// next tick is inside the current lp_run_days call (before day[1]), each node's
+// thread also begins that day (day_begin_node) and the last workgroup runs the
+// early-stop rule for it; the last workgroup plans the next day's E / I list
+// pieces and moves the day counter.
 __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, int advance) {
   const int32_t t = d.day[0];
+  const bool begin_next = advance && t + 1 < d.day[1];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < d.n_nodes) day_end_node(d, n, t, mask, tx_adjust);
+  if (n < d.n_nodes) {
+    day_end_node(d, n, t, mask, tx_adjust);
+    if (begin_next) day_begin_node(d, n, t + 1, mask);
+  }
   if (advance && last_block(d.ticket + 2)) {
     ei_plan(d, (t + 1) & 1);
+    for (int sim = threadIdx.x; sim < d.n_sims; sim += WG) {
+      if (begin_next) day_stop_check(d, sim, t + 1, mask, d.sim_cases[sim]);
+      d.sim_cases[sim] = 0;
+    }
+    if (begin_next && threadIdx.x == 0) d.sel_count[0] = 0;
     if (threadIdx.x == 0) d.day[0] = t + 1;
   }
 }
