@@ -8,7 +8,7 @@ runs ``--warmup`` days untimed, then times exactly ``--steps`` days bracketed by
 torch.cuda.synchronize().  value = sum over timed days of live agents / seconds.
 
 Also reported on the same JSON line:
-  roofline      the agent pass (k_plain on plain days, k_full on VD / RI / SIA days):
+  roofline      the agent pass (k_ei every day, plus k_full / k_ri_list on VD / RI / SIA days):
                 algorithmic bytes per day (agent_pass_bytes below) over its
                 HIP-event-timed duration, against MI355X's 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
@@ -217,7 +217,7 @@ def main():
         tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
         roofline = {
             "bound": "hbm",
-            "kernel": "agent pass (k_plain / k_full)",
+            "kernel": "agent pass (k_ei + k_full / k_ri_list)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
