@@ -211,17 +211,16 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
     case 7: hipLaunchKernelGGL((lp::k_full<7, true>), g, b, 0, s, c->d, mask); break;
     case lp::KIND_FULL_LIST + 0: case lp::KIND_FULL_LIST + 1: case lp::KIND_FULL_LIST + 2:
     case lp::KIND_FULL_LIST + 3: case lp::KIND_FULL_LIST + 4: case lp::KIND_FULL_LIST + 5:
-    case lp::KIND_FULL_LIST + 6:
+    case lp::KIND_FULL_LIST + 6: {
+      const int f = kind - lp::KIND_FULL_LIST + 1;
+      const bool ri_sparse = sparse_ri(kind);
+      const int fd = ri_sparse ? (f & lp::FULL_VD) : f;   // F of the dense pass (0: none)
       hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask);
-      if (sparse_ri(kind)) {
-        // RI without SIA: the VD step alone (RI masked out), then RI from the candidate lists
-        if ((kind - lp::KIND_FULL_LIST + 1) & lp::FULL_VD)
-          hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, mask & ~LP_STAGE_RI);
-        hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);
-        break;
-      }
-      switch (kind - lp::KIND_FULL_LIST + 1) {
-        case 1: hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, mask); break;
+      // with RI from the lists the VD step runs alone (RI masked out of its day flags)
+      const int fm = ri_sparse ? mask & ~LP_STAGE_RI : mask;
+      switch (fd) {
+        case 0: break;
+        case 1: hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, fm); break;
         case 2: hipLaunchKernelGGL((lp::k_full<2, false>), g, b, 0, s, c->d, mask); break;
         case 3: hipLaunchKernelGGL((lp::k_full<3, false>), g, b, 0, s, c->d, mask); break;
         case 4: hipLaunchKernelGGL((lp::k_full<4, false>), g, b, 0, s, c->d, mask); break;
@@ -229,7 +228,9 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
         case 6: hipLaunchKernelGGL((lp::k_full<6, false>), g, b, 0, s, c->d, mask); break;
         default: hipLaunchKernelGGL((lp::k_full<7, false>), g, b, 0, s, c->d, mask); break;
       }
+      if (ri_sparse) hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);
       break;
+    }
     default: hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), g, b, 0, s, c->d, mask); break;
   }
 }
@@ -282,6 +283,7 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   if (ev) { HIPCHK(c, hipEventRecord(c->ev[c->ev_used + 1], s)); c->ev_used += 2; }
   if (recount) {
     hipLaunchKernelGGL(lp::k_pack, dim3(rec_grid(c)), dim3(lp::WG), 0, s, d);
+    hipLaunchKernelGGL(lp::k_node_info, dim3(grid_of(N, 256)), dim3(256), 0, s, d);
     // the pass left ri_timer actual as of today if it ran RI; otherwise it is actual as
     // of the latest applied RI step, as before the pass
     const bool ri = host_stage_flags(c->day_flags_h[t], mask) & LP_DAY_RI;
@@ -355,7 +357,7 @@ int lp_bind(lp_ctx* c, const char* name, void* ptr, int32_t dtype, int64_t n) {
 
 int lp_set_stream(lp_ctx* c, void* s) {
   if (!c) return LP_ERR_ARG;
-  for (int k = 0; k < 8; ++k)
+  for (int k = 0; k < lp::N_KINDS; ++k)
     if (c->graph_ready[k]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
   c->stream = reinterpret_cast<hipStream_t>(s);
   return LP_OK;
@@ -460,6 +462,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.beta_f, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.sim_cases, (size_t)S))) return rc;
+  if ((rc = alloc(c, &d.ninfo, (size_t)N))) return rc;
   if ((rc = alloc(c, &d.ri_list, 2 * (size_t)g.capacity))) return rc;
   if ((rc = alloc(c, &d.ri_lcnt, 2 * (size_t)N))) return rc;
   HIPCHK(c, hipMemsetAsync(d.beta_f, 0, (size_t)N * LP_MAX_STRAINS * sizeof(float), c->stream));

@@ -57,6 +57,16 @@ enum : int {
 
 enum : int { ERR_CAPACITY = 1, ERR_OVERFLOW = 2, ERR_SAMPLING = 4, ERR_KIND = 8, ERR_LIST = 16 };
 
+// Per-node constants the list and sampling kernels need, gathered in one 64-byte
+// record (rebuilt by k_node_info after every RECOUNT pass).
+struct NodeInfo {
+  uint64_t seed, thr_paralysis;
+  int64_t slot_base, seg_begin;
+  float r0k[LP_MAX_STRAINS];
+  int32_t sim, lnode;          // sim index; node index inside the sim
+  int32_t span_begin, n_spans;
+};
+
 struct Dev {
   // agent columns
   int8_t* state; int8_t* strain; int16_t* node_id;
@@ -92,6 +102,7 @@ struct Dev {
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
   float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
   int32_t* sim_cases;      // [sims] 1 when a node of the sim logged E or I today
+  NodeInfo* ninfo;         // [nodes]
   const int32_t* ri_cnt;   // [nt + 1] ri_cnt[t + 1] = RI days among ticks 0..t
   int32_t* ri_ref;         // [2] ri_timer reference tick; latest tick whose RI step is applied
   uint32_t* ri_list;       // [2][capacity] per-node lists of RI candidates (slots)
@@ -495,6 +506,7 @@ __global__ void k_set_ri_ref(Dev d, int32_t through) {
 
 // Pieces of EI_PIECE list entries: one wavefront of k_ei takes one piece at a time.
 constexpr int EI_PIECE = 256;
+constexpr int EI_LDS_NODES = 4095;   // k_ei stages the piece offsets in LDS up to this many nodes
 
 __device__ __forceinline__ uint32_t* ei_list_of(const Dev& d, int b, int node) {
   return d.ei_list + (size_t)b * (size_t)d.capacity + d.seg_begin[node];
@@ -560,6 +572,26 @@ __device__ __forceinline__ void wave_append_slot(const Dev& d, int b, int node, 
   if (lane == first) base = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, __popcll(mk));
   base = __shfl(base, first, 64);
   if (flag) ei_list_of(d, b, node)[base + rank] = (uint32_t)slot;
+}
+
+// NodeInfo of every node (one thread per node).
+__global__ void k_node_info(Dev d) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.n_nodes) return;
+  const int sim = d.node_sim[n];
+  const lp_sim_params& sp = d.sims[sim];
+  NodeInfo ni;
+  ni.seed = sp.seed;
+  ni.thr_paralysis = sp.thr_paralysis;
+  ni.slot_base = sp.slot_base;
+  ni.seg_begin = d.seg_begin[n];
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) ni.r0k[k] = sp.strain_r0[k];
+  ni.sim = sim;
+  ni.lnode = n - sp.node_begin;
+  ni.span_begin = d.node_span_begin[n];
+  ni.n_spans = d.node_span_begin[n + 1] - ni.span_begin;
+  d.ninfo[n] = ni;
 }
 
 // Agent records <- bound columns, slots [0, capacity) (one quad per thread).
@@ -745,6 +777,8 @@ __device__ __forceinline__ int find_piece_node(const int32_t* off, int N, int32_
 }
 
 __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
+  // the piece offsets in LDS (when they fit) for the node lookup of each piece
+  __shared__ int32_t s_off[EI_LDS_NODES + 1];
   const int32_t t = d.day[0];
   const int cur = t & 1, nxt = cur ^ 1;
   const int N = d.n_nodes, K = d.K;
@@ -754,15 +788,36 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
   const bool do_ds = flags & LP_DAY_DS;
   const bool do_tally = (flags & LP_DAY_TX) != 0;
   const bool vd = flags & LP_DAY_VD_STEP;   // agents due to die today leave before DiseaseState
-  const int32_t total = d.ei_piece_off[N];
+  const bool lds = N <= EI_LDS_NODES;
+  int32_t total;
+  if (lds) {
+    for (int x = threadIdx.x; x <= N; x += WG) s_off[x] = d.ei_piece_off[x];
+    __syncthreads();
+    total = s_off[N];
+  } else {
+    total = d.ei_piece_off[N];
+  }
   for (int32_t p = gw; p < total; p += nw) {
-    const int n = find_piece_node(d.ei_piece_off, N, p, lane);
-    const int sim = d.node_sim[n];
-    if (sim_stopped(d, sim, t)) continue;
-    const EIConst ec = ei_const(d.sims[sim]);
+    int n;
+    if (lds) {
+      int a = 0, b = N;   // largest n with s_off[n] <= p
+      while (b - a > 1) { const int m = (a + b) >> 1; if (s_off[m] <= p) a = m; else b = m; }
+      n = a;
+    } else {
+      n = find_piece_node(d.ei_piece_off, N, p, lane);
+    }
+    n = __builtin_amdgcn_readfirstlane(n);   // the same in every lane: scalar loads of the node's data
+    const NodeInfo ni = d.ninfo[n];
     const int32_t cnt = d.ei_cnt[(size_t)cur * N + n];
-    const int32_t j0 = (p - d.ei_piece_off[n]) * EI_PIECE;
-    const uint32_t* L = ei_list_of(d, cur, n);
+    const int32_t j0 = (p - (lds ? s_off[n] : d.ei_piece_off[n])) * EI_PIECE;
+    const bool stopped = sim_stopped(d, ni.sim, t);
+    EIConst ec;
+    ec.seed = ni.seed;
+    ec.thr_paralysis = ni.thr_paralysis;
+    ec.slot_base = ni.slot_base;
+#pragma unroll
+    for (int k = 0; k < LP_MAX_STRAINS; ++k) ec.r0k[k] = ni.r0k[k];
+    const uint32_t* L = d.ei_list + (size_t)cur * (size_t)d.capacity + ni.seg_begin;
     // four entries per lane; lanes past the end repeat entry j0 (unconditional loads)
     uint32_t sl[4];
     bool v[4];
@@ -783,6 +838,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) dd[q] = d.dod[sl[q]];
     }
+    if (stopped) continue;
     // tallies: E and I counts per strain in 16-bit fields, I->R / new potentially
     // paralysed / new paralysed in 16-bit fields, infectious weight per strain
     int64_t e_pk = 0, i_pk = 0, misc = 0, ib0 = 0, ib1 = 0, ib2 = 0, ib3 = 0;
@@ -831,7 +887,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
         int base = 0;
         if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)nxt * N + n, tot);
         base = __shfl(base, 0, 64);
-        uint32_t* Ln = ei_list_of(d, nxt, n);
+        uint32_t* Ln = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           if ((surv >> q) & 1u) Ln[base + rank[q]] = sl[q];
@@ -2161,9 +2217,9 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
 
 // S -> E of a selected agent with its strain drawn by cumulative strain
 // probability (model.py:1125-1147); returns the strain (the caller counts it).
-__device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, const lp_sim_params& sp, int64_t slot,
-                                                    int32_t t) {
-  const uint4 o = draw(sp.seed, (uint64_t)(slot - sp.slot_base), t, ST_STRAIN);
+__device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, uint64_t seed, int64_t slot_base,
+                                                    int64_t slot, int32_t t) {
+  const uint4 o = draw(seed, (uint64_t)(slot - slot_base), t, ST_STRAIN);
   int k = 0;
   for (int s = 0; s < d.K; ++s) {
     if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
@@ -2207,15 +2263,15 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   uint32_t lnode = 0;
   for (int64_t p = p0; p < p1; ++p) {
     if (node < 0 || p >= node_hi) {
-      node = (int)find_node(off, N, p);
+      node = __builtin_amdgcn_readfirstlane((int)find_node(off, N, p));
       node_lo = off[node];
       node_hi = off[node + 1];
-      const lp_sim_params& sp = d.sims[d.node_sim[node]];
-      seed = sp.seed;
-      lnode = (uint32_t)(node - sp.node_begin);
+      const NodeInfo& ni = d.ninfo[node];
+      seed = ni.seed;
+      lnode = (uint32_t)ni.lnode;
       W = d.W_round[node];
-      s0 = d.node_span_begin[node];
-      ns = (int)(d.node_span_begin[node + 1] - s0);
+      s0 = ni.span_begin;
+      ns = ni.n_spans;
       spx = (ns <= 64 && lane < ns) ? d.span_prefix[s0 + lane] : INT64_MAX;
     }
     const int64_t j = p - node_lo;
@@ -2264,16 +2320,16 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           atomicOr(d.err, ERR_SAMPLING);
         } else {
           // everything the exposure reads, loaded ahead of the claim
-          const int64_t lo = d.seg_begin[nd], hi = d.live_end[nd];
+          const NodeInfo& ni = d.ninfo[nd];
+          const int64_t lo = ni.seg_begin, hi = d.live_end[nd];
           const int64_t w = weight_q24(d.risk[slot]);
-          const int64_t sp_i = span_of(d, nd, slot);
-          const lp_sim_params& sp = d.sims[d.node_sim[nd]];
+          const int64_t sp_i = ni.span_begin + (slot / LP_TILE - lo / LP_TILE) / d.tpw;
           if (slot >= hi || slot < lo) {
             atomicOr(d.err, ERR_SAMPLING);
           } else if (claim(d.state, slot)) {         // the first probe of an agent claims it
             node = nd;
             unweight_in(d, sp_i, slot, w);
-            k = expose_agent_nocount(d, node, sp, slot, t);
+            k = expose_agent_nocount(d, node, ni.seed, ni.slot_base, slot, t);
           }
         }
       }
@@ -2324,7 +2380,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       bool ex = false;
       if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
-        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, i, t)], 1);
+        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp.seed, sp.slot_base, i, t)], 1);
         ex = true;
       }
       wave_append_slot(d, nxt, n, i, ex, lane);
@@ -2382,7 +2438,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
                                                         __HIP_MEMORY_SCOPE_AGENT)
                                     : SEL_INVALID;
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, sp, slot, t)], 1);
+        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, sp.seed, sp.slot_base, slot, t)], 1);
         wave_append_slot(d, nxt, n, slot, e != SEL_INVALID, lane);
       }
       __threadfence();
