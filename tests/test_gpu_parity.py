@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from cases import CASES, make
+from laser_polio_amd import engine as engmod
 from oracle.oracle import OracleSim
 
 RESULT_NAMES = [
@@ -94,6 +95,26 @@ def test_host_edit_between_days_matches_oracle(use_graph):
     _host_edit(p.disease_state, p.infection_timer, p.acq_risk_multiplier, slots)
     eng = sim._ensure_engine()          # pushes the edited columns (and invalidates)
     eng.run_days(k, sim.nt, use_graph=use_graph)
+    eng.sync()
+    sim._mark_device_newer()
+    _compare(sim, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,k", [("campaigns", 7), ("campaigns", 10), ("campaigns", 14), ("sia", 14),
+                                    ("small3", 9), ("ri_only", 15)])
+def test_stage_calls_between_fused_runs_match_oracle(case, k):
+    """A day run as stage calls between two fused runs (the path of a seed_schedule day)
+    hands the device-held state (E/I lists, RI candidates, reference-tick ri_timer,
+    agent records) from the fused path to the per-stage path and back."""
+    print(f"[parity-mixed] case={case} k={k}", flush=True)
+    sim = make(case)
+    ref = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop).run()
+    eng = sim._ensure_engine()
+    eng.run_days(0, k, use_graph=True)
+    eng.step(k, engmod.STAGE_VITAL | engmod.STAGE_DISEASE)
+    eng.step(k, engmod.STAGE_RI | engmod.STAGE_SIA | engmod.STAGE_TRANSMISSION | engmod.STAGE_LOG)
+    eng.run_days(k + 1, sim.nt, use_graph=True)
     eng.sync()
     sim._mark_device_newer()
     _compare(sim, ref)
