@@ -2215,6 +2215,21 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
   unweight_in(d, span_of(d, node, slot), slot, weight_q24(d.risk[slot]));
 }
 
+// S -> E of a selected agent with its strain drawn against the node's cumulative
+// strain thresholds thr[0..K) (already loaded); returns the strain.
+__device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* thr, uint64_t seed, int64_t slot_base,
+                                                int64_t slot, int32_t t) {
+  const uint4 o = draw(seed, (uint64_t)(slot - slot_base), t, ST_STRAIN);
+  int k = 0;
+#pragma unroll
+  for (int s = LP_MAX_STRAINS - 1; s >= 0; --s)
+    if (s < d.K && (uint64_t)o.x < thr[s]) k = s;   // the first strain whose threshold exceeds the draw
+  d.state[slot] = 1;
+  *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
+  *rec_byte(d, slot, RB_STATE) = 1;
+  return k;
+}
+
 // S -> E of a selected agent with its strain drawn by cumulative strain
 // probability (model.py:1125-1147); returns the strain (the caller counts it).
 __device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, uint64_t seed, int64_t slot_base,
@@ -2324,12 +2339,15 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           const int64_t lo = ni.seg_begin, hi = d.live_end[nd];
           const int64_t w = weight_q24(d.risk[slot]);
           const int64_t sp_i = ni.span_begin + (slot / LP_TILE - lo / LP_TILE) / d.tpw;
+          uint64_t thr[LP_MAX_STRAINS];
+#pragma unroll
+          for (int q = 0; q < LP_MAX_STRAINS; ++q) thr[q] = d.thr_strain[(size_t)nd * LP_MAX_STRAINS + q];
           if (slot >= hi || slot < lo) {
             atomicOr(d.err, ERR_SAMPLING);
           } else if (claim(d.state, slot)) {         // the first probe of an agent claims it
             node = nd;
             unweight_in(d, sp_i, slot, w);
-            k = expose_agent_nocount(d, node, ni.seed, ni.slot_base, slot, t);
+            k = expose_agent_thr(d, thr, ni.seed, ni.slot_base, slot, t);
           }
         }
       }
@@ -2367,30 +2385,31 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   __shared__ int64_t sh_W;
   __shared__ int32_t s_ne[LP_MAX_STRAINS];   // the node's new exposures by strain (one global add each)
   __shared__ int32_t s_claimed;
-  if (d.size[n] == 0) return;
-  const lp_sim_params& sp = d.sims[d.node_sim[n]];
+  // most nodes have nothing left after round 1: one round of loads decides
+  const int32_t size = d.size[n], take_all = d.take_all[n], uniq0 = d.n_uniq[n];
+  if (size == 0 || (!take_all && size - uniq0 <= 0)) return;
+  const NodeInfo& ni = d.ninfo[n];
   if (tid < LP_MAX_STRAINS) s_ne[tid] = 0;
   if (tid == 0) s_claimed = 0;
   __syncthreads();
   const int nxt = (t + 1) & 1;
-  if (d.take_all[n]) {
-    const int64_t lo = d.seg_begin[n], hi = d.live_end[n];
+  if (take_all) {
+    const int64_t lo = ni.seg_begin, hi = d.live_end[n];
     for (int64_t i0 = lo; i0 < hi; i0 += WG) {         // uniform trip count: whole waves append
       const int64_t i = i0 + tid;
       bool ex = false;
       if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
-        atomicAdd(&s_ne[expose_agent_nocount(d, n, sp.seed, sp.slot_base, i, t)], 1);
+        atomicAdd(&s_ne[expose_agent_nocount(d, n, ni.seed, ni.slot_base, i, t)], 1);
         ex = true;
       }
       wave_append_slot(d, nxt, n, i, ex, lane);
     }
   } else {
-    const int32_t uniq0 = d.n_uniq[n];
-    const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
+    const int s0 = ni.span_begin, s1 = ni.span_begin + ni.n_spans;
     for (int round = 2;; ++round) {
       if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); break; }
-      const int32_t need = d.size[n] - (uniq0 + s_claimed);
+      const int32_t need = size - (uniq0 + s_claimed);
       if (need <= 0) break;
       const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
       if (W <= 0) break;
@@ -2406,8 +2425,8 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       const int64_t base = sh_base;
       if (base < 0) break;
       for (int j = wave; j < need; j += WG / 64) {
-        const uint4 o = philox((uint32_t)j, (uint32_t)(n - sp.node_begin), (uint32_t)t,
-                               ST_PROBE | ((uint32_t)round << 8), sp.seed);
+        const uint4 o = philox((uint32_t)j, (uint32_t)ni.lnode, (uint32_t)t,
+                               ST_PROBE | ((uint32_t)round << 8), ni.seed);
         const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
         const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
         const int64_t slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
@@ -2438,7 +2457,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
                                                         __HIP_MEMORY_SCOPE_AGENT)
                                     : SEL_INVALID;
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, sp.seed, sp.slot_base, slot, t)], 1);
+        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, ni.seed, ni.slot_base, slot, t)], 1);
         wave_append_slot(d, nxt, n, slot, e != SEL_INVALID, lane);
       }
       __threadfence();
