@@ -83,7 +83,7 @@ typedef struct {
   int32_t n_nodes;         /* nodes over all sims (node ids are global) */
   int32_t n_sims;          /* independent simulations in the batch */
   int32_t n_strains;       /* 1..LP_MAX_STRAINS */
-  int32_t nt;              /* ticks = dur + 1 */
+  int32_t nt;              /* ticks = dur + 1, at most 2^23 */
   int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
   int32_t tiles_per_block; /* tiles of 1024 slots per span: 1, 2, 4, 8 or 16 (0 = default 16) */
   int32_t mix_mode;        /* must be 0: per-node fixed-order tree reduction of the network mix */

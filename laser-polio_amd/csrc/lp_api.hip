@@ -106,6 +106,7 @@ struct lp_ctx {
   bool recount_pending = true;                     // next agent pass recounts carried statistics
   int32_t list_tick = -1;                          // the tick whose E / I list is complete (-1: none)
   bool rec_dirty = false;                          // agent records newer than the bound columns
+  bool rec_actual = true;                          // records hold the actual timers of E / I agents
   std::vector<int32_t> day_flags_h;
   std::vector<int32_t> ri_cnt_h;
   std::string err;
@@ -242,8 +243,23 @@ int rec_grid(const lp_ctx* c) {
   return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
 }
 
+// Day kinds that process the E / I list; afterwards the records of listed agents
+// keep the timers of when they entered the list until enqueue_actualize.
+bool list_kind(int kind) { return kind == lp::KIND_PLAIN_LIST || kind >= lp::KIND_FULL_LIST; }
+
+// Records <- actual timers of the agents in the complete E / I list (that of
+// list_tick), when days since the last call processed the list.
+void enqueue_actualize(lp_ctx* c) {
+  if (c->rec_actual) return;
+  if (c->list_tick >= 0)
+    hipLaunchKernelGGL(lp::k_ei_actual, dim3(c->cfg.n_nodes), dim3(lp::WG), 0, c->stream, c->d, c->list_tick & 1,
+                       c->list_tick);
+  c->rec_actual = true;
+}
+
 // Bound columns <- device-held state: agent records and actual ri_timer values.
 void enqueue_sync_columns(lp_ctx* c) {
+  enqueue_actualize(c);
   hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, c->stream, c->d);
   hipLaunchKernelGGL(lp::k_ri_sync, dim3(c->n_spans), dim3(lp::WG), 0, c->stream, c->d);
   hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, c->stream, c->d, -2);
@@ -258,6 +274,9 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   const Dev& d = c->d;
   const int N = c->cfg.n_nodes;
   hipStream_t s = c->stream;
+  // passes that read the records need their actual timers (never inside a captured
+  // graph: run_days calls enqueue_actualize before capturing)
+  if (!list_kind(kind)) enqueue_actualize(c);
   // a RECOUNT pass reads the bound columns: bring them up to date first; it rebuilds the
   // RI candidate lists
   if (recount && c->rec_dirty) enqueue_sync_columns(c);
@@ -325,7 +344,7 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
   if (!cfg || !out) return LP_ERR_ARG;
   *out = nullptr;
   if (cfg->capacity <= 0 || cfg->capacity % LP_TILE != 0 || cfg->n_nodes <= 0 || cfg->n_sims <= 0 ||
-      cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->n_sia_events < 0 ||
+      cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->nt > (1 << 23) || cfg->n_sia_events < 0 ||
       cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode != 0)
     return LP_ERR_ARG;
   lp_ctx* c = new lp_ctx();
@@ -531,10 +550,13 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
       continue;
     }
     int kind = day_kind(c, t, LP_STAGE_ALL, 0);
-    if (c->list_tick == t) {
+    // the list kinds keep E / I timers as event days, which needs DiseaseState every day
+    if (c->list_tick == t && (host_stage_flags(c->day_flags_h[t], LP_STAGE_ALL) & LP_DAY_DS)) {
       if (kind == lp::KIND_PLAIN) kind = lp::KIND_PLAIN_LIST;
       else if (kind >= lp::KIND_FULL && kind < lp::KIND_RECOUNT) kind = lp::KIND_FULL_LIST + kind - lp::KIND_FULL;
     }
+    if (list_kind(kind)) c->rec_actual = false;
+    else enqueue_actualize(c);
     c->list_tick = t + 1;   // every day kind run here completes the next tick's list
     if (use_graph && !c->timing && !first) {
       if (!c->graph_ready[kind]) {
@@ -567,6 +589,7 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);
   const int recount = (log_only || c->recount_pending) ? 1 : 0;
   c->recount_pending = false;
+  enqueue_actualize(c);
   c->list_tick = -1;   // stage calls do not keep the E / I list
   return enqueue_day(c, t, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false,
                      true);

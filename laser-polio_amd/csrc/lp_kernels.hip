@@ -96,7 +96,7 @@ struct Dev {
   int64_t* node_tally; int32_t* potp_run; int32_t* p_run;
   int64_t* node_carry;   // [nodes][3] carried counts: S, S with positive weight, R
   int32_t* chunk_min_dod;  // [chunks] earliest date_of_death among the chunk's live agents
-  uint32_t* ei_list;       // [2][capacity] per-node lists of E / I agents (slots)
+  uint4* ei_list;          // [2][capacity] per-node lists of E / I agents (16-byte entries)
   int32_t* ei_cnt;         // [2][nodes] list lengths
   int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
@@ -412,51 +412,6 @@ __device__ __forceinline__ EIConst ei_const(const lp_sim_params& sp) {
   return c;
 }
 
-// One compacted quad holding E or I agents: loads the quad's four records (one
-// 64-byte segment, four loads in flight together), runs the disease progression
-// of its E and I agents (model.py:419-452) and tallies them as E / I.  `wei`:
-// state bytes that identify today's E and I agents; `w`: the quad's state word
-// to update and write back if it differs from `w_orig`.  Returns the agents
-// (bit a) that are still E or I afterwards.
-__device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
-                                        uint32_t w_orig, bool do_ds, bool do_tally, const EIConst& ec,
-                                        int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r) {
-  const uint4* rq = d.rec + i0;
-  uint4 r[4] = {rq[0], rq[1], rq[2], rq[3]};
-  uint32_t surv = 0;
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int64_t i = i0 + a;
-    int8_t sa = byte_of(wei, a);
-    if ((sa != 1 && sa != 2) || i >= live_hi) continue;
-    const uint4 r0 = r[a];
-    const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
-    bool to_pp = false, to_par = false;
-    if (do_ds && progress_rec(sa, r[a], ec.seed, (uint64_t)(i - ec.slot_base), t, ec.thr_paralysis, to_pp, to_par))
-      c_r += 1;
-    if (to_pp) atomicAdd(&lc[L_NEWPP], 1);
-    if (to_par) atomicAdd(&lc[L_NEWP], 1);
-    if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
-      if (sa == 1) {
-        atomicAdd(&lc[L_E0 + k], 1);
-      } else if (sa == 2) {
-        atomicAdd(&lc[L_I0 + k], 1);
-        const float r0k = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
-        atomicAdd(&lib[k], (unsigned long long)weight_q24(__uint_as_float(r0.z) * r0k));
-      }
-    }
-    uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + i);
-    if (r[a].x != r0.x) rw[0] = r[a].x;
-    if (r[a].y != r0.y) rw[1] = r[a].y;
-    if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
-    if (sa != byte_of(wei, a)) *rec_byte(d, i, RB_STATE) = sa;
-    w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
-    if (sa == 1 || sa == 2) surv |= 1u << a;
-  }
-  if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
-  return surv;
-}
-
 // ri_timer <- actual values: for live agents that are not chronically missed,
 // minus RI's decrements from the reference tick through the latest applied RI
 // step (one workgroup per span); k_set_ri_ref then moves the reference there.
@@ -495,7 +450,7 @@ __global__ void k_set_ri_ref(Dev d, int32_t through) {
 }
 
 // ---------------------------------------------------------------------------
-// E/I lists.  ei_list[b] holds, per node n, the slots of the node's E and I
+// E/I lists.  ei_list[b] holds, per node n, entries of the node's E and I
 // agents at ei_list + b * capacity + seg_begin[n] (room for the whole segment),
 // ei_cnt[b * n_nodes + n] of them, in no particular order.  Day t reads list
 // t & 1 (the agents that are E or I when the day starts) and every pass of the
@@ -504,12 +459,89 @@ __global__ void k_set_ri_ref(Dev d, int32_t through) {
 // agents it exposes.  Plain days then process the list alone (k_ei) instead of
 // scanning every agent; the order of a list does not affect any result.
 
+// E/I list entry (16 bytes).  While the lists carry the disease timers, an E or
+// I agent's countdowns are kept as absolute days (the day its timer reads 0 at the
+// start of DiseaseState) and its record keeps the timers of when it entered the
+// list (an I agent's exposure timer is actual: 0 after E -> I, as drawn for one
+// infected from the start); ei_actual_timers writes the actual timers back.
+//   x: slot
+//   y: bits 0-1 state (1 E, 2 I); 2 strain 0 (paralysis timer tracked); 3 paralysis
+//      decision pending (potentially_paralyzed -1); 4-5 strain; 8-31 day the
+//      paralysis timer reads 0 (strain-0 I)
+//   z: day the exposure timer (E) or the infection timer (I) reads 0
+//   w: infectious weight rint(infectivity * strain r0 * 2^24) (I); W_REC when it
+//      does not fit 32 bits (then recomputed from the record)
+constexpr uint32_t W_REC = 0xFFFFFFFFu;
+constexpr uint32_t DAY_NONE = 0xFFFFFFu;
+
+__device__ __forceinline__ int ent_state(const uint4& e) { return (int)(e.y & 3u); }
+__device__ __forceinline__ int ent_strain(const uint4& e) { return (int)((e.y >> 4) & 3u); }
+__device__ __forceinline__ bool ent_s0(const uint4& e) { return (e.y >> 2) & 1u; }
+__device__ __forceinline__ bool ent_pend(const uint4& e) { return (e.y >> 3) & 1u; }
+__device__ __forceinline__ int32_t ent_pday(const uint4& e) { return (int32_t)(e.y >> 8); }
+// The next day the entry's agent changes: E -> I, the paralysis decision, I -> R.
+__device__ __forceinline__ int32_t ent_event(const uint4& e) {
+  const int32_t d1 = (int32_t)e.z, p = ent_pday(e);
+  return ent_pend(e) && p < d1 ? p : d1;
+}
+__device__ __forceinline__ uint32_t ent_flags_i(int k, bool s0, bool pend, int32_t p_day) {
+  return 2u | (s0 ? 4u : 0u) | (pend ? 8u : 0u) | ((uint32_t)k << 4) | ((uint32_t)p_day << 8);
+}
+__device__ __forceinline__ uint32_t weight_entry(int64_t w) { return w >= (int64_t)W_REC ? W_REC : (uint32_t)w; }
+
+// Entry of an E agent whose exposure timer reads `et` at the start of day t_next.
+__device__ __forceinline__ uint4 entry_e(uint32_t slot, int k, uint32_t et, int32_t t_next) {
+  return make_uint4(slot, 1u | ((uint32_t)k << 4), (uint32_t)t_next + et, 0u);
+}
+
+// Entry of an I agent with timers `it`, `pt` and potentially_paralyzed `pp` at the start
+// of day t_next.
+__device__ __forceinline__ uint4 entry_i(uint32_t slot, int k, uint32_t it, uint32_t pt, int8_t pp, int32_t t_next,
+                                         int64_t weight) {
+  const bool s0 = k == 0, pend = s0 && pp == -1;
+  const int32_t p_day = s0 ? t_next + (int32_t)pt : (int32_t)DAY_NONE;
+  return make_uint4(slot, ent_flags_i(k, s0, pend, p_day), (uint32_t)t_next + it, weight_entry(weight));
+}
+
+// Entry of an E or I agent from its (actual) record at the start of day t_next.
+__device__ __forceinline__ uint4 entry_from_rec(uint32_t slot, int s, const uint4& r, int32_t t_next, const float* r0k) {
+  const int k = (int)((r.y >> 16) & (LP_MAX_STRAINS - 1));
+  if (s == 1) return entry_e(slot, k, r.x & 0xFFFFu, t_next);
+  const float rk = k == 0 ? r0k[0] : k == 1 ? r0k[1] : k == 2 ? r0k[2] : r0k[3];
+  return entry_i(slot, k, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t_next, weight_q24(__uint_as_float(r.z) * rk));
+}
+
+// A timer that reads 0 on day `end`, as of the start of day t_next (it stays at 0).
+__device__ __forceinline__ uint32_t timer_left(int32_t end, int32_t t_next) {
+  const int32_t v = end - t_next;
+  return v < 0 ? 0u : (uint32_t)v;
+}
+
+// The record's actual timers of an E or I agent at the start of day t_next.
+__device__ __forceinline__ void ei_actual_timers(const Dev& d, const uint4& e, int32_t t_next) {
+  uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + e.x);
+  uint16_t* hw = reinterpret_cast<uint16_t*>(rw);
+  if (ent_state(e) == 1) {
+    hw[0] = (uint16_t)timer_left((int32_t)e.z, t_next);          // exposure timer
+  } else {
+    hw[1] = (uint16_t)timer_left((int32_t)e.z, t_next);          // infection timer
+    if (ent_s0(e)) hw[2] = (uint16_t)timer_left(ent_pday(e), t_next);   // paralysis timer
+  }
+}
+
 // Pieces of EI_PIECE list entries: one wavefront of k_ei takes one piece at a time.
 constexpr int EI_PIECE = 256;
 constexpr int EI_LDS_NODES = 4095;   // k_ei stages the piece offsets in LDS up to this many nodes
 
-__device__ __forceinline__ uint32_t* ei_list_of(const Dev& d, int b, int node) {
+__device__ __forceinline__ uint4* ei_list_of(const Dev& d, int b, int node) {
   return d.ei_list + (size_t)b * (size_t)d.capacity + d.seg_begin[node];
+}
+
+// Appends one entry (one atomic, returning); for the rare paths (inline DiseaseState,
+// RECOUNT) that append from divergent code.
+__device__ __forceinline__ void append_entry(const Dev& d, int b, int node, const uint4& e) {
+  const int pos = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, 1);
+  ei_list_of(d, b, node)[pos] = e;
 }
 
 // Appends slots i0 + a (bit a of `bits`) of every lane to node `node`'s list b:
@@ -532,8 +564,53 @@ __device__ __forceinline__ void wave_append_quad_to(uint32_t* L, int32_t* cnt, i
   }
 }
 
-__device__ __forceinline__ void wave_append_quad(const Dev& d, int b, int node, int64_t i0, uint32_t bits, int lane) {
-  wave_append_quad_to(ei_list_of(d, b, node), d.ei_cnt + (size_t)b * d.n_nodes + node, i0, bits, lane);
+
+// One compacted quad holding E or I agents: loads the quad's four records (one
+// 64-byte segment, four loads in flight together), runs the disease progression
+// of its E and I agents (model.py:419-452) and tallies them as E / I.  `wei`:
+// state bytes that identify today's E and I agents; `w`: the quad's state word
+// to update and write back if it differs from `w_orig`.  The agents still E or I
+// afterwards join node `node`'s list of tomorrow; returns them (bit a).
+__device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t live_hi, uint32_t wei, uint32_t w,
+                                        uint32_t w_orig, bool do_ds, bool do_tally, const EIConst& ec,
+                                        int32_t t, int32_t* lc, unsigned long long* lib, int32_t& c_r, int node) {
+  const uint4* rq = d.rec + i0;
+  uint4 r[4] = {rq[0], rq[1], rq[2], rq[3]};
+  uint32_t surv = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + a;
+    int8_t sa = byte_of(wei, a);
+    if ((sa != 1 && sa != 2) || i >= live_hi) continue;
+    const uint4 r0 = r[a];
+    const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
+    bool to_pp = false, to_par = false;
+    if (do_ds && progress_rec(sa, r[a], ec.seed, (uint64_t)(i - ec.slot_base), t, ec.thr_paralysis, to_pp, to_par))
+      c_r += 1;
+    if (to_pp) atomicAdd(&lc[L_NEWPP], 1);
+    if (to_par) atomicAdd(&lc[L_NEWP], 1);
+    if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
+      if (sa == 1) {
+        atomicAdd(&lc[L_E0 + k], 1);
+      } else if (sa == 2) {
+        atomicAdd(&lc[L_I0 + k], 1);
+        const float r0k = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
+        atomicAdd(&lib[k], (unsigned long long)weight_q24(__uint_as_float(r0.z) * r0k));
+      }
+    }
+    uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + i);
+    if (r[a].x != r0.x) rw[0] = r[a].x;
+    if (r[a].y != r0.y) rw[1] = r[a].y;
+    if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
+    if (sa != byte_of(wei, a)) *rec_byte(d, i, RB_STATE) = sa;
+    w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
+    if (sa == 1 || sa == 2) {                            // into tomorrow's list, timers as of tomorrow
+      surv |= 1u << a;
+      append_entry(d, (t + 1) & 1, node, entry_from_rec((uint32_t)i, sa, r[a], t + 1, ec.r0k));
+    }
+  }
+  if (w != w_orig) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w;
+  return surv;
 }
 
 // Remove an exposed agent's weight w from its chunk and from span s.
@@ -561,9 +638,9 @@ __device__ __forceinline__ int32_t* ri_cnt_of(const Dev& d, int b, int node) {
   return d.ri_lcnt + (size_t)b * d.n_nodes + node;
 }
 
-// Appends `slot` of each active lane with `flag` to node `node`'s list b (every
+// Appends entry `e` of each active lane with `flag` to node `node`'s list b (every
 // active lane passes the same node): one atomic per call.
-__device__ __forceinline__ void wave_append_slot(const Dev& d, int b, int node, int64_t slot, bool flag, int lane) {
+__device__ __forceinline__ void wave_append_entry(const Dev& d, int b, int node, const uint4& e, bool flag, int lane) {
   const uint64_t mk = __ballot(flag);
   if (mk == 0) return;
   const int first = __ffsll((unsigned long long)mk) - 1;
@@ -571,7 +648,13 @@ __device__ __forceinline__ void wave_append_slot(const Dev& d, int b, int node, 
   int base = 0;
   if (lane == first) base = atomicAdd(d.ei_cnt + (size_t)b * d.n_nodes + node, __popcll(mk));
   base = __shfl(base, first, 64);
-  if (flag) ei_list_of(d, b, node)[base + rank] = (uint32_t)slot;
+  if (flag) ei_list_of(d, b, node)[base + rank] = e;
+}
+
+// The entry of an agent exposed on day t with strain k (its exposure timer is the
+// record's, untouched while it was S).
+__device__ __forceinline__ uint4 exposed_entry(const Dev& d, int64_t slot, int k, int32_t t) {
+  return entry_e((uint32_t)slot, k, reinterpret_cast<const uint32_t*>(d.rec + slot)[0] & 0xFFFFu, t + 1);
 }
 
 // NodeInfo of every node (one thread per node).
@@ -721,7 +804,6 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 
   int32_t c_r = 0;
   const EIConst ec = ei_const(sp);
-  const int nxt = (t + 1) & 1;
   for (int base = 0; base < n_ei; base += 64) {
     int j = base + lane, su = -1, src = 0;
     static_for<U>([&](auto uc) {
@@ -732,15 +814,11 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
         else j -= c;
       }
     });
-    uint32_t surv = 0;
-    int64_t i0 = 0;
-    if (su >= 0) {
-      const int qt = wave * 64 + src;
-      i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
-      const uint32_t w0 = s_st[su][qt];
-      surv = ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, ec, t, lc, lib, c_r);
-    }
-    wave_append_quad(d, nxt, node, i0, surv, lane);
+    if (su < 0) continue;
+    const int qt = wave * 64 + src;
+    const int64_t i0 = ((int64_t)span.y + su) * LP_TILE + (int64_t)qt * 4;
+    const uint32_t w0 = s_st[su][qt];
+    ei_quad(d, i0, live_hi, w0, w0, w0, do_ds, do_tally, ec, t, lc, lib, c_r, node);
   }
   {
     const int32_t a2 = wave_sum_i32(c_r);
@@ -817,61 +895,97 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
     ec.slot_base = ni.slot_base;
 #pragma unroll
     for (int k = 0; k < LP_MAX_STRAINS; ++k) ec.r0k[k] = ni.r0k[k];
-    const uint32_t* L = d.ei_list + (size_t)cur * (size_t)d.capacity + ni.seg_begin;
+    const uint4* L = d.ei_list + (size_t)cur * (size_t)d.capacity + ni.seg_begin;
     // four entries per lane; lanes past the end repeat entry j0 (unconditional loads)
-    uint32_t sl[4];
+    uint4 ent[4];
     bool v[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int32_t j = j0 + q * 64 + lane;
       v[q] = j < cnt;
-      const uint32_t raw = L[v[q] ? j : j0];
-      if ((int64_t)raw >= d.capacity) { atomicOr(d.err, ERR_LIST); v[q] = false; }
-      sl[q] = (int64_t)raw < d.capacity ? raw : 0u;
+      ent[q] = L[v[q] ? j : j0];
+      if (v[q] && (int64_t)ent[q].x >= d.capacity) { atomicOr(d.err, ERR_LIST); v[q] = false; }
     }
-    int8_t s[4];
-    uint4 r[4];
     int32_t dd[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+    if (vd && !stopped) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { r[q] = d.rec[sl[q]]; s[q] = (int8_t)(r[q].w >> 16); }
-    if (vd) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) dd[q] = d.dod[sl[q]];
+      for (int q = 0; q < 4; ++q) if (v[q]) dd[q] = d.dod[ent[q].x];
     }
-    if (stopped) continue;
     // tallies: E and I counts per strain in 16-bit fields, I->R / new potentially
     // paralysed / new paralysed in 16-bit fields, infectious weight per strain
     int64_t e_pk = 0, i_pk = 0, misc = 0, ib0 = 0, ib1 = 0, ib2 = 0, ib3 = 0;
     uint32_t surv = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int8_t sa = s[q];
-      if (!v[q] || (sa != 1 && sa != 2) || dd[q] <= t) continue;   // dying today: the VD pass takes it
-      const uint32_t slot = sl[q];
-      const uint4 r0 = r[q];
-      const int k = (r0.y >> 16) & (LP_MAX_STRAINS - 1);
-      bool to_pp = false, to_par = false;
-      if (do_ds && progress_rec(sa, r[q], ec.seed, (uint64_t)((int64_t)slot - ec.slot_base), t, ec.thr_paralysis,
-                                to_pp, to_par))
-        misc += 1;
-      if (to_pp) misc += 1ll << 16;
-      if (to_par) misc += 1ll << 32;
+      uint4 e = ent[q];
+      int sa = ent_state(e);
+      if (!v[q] || (sa != 1 && sa != 2)) continue;
+      // a stopped simulation keeps its agents as they are: actual timers into the records
+      if (stopped) { ei_actual_timers(d, e, t); continue; }
+      const uint32_t slot = e.x;
+      if (dd[q] <= t) { ei_actual_timers(d, e, t); continue; }   // dies today: the VD pass takes it
+      const int k = ent_strain(e);
+      if (do_ds && ent_event(e) <= t) {
+        // today's transitions (model.py:419-452): the record is read and written only now
+        const uint4 r = d.rec[slot];
+        int32_t i_end = (int32_t)e.z, p_day = ent_pday(e);
+        bool s0 = ent_s0(e), pend = ent_pend(e), to_par = false, wy = false;
+        uint32_t x = r.x, y = r.y, wgt = e.w;
+        bool wx = false;
+        const int s_in = sa;
+        if (sa == 1) {                                    // E -> I: the exposure timer reads 0
+          sa = 2;
+          x &= 0xFFFF0000u;
+          wx = true;
+          i_end = t + (int32_t)(x >> 16);
+          s0 = k == 0;
+          pend = s0 && (int8_t)(y >> 24) == -1;
+          p_day = s0 ? t + (int32_t)(y & 0xFFFFu) : (int32_t)DAY_NONE;
+          const float rk = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
+          wgt = weight_entry(weight_q24(__uint_as_float(r.z) * rk));
+        }
+        if (pend && p_day <= t) {                         // the paralysis timer reads 0 with pp -1
+          pend = false;
+          wy = true;
+          if ((r.w & 0xFFu) == 0) {                       // not ipv_protected
+            y = (y & 0x00FFFFFFu) | (1u << 24);
+            misc += 1ll << 16;
+            const uint4 o = draw(ec.seed, (uint64_t)((int64_t)slot - ec.slot_base), t, ST_PARALYSIS);
+            if ((uint64_t)o.x < ec.thr_paralysis) { to_par = true; misc += 1ll << 32; }
+          } else {
+            y = y & 0x00FFFFFFu;
+          }
+        }
+        if (i_end <= t) {                                 // I -> R: the infection timer reads 0
+          sa = 3;
+          misc += 1;
+          // the infection timer stays 0; strain 0's paralysis timer stops after today's count-down
+          if (s0) y = (y & 0xFFFF0000u) | timer_left(p_day, t + 1);
+          reinterpret_cast<uint2*>(d.rec + slot)[0] = make_uint2(x & 0xFFFFu, y);
+        } else {
+          if (wx) reinterpret_cast<uint32_t*>(d.rec + slot)[0] = x;
+          if (wy) *rec_byte(d, slot, RB_PP) = (int8_t)(y >> 24);
+        }
+        if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
+        if (sa != s_in) { d.state[slot] = (int8_t)sa; *rec_byte(d, slot, RB_STATE) = (int8_t)sa; }
+        if (sa == 2) e = make_uint4(slot, ent_flags_i(k, s0, pend, p_day), (uint32_t)i_end, wgt);
+      }
+      if (sa != 1 && sa != 2) continue;
+      surv |= 1u << q;
+      ent[q] = e;
       if (do_tally) {                                     // tx_step_prep / count_SEIRP tally of E and I
         if (sa == 1) {
           e_pk += 1ll << (16 * k);
-        } else if (sa == 2) {
+        } else {
           i_pk += 1ll << (16 * k);
-          const float r0k = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
-          const int64_t wv = weight_q24(__uint_as_float(r0.z) * r0k);
+          int64_t wv = (int64_t)e.w;
+          if (e.w == W_REC) {
+            const float rk = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
+            wv = weight_q24(__uint_as_float(d.rec[slot].z) * rk);
+          }
           ib0 += k == 0 ? wv : 0; ib1 += k == 1 ? wv : 0; ib2 += k == 2 ? wv : 0; ib3 += k == 3 ? wv : 0;
         }
       }
-      uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + slot);
-      if (r[q].x != r0.x) rw[0] = r[q].x;
-      if (r[q].y != r0.y) rw[1] = r[q].y;
-      if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
-      if (sa != s[q]) { d.state[slot] = sa; *rec_byte(d, slot, RB_STATE) = sa; }
-      if (sa == 1 || sa == 2) surv |= 1u << q;
     }
     // survivors -> next day's list: one atomic per piece
     {
@@ -887,12 +1001,13 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
         int base = 0;
         if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)nxt * N + n, tot);
         base = __shfl(base, 0, 64);
-        uint32_t* Ln = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin;
+        uint4* Ln = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          if ((surv >> q) & 1u) Ln[base + rank[q]] = sl[q];
+          if ((surv >> q) & 1u) Ln[base + rank[q]] = ent[q];
       }
     }
+    if (stopped) continue;
     // the piece's tallies -> node_acc (lanes 0..14 one field each)
     e_pk = wave_sum_i64(e_pk); i_pk = wave_sum_i64(i_pk); misc = wave_sum_i64(misc);
     ib0 = wave_sum_i64(ib0); ib1 = wave_sum_i64(ib1); ib2 = wave_sum_i64(ib2); ib3 = wave_sum_i64(ib3);
@@ -915,6 +1030,20 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
     }
     if (f >= 0 && val != 0)
       atomicAdd(reinterpret_cast<unsigned long long*>(d.node_acc + (size_t)n * NPART + f), (unsigned long long)val);
+  }
+}
+
+// Actual timers into the records of every agent in E / I list b, whose entries are
+// as of day t_next (one workgroup per node): run before anything reads the records'
+// timers after days that processed the list (column sync, stage calls, a day
+// without DiseaseState).
+__global__ void __launch_bounds__(WG) k_ei_actual(Dev d, int b, int32_t t_next) {
+  const int n = blockIdx.x;
+  const int32_t cnt = d.ei_cnt[(size_t)b * d.n_nodes + n];
+  const uint4* L = ei_list_of(d, b, n);
+  for (int32_t j = threadIdx.x; j < cnt; j += WG) {
+    const uint4 e = L[j];
+    if ((int64_t)e.x < d.capacity) ei_actual_timers(d, e, t_next);
   }
 }
 
@@ -1022,7 +1151,10 @@ __global__ void __launch_bounds__(WG) k_ri_list(Dev d, int mask) {
     }
     if (__ballot(expo != 0)) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) wave_append_slot(d, nxt, n, (int64_t)sl[q], (expo >> q) & 1u, lane);
+      for (int q = 0; q < 4; ++q) {
+        const bool f = (expo >> q) & 1u;
+        wave_append_entry(d, nxt, n, f ? exposed_entry(d, sl[q], rk, t) : make_uint4(0u, 0u, 0u, 0u), f, lane);
+      }
     }
   }
   const int32_t a0 = wave_sum_i32(c_riv), a1 = wave_sum_i32(c_rip), a2 = wave_sum_i32(c_ipv),
@@ -1102,8 +1234,8 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   constexpr int GW = EI ? G : 1;
   constexpr int GW2 = EI && (F & (FULL_RI | FULL_SIA)) ? G : 1;
   __shared__ uint32_t s_w0[GW][WG], s_w1[GW][WG], s_w2[GW2][WG];
-  // per quad of the span: its agents that are E or I after today's pass (bit a),
-  // appended to the next day's list once at the end
+  // per quad of the span: its agents exposed by RI / SIA today (bit a), appended to
+  // the next day's list once at the end (the inline E / I stage appends its own)
   __shared__ uint8_t s_sv[MAX_TPW][WG];
   __shared__ int32_t lc[L_N];
   __shared__ unsigned long long lib[LP_MAX_STRAINS];
@@ -1381,9 +1513,8 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       const int qt = wave * 64 + src;
       const int64_t i0 = ((int64_t)tile0 + su) * LP_TILE + (int64_t)qt * 4;
       const uint32_t wb = GW2 == G ? s_w2[GW2 == G ? su : 0][qt] : s_w1[GW == G ? su : 0][qt];
-      const uint32_t surv = ei_quad(d, i0, live_hi, s_w1[GW == G ? su : 0][qt], wb, s_w0[GW == G ? su : 0][qt],
-                                    do_ds, do_tally, ec, t, lc, lib, c_r);
-      if (surv) s_sv[g0 + su][qt] |= (uint8_t)surv;
+      ei_quad(d, i0, live_hi, s_w1[GW == G ? su : 0][qt], wb, s_w0[GW == G ? su : 0][qt], do_ds, do_tally, ec, t,
+              lc, lib, c_r, node);
     }
     // RI's IPV doses land after DiseaseState has read ipv_protected (model.py:226-241)
     if (RI && ipv_new) {
@@ -1399,7 +1530,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   }
 
   {
-    // the span's E / I agents after today -> next day's list: one atomic per wave
+    // the span's RI / SIA exposures -> next day's list: one atomic per wave
     int c = 0;
     for (int u = 0; u < span0.z; ++u) c += __popc(s_sv[u][tid]);
     const int incl = (int)wave_incl_scan_i64(c, lane);
@@ -1408,13 +1539,13 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       int base = 0;
       if (lane == 0) base = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, tot);
       int pos = __shfl(base, 0, 64) + incl - c;
-      uint32_t* L = ei_list_of(d, nxt, node);
+      uint4* L = ei_list_of(d, nxt, node);
       for (int u = 0; u < span0.z; ++u) {
         const uint32_t bits = s_sv[u][tid];
         const int64_t i0 = ((int64_t)span0.y + u) * LP_TILE + (int64_t)tid * 4;
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-          if ((bits >> a) & 1u) L[pos++] = (uint32_t)(i0 + a);
+          if ((bits >> a) & 1u) L[pos++] = exposed_entry(d, i0 + a, *rec_byte(d, i0 + a, RB_STRAIN), t);
       }
     }
   }
@@ -1735,13 +1866,17 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
         if (do_ri && (r2.x != rt[u].x || r2.y != rt[u].y)) *reinterpret_cast<uint2*>(d.ri_timer + i0) = r2;
       }
       if (u < span.z) {                                   // E / I list of the next day (uniform condition)
-        uint32_t bits = 0;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int8_t s = byte_of(wf, a);
-          if (tv[u] && i0 + a < live_hi && (s == 1 || s == 2)) bits |= 1u << a;
+          if (!(tv[u] && i0 + a < live_hi && (s == 1 || s == 2))) continue;
+          const int64_t i = i0 + a;   // timers as written by this pass: as of tomorrow
+          const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
+          const uint4 e = s == 1 ? entry_e((uint32_t)i, k, d.exp_timer[i], t + 1)
+                                 : entry_i((uint32_t)i, k, d.inf_timer[i], d.par_timer[i], d.pp[i], t + 1,
+                                           weight_q24(d.infect[i] * sp.strain_r0[k]));
+          append_entry(d, (t + 1) & 1, node, e);
         }
-        wave_append_quad(d, (t + 1) & 1, node, i0, bits, lane);
         // RI candidates: live agents not chronically missed whose window can still open
         // (ri_timer is actual here; re-read after this pass's own RI step)
         uint32_t rb = 0;
@@ -2363,13 +2498,13 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
         if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
-        wave_append_slot(d, nxt, n0, slot, node >= 0, lane);
+        wave_append_entry(d, nxt, n0, node >= 0 ? exposed_entry(d, slot, k, t) : make_uint4(0u, 0u, 0u, 0u),
+                          node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
       atomicAdd(d.n_uniq + node, 1);
-      const int pos = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, 1);
-      ei_list_of(d, nxt, node)[pos] = (uint32_t)slot;
+      append_entry(d, nxt, node, exposed_entry(d, slot, k, t));
     }
   }
 }
@@ -2398,12 +2533,14 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
     for (int64_t i0 = lo; i0 < hi; i0 += WG) {         // uniform trip count: whole waves append
       const int64_t i = i0 + tid;
       bool ex = false;
+      int kx = 0;
       if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
         unweight(d, n, i);
-        atomicAdd(&s_ne[expose_agent_nocount(d, n, ni.seed, ni.slot_base, i, t)], 1);
+        kx = expose_agent_nocount(d, n, ni.seed, ni.slot_base, i, t);
+        atomicAdd(&s_ne[kx], 1);
         ex = true;
       }
-      wave_append_slot(d, nxt, n, i, ex, lane);
+      wave_append_entry(d, nxt, n, ex ? exposed_entry(d, i, kx, t) : make_uint4(0u, 0u, 0u, 0u), ex, lane);
     }
   } else {
     const int s0 = ni.span_begin, s1 = ni.span_begin + ni.n_spans;
@@ -2457,8 +2594,13 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
                                                         __HIP_MEMORY_SCOPE_AGENT)
                                     : SEL_INVALID;
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
-        if (e != SEL_INVALID) atomicAdd(&s_ne[expose_agent_nocount(d, n, ni.seed, ni.slot_base, slot, t)], 1);
-        wave_append_slot(d, nxt, n, slot, e != SEL_INVALID, lane);
+        const bool ex = e != SEL_INVALID;
+        int kx = 0;
+        if (ex) {
+          kx = expose_agent_nocount(d, n, ni.seed, ni.slot_base, slot, t);
+          atomicAdd(&s_ne[kx], 1);
+        }
+        wave_append_entry(d, nxt, n, ex ? exposed_entry(d, slot, kx, t) : make_uint4(0u, 0u, 0u, 0u), ex, lane);
       }
       __threadfence();
       __syncthreads();
