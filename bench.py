@@ -44,31 +44,28 @@ def log(*a):
 def agent_pass_bytes(res, flags, t0, t1):
     """Algorithmic bytes of the agent pass over ticks [t0, t1).
 
-    DiseaseState and the E / I tally run over the per-node E / I lists: per E or I agent
-    its list entry read and written back (4 + 4) and its state (1); per E agent 5 more
-    (strain, exposure timer read and written), per I agent 9 (strain, infection timer
-    read and written, infectivity), per strain-0 I agent 3 more (paralysis timer and
-    potentially_paralyzed, read by DiseaseState for strain-0 infections only).  VD step
-    days add the per-chunk earliest death dates (4 B per 256 slots), 6 B per death
-    (date_of_death, potentially_paralyzed, paralyzed) and 17 B per birth (date_of_birth,
-    date_of_death, node_id, ri_timer and state written, risk read).  RI and SIA days scan
-    every agent's state and chronically_missed (2 B); RI days add ri_timer (2 B, read:
-    the decrement is applied by arithmetic), SIA days date_of_birth (4 B).  N_live = live
-    agents that day (results.pop); E, I and strain-0 I are the counts at the start of the
-    day (the previous row).
+    DiseaseState and the E / I tally cost 17 B per E or I agent: its timers read and
+    written (2 x (2 + 2)), strain (1), infectivity (4) and the paralysis fields (4); the
+    state bytes of S and R agents are not compulsory once E and I agents are found
+    without a scan.  VD step days add the per-chunk earliest death dates (4 B per 256
+    slots), 6 B per death (date_of_death, potentially_paralyzed, paralyzed) and 17 B per
+    birth (date_of_birth, date_of_death, node_id, ri_timer and state written, risk read).
+    RI and SIA days scan every agent's state and chronically_missed (2 B); RI days add
+    ri_timer (2 B, read: the decrement is applied by arithmetic), SIA days date_of_birth
+    (4 B).  N_live = live agents that day (results.pop); E and I are the counts at the
+    start of the day (the previous row).
     """
     from laser_polio_amd import engine as em
 
     pop = res["pop"].astype(np.int64).sum(axis=1)
     n_e = res["E"].astype(np.int64).sum(axis=1)
     n_i = res["I"].astype(np.int64).sum(axis=1)
-    n_i0 = res["I_by_strain"][:, :, 0].astype(np.int64).sum(axis=1)
     births = res["births"].astype(np.int64).sum(axis=1) if "births" in res else np.zeros_like(pop)
     deaths = res["deaths"].astype(np.int64).sum(axis=1) if "deaths" in res else np.zeros_like(pop)
     total = 0
     for t in range(t0, t1):
         n_live, f = int(pop[t]), int(flags[t])
-        b = 14 * int(n_e[t - 1]) + 18 * int(n_i[t - 1]) + 3 * int(n_i0[t - 1])
+        b = 17 * (int(n_e[t - 1]) + int(n_i[t - 1]))
         if f & em.DAY_VD_STEP:
             b += 4 * (n_live // 256) + 6 * int(deaths[t]) + 17 * int(births[t])
         if f & (em.DAY_RI | em.DAY_SIA):

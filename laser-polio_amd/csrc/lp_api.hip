@@ -314,8 +314,9 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
                      kind < lp::KIND_PLAIN_LIST ? 0 : spans_written ? 2 : 1);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
-    hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
+    // with the probe offsets in LDS k_sample plans round 1 itself
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
+    if (!off_lds) hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
     hipLaunchKernelGGL(lp::k_sample, dim3(2048), dim3(lp::WG), off_lds, s, d);
     hipLaunchKernelGGL(lp::k_commit, dim3(512), dim3(lp::WG), 0, s, d);
     hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);

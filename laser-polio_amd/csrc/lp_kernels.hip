@@ -23,7 +23,8 @@
 //                  weighted sampling without replacement by rounds of with-
 //                  replacement probes + unique (model.py:1089-1122): every probe
 //                  searches its node's span prefix, then a span's chunks, then one
-//                  chunk, one wavefront per probe; round 1 grid-wide (k_commit
+//                  chunk, one wavefront per probe; round 1 grid-wide (planned
+//                  inside k_sample, by k_plan above 8191 nodes; k_commit
 //                  claims and exposes its agents), later rounds and the take-all
 //                  nodes per node in k_finish; each exposed agent's strain is drawn
 //                  as it is exposed (S -> E) and it joins the next day's E / I list
@@ -463,14 +464,17 @@ __global__ void k_set_ri_ref(Dev d, int32_t through) {
 // I agent's countdowns are kept as absolute days (the day its timer reads 0 at the
 // start of DiseaseState) and its record keeps the timers of when it entered the
 // list (an I agent's exposure timer is actual: 0 after E -> I, as drawn for one
-// infected from the start); ei_actual_timers writes the actual timers back.
+// infected from the start); ei_actual_timers writes the actual timers back.  An E
+// entry also holds what E -> I needs (none of it changes while S or E), so the
+// list pass reads a record only for the paralysis decision (ipv_protected).
 //   x: slot
 //   y: bits 0-1 state (1 E, 2 I); 2 strain 0 (paralysis timer tracked); 3 paralysis
-//      decision pending (potentially_paralyzed -1); 4-5 strain; 8-31 day the
-//      paralysis timer reads 0 (strain-0 I)
-//   z: day the exposure timer (E) or the infection timer (I) reads 0
-//   w: infectious weight rint(infectivity * strain r0 * 2^24) (I); W_REC when it
-//      does not fit 32 bits (then recomputed from the record)
+//      decision pending (potentially_paralyzed -1); 4-5 strain; 8-31 E: day the
+//      exposure timer reads 0, I: day the paralysis timer reads 0 (strain 0)
+//   z: E: infection timer | paralysis timer << 16; I: day the infection timer
+//      reads 0
+//   w: infectious weight rint(infectivity * strain r0 * 2^24); W_REC when it does
+//      not fit 32 bits (then recomputed from the record)
 constexpr uint32_t W_REC = 0xFFFFFFFFu;
 constexpr uint32_t DAY_NONE = 0xFFFFFFu;
 
@@ -478,37 +482,43 @@ __device__ __forceinline__ int ent_state(const uint4& e) { return (int)(e.y & 3u
 __device__ __forceinline__ int ent_strain(const uint4& e) { return (int)((e.y >> 4) & 3u); }
 __device__ __forceinline__ bool ent_s0(const uint4& e) { return (e.y >> 2) & 1u; }
 __device__ __forceinline__ bool ent_pend(const uint4& e) { return (e.y >> 3) & 1u; }
-__device__ __forceinline__ int32_t ent_pday(const uint4& e) { return (int32_t)(e.y >> 8); }
+__device__ __forceinline__ int32_t ent_day(const uint4& e) { return (int32_t)(e.y >> 8); }
 // The next day the entry's agent changes: E -> I, the paralysis decision, I -> R.
 __device__ __forceinline__ int32_t ent_event(const uint4& e) {
-  const int32_t d1 = (int32_t)e.z, p = ent_pday(e);
-  return ent_pend(e) && p < d1 ? p : d1;
+  const int32_t d8 = ent_day(e);
+  if (ent_state(e) == 1) return d8;
+  return ent_pend(e) && d8 < (int32_t)e.z ? d8 : (int32_t)e.z;
 }
-__device__ __forceinline__ uint32_t ent_flags_i(int k, bool s0, bool pend, int32_t p_day) {
-  return 2u | (s0 ? 4u : 0u) | (pend ? 8u : 0u) | ((uint32_t)k << 4) | ((uint32_t)p_day << 8);
+__device__ __forceinline__ uint32_t ent_flags(int s, int k, bool pend, int32_t day) {
+  return (uint32_t)s | (k == 0 ? 4u : 0u) | (pend ? 8u : 0u) | ((uint32_t)k << 4) | ((uint32_t)day << 8);
 }
 __device__ __forceinline__ uint32_t weight_entry(int64_t w) { return w >= (int64_t)W_REC ? W_REC : (uint32_t)w; }
+__device__ __forceinline__ float r0_sel(const float* r0k, int k) {
+  return k == 0 ? r0k[0] : k == 1 ? r0k[1] : k == 2 ? r0k[2] : r0k[3];
+}
 
-// Entry of an E agent whose exposure timer reads `et` at the start of day t_next.
-__device__ __forceinline__ uint4 entry_e(uint32_t slot, int k, uint32_t et, int32_t t_next) {
-  return make_uint4(slot, 1u | ((uint32_t)k << 4), (uint32_t)t_next + et, 0u);
+// Entry of an E agent with timers `et`, `it`, `pt`, potentially_paralyzed `pp` at the
+// start of day t_next and infectious weight `weight` once I.
+__device__ __forceinline__ uint4 entry_e(uint32_t slot, int k, uint32_t et, uint32_t it, uint32_t pt, int8_t pp,
+                                         int32_t t_next, int64_t weight) {
+  return make_uint4(slot, ent_flags(1, k, k == 0 && pp == -1, t_next + (int32_t)et), it | (pt << 16),
+                    weight_entry(weight));
 }
 
 // Entry of an I agent with timers `it`, `pt` and potentially_paralyzed `pp` at the start
 // of day t_next.
 __device__ __forceinline__ uint4 entry_i(uint32_t slot, int k, uint32_t it, uint32_t pt, int8_t pp, int32_t t_next,
                                          int64_t weight) {
-  const bool s0 = k == 0, pend = s0 && pp == -1;
-  const int32_t p_day = s0 ? t_next + (int32_t)pt : (int32_t)DAY_NONE;
-  return make_uint4(slot, ent_flags_i(k, s0, pend, p_day), (uint32_t)t_next + it, weight_entry(weight));
+  const int32_t p_day = k == 0 ? t_next + (int32_t)pt : (int32_t)DAY_NONE;
+  return make_uint4(slot, ent_flags(2, k, k == 0 && pp == -1, p_day), (uint32_t)t_next + it, weight_entry(weight));
 }
 
 // Entry of an E or I agent from its (actual) record at the start of day t_next.
 __device__ __forceinline__ uint4 entry_from_rec(uint32_t slot, int s, const uint4& r, int32_t t_next, const float* r0k) {
   const int k = (int)((r.y >> 16) & (LP_MAX_STRAINS - 1));
-  if (s == 1) return entry_e(slot, k, r.x & 0xFFFFu, t_next);
-  const float rk = k == 0 ? r0k[0] : k == 1 ? r0k[1] : k == 2 ? r0k[2] : r0k[3];
-  return entry_i(slot, k, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t_next, weight_q24(__uint_as_float(r.z) * rk));
+  const int64_t w = weight_q24(__uint_as_float(r.z) * r0_sel(r0k, k));
+  if (s == 1) return entry_e(slot, k, r.x & 0xFFFFu, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t_next, w);
+  return entry_i(slot, k, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t_next, w);
 }
 
 // A timer that reads 0 on day `end`, as of the start of day t_next (it stays at 0).
@@ -519,13 +529,12 @@ __device__ __forceinline__ uint32_t timer_left(int32_t end, int32_t t_next) {
 
 // The record's actual timers of an E or I agent at the start of day t_next.
 __device__ __forceinline__ void ei_actual_timers(const Dev& d, const uint4& e, int32_t t_next) {
-  uint32_t* rw = reinterpret_cast<uint32_t*>(d.rec + e.x);
-  uint16_t* hw = reinterpret_cast<uint16_t*>(rw);
+  uint16_t* hw = reinterpret_cast<uint16_t*>(d.rec + e.x);
   if (ent_state(e) == 1) {
-    hw[0] = (uint16_t)timer_left((int32_t)e.z, t_next);          // exposure timer
+    hw[0] = (uint16_t)timer_left(ent_day(e), t_next);           // exposure timer
   } else {
-    hw[1] = (uint16_t)timer_left((int32_t)e.z, t_next);          // infection timer
-    if (ent_s0(e)) hw[2] = (uint16_t)timer_left(ent_pday(e), t_next);   // paralysis timer
+    hw[1] = (uint16_t)timer_left((int32_t)e.z, t_next);         // infection timer
+    if (ent_s0(e)) hw[2] = (uint16_t)timer_left(ent_day(e), t_next);   // paralysis timer
   }
 }
 
@@ -651,10 +660,13 @@ __device__ __forceinline__ void wave_append_entry(const Dev& d, int b, int node,
   if (flag) ei_list_of(d, b, node)[base + rank] = e;
 }
 
-// The entry of an agent exposed on day t with strain k (its exposure timer is the
-// record's, untouched while it was S).
-__device__ __forceinline__ uint4 exposed_entry(const Dev& d, int64_t slot, int k, int32_t t) {
-  return entry_e((uint32_t)slot, k, reinterpret_cast<const uint32_t*>(d.rec + slot)[0] & 0xFFFFu, t + 1);
+// The entry of an agent exposed on day t with strain k, whose r0 scalar is rk (its
+// timers, potentially_paralyzed and infectivity are the record's, untouched while
+// it was S).
+__device__ __forceinline__ uint4 exposed_entry(const Dev& d, int64_t slot, int k, int32_t t, float rk) {
+  const uint4 r = d.rec[slot];
+  return entry_e((uint32_t)slot, k, r.x & 0xFFFFu, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t + 1,
+                 weight_q24(__uint_as_float(r.z) * rk));
 }
 
 // NodeInfo of every node (one thread per node).
@@ -926,49 +938,38 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
       if (dd[q] <= t) { ei_actual_timers(d, e, t); continue; }   // dies today: the VD pass takes it
       const int k = ent_strain(e);
       if (do_ds && ent_event(e) <= t) {
-        // today's transitions (model.py:419-452): the record is read and written only now
-        const uint4 r = d.rec[slot];
-        int32_t i_end = (int32_t)e.z, p_day = ent_pday(e);
-        bool s0 = ent_s0(e), pend = ent_pend(e), to_par = false, wy = false;
-        uint32_t x = r.x, y = r.y, wgt = e.w;
-        bool wx = false;
+        // today's transitions (model.py:419-452)
+        uint16_t* hw = reinterpret_cast<uint16_t*>(d.rec + slot);
+        int32_t i_end = (int32_t)e.z, p_day = ent_day(e);
+        bool pend = ent_pend(e), to_par = false;
+        const bool s0 = k == 0;
         const int s_in = sa;
         if (sa == 1) {                                    // E -> I: the exposure timer reads 0
           sa = 2;
-          x &= 0xFFFF0000u;
-          wx = true;
-          i_end = t + (int32_t)(x >> 16);
-          s0 = k == 0;
-          pend = s0 && (int8_t)(y >> 24) == -1;
-          p_day = s0 ? t + (int32_t)(y & 0xFFFFu) : (int32_t)DAY_NONE;
-          const float rk = k == 0 ? ec.r0k[0] : k == 1 ? ec.r0k[1] : k == 2 ? ec.r0k[2] : ec.r0k[3];
-          wgt = weight_entry(weight_q24(__uint_as_float(r.z) * rk));
+          i_end = t + (int32_t)(e.z & 0xFFFFu);
+          p_day = s0 ? t + (int32_t)(e.z >> 16) : (int32_t)DAY_NONE;
+          hw[0] = 0;
         }
         if (pend && p_day <= t) {                         // the paralysis timer reads 0 with pp -1
           pend = false;
-          wy = true;
-          if ((r.w & 0xFFu) == 0) {                       // not ipv_protected
-            y = (y & 0x00FFFFFFu) | (1u << 24);
+          int8_t pp = 0;
+          if (*rec_byte(d, slot, RB_IPV) == 0) {          // not ipv_protected
+            pp = 1;
             misc += 1ll << 16;
             const uint4 o = draw(ec.seed, (uint64_t)((int64_t)slot - ec.slot_base), t, ST_PARALYSIS);
             if ((uint64_t)o.x < ec.thr_paralysis) { to_par = true; misc += 1ll << 32; }
-          } else {
-            y = y & 0x00FFFFFFu;
           }
+          *rec_byte(d, slot, RB_PP) = pp;
         }
         if (i_end <= t) {                                 // I -> R: the infection timer reads 0
           sa = 3;
           misc += 1;
-          // the infection timer stays 0; strain 0's paralysis timer stops after today's count-down
-          if (s0) y = (y & 0xFFFF0000u) | timer_left(p_day, t + 1);
-          reinterpret_cast<uint2*>(d.rec + slot)[0] = make_uint2(x & 0xFFFFu, y);
-        } else {
-          if (wx) reinterpret_cast<uint32_t*>(d.rec + slot)[0] = x;
-          if (wy) *rec_byte(d, slot, RB_PP) = (int8_t)(y >> 24);
+          hw[1] = 0;
+          if (s0) hw[2] = (uint16_t)timer_left(p_day, t + 1);   // stops after today's count-down
         }
         if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
         if (sa != s_in) { d.state[slot] = (int8_t)sa; *rec_byte(d, slot, RB_STATE) = (int8_t)sa; }
-        if (sa == 2) e = make_uint4(slot, ent_flags_i(k, s0, pend, p_day), (uint32_t)i_end, wgt);
+        if (sa == 2) e = make_uint4(slot, ent_flags(2, k, pend, p_day), (uint32_t)i_end, e.w);
       }
       if (sa != 1 && sa != 2) continue;
       surv |= 1u << q;
@@ -1153,7 +1154,8 @@ __global__ void __launch_bounds__(WG) k_ri_list(Dev d, int mask) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const bool f = (expo >> q) & 1u;
-        wave_append_entry(d, nxt, n, f ? exposed_entry(d, sl[q], rk, t) : make_uint4(0u, 0u, 0u, 0u), f, lane);
+        wave_append_entry(d, nxt, n, f ? exposed_entry(d, sl[q], rk, t, sp.strain_r0[rk]) : make_uint4(0u, 0u, 0u, 0u),
+                          f, lane);
       }
     }
   }
@@ -1545,7 +1547,10 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
         const int64_t i0 = ((int64_t)span0.y + u) * LP_TILE + (int64_t)tid * 4;
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-          if ((bits >> a) & 1u) L[pos++] = exposed_entry(d, i0 + a, *rec_byte(d, i0 + a, RB_STRAIN), t);
+          if ((bits >> a) & 1u) {
+            const int k = *rec_byte(d, i0 + a, RB_STRAIN) & (LP_MAX_STRAINS - 1);
+            L[pos++] = exposed_entry(d, i0 + a, k, t, r0_sel(ec.r0k, k));
+          }
       }
     }
   }
@@ -1872,9 +1877,10 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
           if (!(tv[u] && i0 + a < live_hi && (s == 1 || s == 2))) continue;
           const int64_t i = i0 + a;   // timers as written by this pass: as of tomorrow
           const int k = d.strain[i] & (LP_MAX_STRAINS - 1);
-          const uint4 e = s == 1 ? entry_e((uint32_t)i, k, d.exp_timer[i], t + 1)
-                                 : entry_i((uint32_t)i, k, d.inf_timer[i], d.par_timer[i], d.pp[i], t + 1,
-                                           weight_q24(d.infect[i] * sp.strain_r0[k]));
+          const int64_t w = weight_q24(d.infect[i] * sp.strain_r0[k]);
+          const uint4 e = s == 1 ? entry_e((uint32_t)i, k, d.exp_timer[i], d.inf_timer[i], d.par_timer[i], d.pp[i],
+                                           t + 1, w)
+                                 : entry_i((uint32_t)i, k, d.inf_timer[i], d.par_timer[i], d.pp[i], t + 1, w);
           append_entry(d, (t + 1) & 1, node, e);
         }
         // RI candidates: live agents not chronically missed whose window can still open
@@ -2064,25 +2070,53 @@ __device__ bool last_block(uint32_t* ticket) {
   return am_last;
 }
 
-// Exclusive scan of the probes of sampling round `round` over nodes (probe_off),
-// and reservation of their space in the selected list; run by the first
-// wavefront of one workgroup.
-__device__ void plan_scan(const Dev& d, int round) {
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
-  const int N = d.n_nodes;
-  const int per = (N + 63) / 64;
-  const int a = lane * per, b = (a + per < N) ? a + per : N;
-  int64_t s = 0;
-  for (int n = a; n < b; ++n) s += d.m_round[n];   // written by k_mix_foi (an earlier launch)
-  const int64_t incl = wave_incl_scan_i64(s, lane);
-  int64_t run = incl - s;
-  for (int n = a; n < b; ++n) {
-    d.probe_off[n] = run;
-    run += d.m_round[n];
+// Exclusive prefix over n in [0, N) of cnt(n), by every thread of one workgroup:
+// emit(n, prefix) for each n; returns the total.  Each pass takes 4 * WG nodes, 4
+// consecutive ones per thread with their loads in flight together.
+template <typename Cnt, typename Emit>
+__device__ int64_t block_scan_nodes(int N, Cnt cnt, Emit emit) {
+  __shared__ int64_t s_w[WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int64_t carry = 0;
+  for (int base = 0; base < N; base += 4 * WG) {
+    int64_t v[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int n = base + 4 * tid + a;
+      v[a] = n < N ? cnt(n) : 0;
+    }
+    const int64_t sum = v[0] + v[1] + v[2] + v[3];
+    const int64_t incl = wave_incl_scan_i64(sum, lane);
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    int64_t off = carry + incl - sum, tot = 0;
+#pragma unroll
+    for (int j = 0; j < WG / 64; ++j) {
+      const int64_t x = s_w[j];
+      off += j < w ? x : 0;
+      tot += x;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int n = base + 4 * tid + a;
+      if (n < N) emit(n, off);
+      off += v[a];
+    }
+    carry += tot;
+    __syncthreads();
   }
-  if (lane == 63) {
-    int64_t M = incl;
+  return carry;
+}
+
+// Exclusive scan of the probes of sampling round `round` over nodes (probe_off),
+// and reservation of their space in the selected list; run by one workgroup.
+__device__ void plan_scan(const Dev& d, int round) {
+  const int N = d.n_nodes;
+  const int64_t total = block_scan_nodes(
+      N, [&](int n) { return (int64_t)d.m_round[n]; },   // written by k_mix_foi (an earlier launch)
+      [&](int n, int64_t off) { d.probe_off[n] = off; });
+  if (threadIdx.x == 0) {
+    int64_t M = total;
     const int64_t base = __hip_atomic_load(d.sel_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (base + M > d.sel_cap) { atomicOr(d.err, ERR_OVERFLOW); M = 0; }
     d.probe_off[N] = M;
@@ -2247,7 +2281,8 @@ __global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, ro
 // ---------------------------------------------------------------------------
 // Infection sampling (model.py:1089-1122): rounds of `need` with-replacement
 // probes proportional to the remaining selection weights; the distinct agents
-// hit are exposed.  Round 1 runs grid-wide (k_plan, k_sample, k_commit);
+// hit are exposed.  Round 1 runs grid-wide (k_sample, with k_plan first above
+// LDS_OFF_MAX - 1 nodes, then k_commit);
 // k_finish then takes each node on its own for its later rounds and handles the
 // take-all nodes.
 
@@ -2382,25 +2417,43 @@ __device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, uint
 
 
 // Sampling round 1, grid-wide: one wavefront per probe resolves it to an agent
-// (k_commit claims the hits).  The probe offsets are staged in LDS when they fit
-// (dynamic shared memory of (n_nodes + 1) * 8 bytes, else none).
+// (k_commit claims the hits).  When the probe offsets fit in LDS (dynamic shared
+// memory of (n_nodes + 1) * 8 bytes) every workgroup scans the nodes' round-1 probe
+// counts itself and workgroup 0 publishes the plan (round 1 is the day's first
+// reservation in the selected list: base 0); otherwise k_plan planned the round.
 constexpr int LDS_OFF_MAX = 8192;
 __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   extern __shared__ int64_t s_off[];
   const int N = d.n_nodes;
-  const int64_t M = d.probe_off[N];
+  const bool lds = N + 1 <= LDS_OFF_MAX;
+  int64_t M;
+  if (lds) {
+    const bool pub = blockIdx.x == 0;
+    const int64_t total = block_scan_nodes(
+        N, [&](int n) { return (int64_t)d.m_round[n]; },   // written by k_mix_foi (an earlier launch)
+        [&](int n, int64_t off) {
+          s_off[n] = off;
+          if (pub) d.probe_off[n] = off;
+        });
+    M = total > d.sel_cap ? 0 : total;
+    if (threadIdx.x == 0) s_off[N] = M;
+    if (pub && threadIdx.x == 0) {
+      if (total > d.sel_cap) atomicOr(d.err, ERR_OVERFLOW);
+      d.probe_off[N] = M;
+      d.round_base[1] = 0;
+      d.sel_count[0] = M;
+    }
+    __syncthreads();
+  } else {
+    M = d.probe_off[N];
+  }
   const int64_t nw = (int64_t)gridDim.x * (WG / 64);
   const int64_t first = (int64_t)blockIdx.x * (WG / 64);
   if (first >= M) return;
-  const bool lds = N + 1 <= LDS_OFF_MAX;
-  if (lds) {
-    for (int i = threadIdx.x; i <= N; i += WG) s_off[i] = d.probe_off[i];
-    __syncthreads();
-  }
   const int64_t* off = lds ? s_off : d.probe_off;
   const int32_t t = d.day[0];
   const int lane = threadIdx.x & 63;
-  const int64_t base = d.round_base[1];
+  const int64_t base = lds ? 0 : d.round_base[1];
   // each wave takes a contiguous range of probes (probes are ordered by node); the
   // node's seed, weight and span prefix (one span per lane, up to 64 spans) are
   // loaded once per node, so a probe costs the chunk-weight and the agent loads
@@ -2498,13 +2551,14 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
         if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
-        wave_append_entry(d, nxt, n0, node >= 0 ? exposed_entry(d, slot, k, t) : make_uint4(0u, 0u, 0u, 0u),
+        wave_append_entry(d, nxt, n0,
+                          node >= 0 ? exposed_entry(d, slot, k, t, d.ninfo[n0].r0k[k]) : make_uint4(0u, 0u, 0u, 0u),
                           node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
       atomicAdd(d.n_uniq + node, 1);
-      append_entry(d, nxt, node, exposed_entry(d, slot, k, t));
+      append_entry(d, nxt, node, exposed_entry(d, slot, k, t, d.ninfo[node].r0k[k]));
     }
   }
 }
@@ -2540,7 +2594,8 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         atomicAdd(&s_ne[kx], 1);
         ex = true;
       }
-      wave_append_entry(d, nxt, n, ex ? exposed_entry(d, i, kx, t) : make_uint4(0u, 0u, 0u, 0u), ex, lane);
+      wave_append_entry(d, nxt, n, ex ? exposed_entry(d, i, kx, t, r0_sel(ni.r0k, kx)) : make_uint4(0u, 0u, 0u, 0u), ex,
+                        lane);
     }
   } else {
     const int s0 = ni.span_begin, s1 = ni.span_begin + ni.n_spans;
@@ -2600,7 +2655,8 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           kx = expose_agent_nocount(d, n, ni.seed, ni.slot_base, slot, t);
           atomicAdd(&s_ne[kx], 1);
         }
-        wave_append_entry(d, nxt, n, ex ? exposed_entry(d, slot, kx, t) : make_uint4(0u, 0u, 0u, 0u), ex, lane);
+        wave_append_entry(d, nxt, n, ex ? exposed_entry(d, slot, kx, t, r0_sel(ni.r0k, kx)) : make_uint4(0u, 0u, 0u, 0u),
+                          ex, lane);
       }
       __threadfence();
       __syncthreads();
@@ -2651,19 +2707,14 @@ __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_ad
 }
 
 // ei_piece_off = exclusive prefix over nodes of the pieces of list b (the last
-// entry is the total); run by the first wavefront of one workgroup.
+// entry is the total); run by every thread of one workgroup.
 __device__ void ei_plan(const Dev& d, int b) {
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x, N = d.n_nodes;
-  const int per = (N + 63) / 64;
-  const int n0 = lane * per, n1 = n0 + per < N ? n0 + per : N;
+  const int N = d.n_nodes;
   const int32_t* cnt = d.ei_cnt + (size_t)b * N;
-  int32_t acc = 0;
-  for (int n = n0; n < n1; ++n) acc += (cnt[n] + EI_PIECE - 1) / EI_PIECE;
-  const int64_t incl = wave_incl_scan_i64(acc, lane);
-  int32_t off = (int32_t)incl - acc;
-  for (int n = n0; n < n1; ++n) { d.ei_piece_off[n] = off; off += (cnt[n] + EI_PIECE - 1) / EI_PIECE; }
-  if (lane == 63) d.ei_piece_off[N] = (int32_t)incl;
+  const int64_t total = block_scan_nodes(
+      N, [&](int n) { return (int64_t)((cnt[n] + EI_PIECE - 1) / EI_PIECE); },
+      [&](int n, int64_t off) { d.ei_piece_off[n] = (int32_t)off; });
+  if (threadIdx.x == 0) d.ei_piece_off[N] = (int32_t)total;
 }
 
 // Per node: transmission incidence and the log rows.  With `advance`: when the
