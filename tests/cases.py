@@ -85,6 +85,17 @@ def make(case, seed=7, **over):
                                {"date": "2019-01-20", "nodes": [1], "age_range": (0, 5 * 365), "vaccinetype": "topv"}],
                  step_size_RI_ABM=14)
         comps = [lp.DiseaseState_ABM, lp.RI_ABM, lp.SIA_ABM, lp.Transmission_ABM]
+    elif case == "many":
+        # more nodes than the per-workgroup LDS tables hold (8200 > 8191): the list pass
+        # and the sampler look nodes up in global memory and round 1 is planned by k_plan
+        rng = np.random.default_rng(3)
+        n = 8200
+        xy = rng.uniform(0, 1000, (n, 2))
+        d = np.sqrt(((xy[:, None, :] - xy[None, :, :]) ** 2).sum(-1))
+        p = dict(dur=9, init_pop=rng.integers(20, 60, n), cbr=np.array([40.0]), r0=30,
+                 r0_scalars=rng.lognormal(0, 0.3, n), init_immun=0.2, init_prev=[5] * 50 + [0] * (n - 50),
+                 distances=d)
+        comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
     else:
         raise KeyError(case)
     base.update(p)

@@ -53,6 +53,18 @@ def test_fused_run_matches_oracle(case, use_graph):
 
 
 @pytest.mark.gpu
+def test_many_nodes_match_oracle():
+    """8200 nodes: node lookups past the LDS-staged tables, round 1 planned by k_plan."""
+    print("[parity] case=many graph=True", flush=True)
+    sim = make("many")
+    ref = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop).run()
+    assert ref.res["new_exposed"].sum() > 0
+    sim.run(use_graph=True)
+    assert sim.t == ref.ticks_run
+    _compare(sim, ref)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", ["small3", "sia"])
 def test_per_component_steps_match_oracle(case):
     """Stepping components one at a time (unfused entry points) gives the same day."""
