@@ -1958,14 +1958,20 @@ __device__ int64_t node_span_scan(int64_t* partials, int64_t* prefix, int s0, in
 __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount, int list) {
   const int32_t t = d.day[0];
   const int n = blockIdx.x;
+  const int tid = threadIdx.x;
+  const size_t N = d.n_nodes, row = (size_t)t * N;
+  // loads that do not depend on the span partials, issued with them
   const int sim = d.node_sim[n];
+  const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
+  const int64_t ea = (list && tid < NPART) ? d.node_acc[(size_t)n * NPART + tid] : 0;
+  const int64_t carry0 = tid < 3 ? d.node_carry[(size_t)n * 3 + tid] : 0;
+  const int32_t pop_prev = (tid == 0 && t >= 1) ? d.r_pop[row - N + n] : 0;
+  const int32_t births_n = tid == 0 ? d.births[n] : 0;
   if (sim_stopped(d, sim, t)) return;
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(d.day_flags[t], mask);
   __shared__ int64_t tally[NPART];
   __shared__ int64_t grp[WG / NPART][NPART];
-  const int tid = threadIdx.x;
-  const int s0 = d.node_span_begin[n], s1 = d.node_span_begin[n + 1];
   {
     // thread (f = tid % 32, g = tid / 32) sums field f of spans g, g+8, ...: coalesced
     const int f = tid % NPART, g = tid / NPART;
@@ -1988,7 +1994,6 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
     // list 1: k_ei alone (only the carried S weights come from the span partials);
     // list 2: k_ei and a FULL-day pass (both add up).  The accumulators are cleared
     // for the next day's k_ei.
-    const int64_t ea = list ? d.node_acc[(size_t)n * NPART + tid] : 0;
     if (list) d.node_acc[(size_t)n * NPART + tid] = 0;
     tally[tid] = list == 1 ? (tid == P_SW ? acc : ea) : acc + ea;
   }
@@ -1996,9 +2001,8 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   if (tid < 3) {
     // carried per-node counts: S, S with positive weight, R (the spans hold deltas)
     const int f = tid == 0 ? P_S : tid == 1 ? P_SPOS : P_R;
-    int64_t* cv = d.node_carry + (size_t)n * 3 + tid;
-    const int64_t v = recount ? tally[f] : *cv + tally[f];
-    *cv = v;
+    const int64_t v = recount ? tally[f] : carry0 + tally[f];
+    d.node_carry[(size_t)n * 3 + tid] = v;
     tally[f] = v;
   }
   __syncthreads();
@@ -2011,13 +2015,13 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
     if (tid == 0) d.W_round[n] = W;
   }
   if (tid != 0) return;
-  const size_t N = d.n_nodes, row = (size_t)t * N, K4 = d.K;
+  const size_t K4 = d.K;
   if (flags & LP_DAY_VD_STEP) {
-    d.r_births[row + n] = d.births[n];
+    d.r_births[row + n] = births_n;
     d.r_deaths[row + n] = (int32_t)tally[P_DEATHS];
-    d.r_pop[row + n] = d.r_pop[row - N + n] + d.births[n] - (int32_t)tally[P_DEATHS];
+    d.r_pop[row + n] = pop_prev + births_n - (int32_t)tally[P_DEATHS];
   } else if (flags & LP_DAY_VD_COPY) {
-    d.r_pop[row + n] = d.r_pop[row - N + n];
+    d.r_pop[row + n] = pop_prev;
   }
   if (flags & LP_DAY_DS) {
     d.r_newpp[row + n] = (int32_t)tally[P_NEWPP];
@@ -2135,12 +2139,45 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   const int ns = sp.n_nodes, li = i - sp.node_begin;
   const float* NT = d.network_t + sp.net_off + (size_t)li * ns;
   const float4* B = reinterpret_cast<const float4*>(d.beta_f) + sp.node_begin;
+  // the node's own inputs, loaded ahead of the mix
+  const int64_t* tl = d.node_tally + (size_t)i * NPART;
+  const int32_t pop_t = d.r_pop[(size_t)t * d.n_nodes + i];
+  const float season = d.season[(size_t)t * d.n_sims + sim];
+  const float keep = d.keep[i], r0s = d.r0s[i];
+  int64_t ibk[LP_MAX_STRAINS];
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) ibk[k] = k < K ? tl[P_IB0 + k] : 0;
+  const int64_t sw = tl[P_SW], scnt = tl[P_S], spos = tl[P_SPOS];
+  // lane l's partial of group v sums sources j = l + 64 v + 256 r for r = 0, 1, ...;
+  // the sources below MIX_PRE * 64 are loaded together first
+  constexpr int MIX_PRE = 16;
+  float ntv[MIX_PRE];
+  float4 bv[MIX_PRE];
+#pragma unroll
+  for (int m = 0; m < MIX_PRE; ++m) {
+    const int j = lane + 64 * m;
+    ntv[m] = j < ns ? NT[j] : 0.0f;
+    bv[m] = j < ns ? B[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
   float acc[4][LP_MAX_STRAINS];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
 #pragma unroll
     for (int k = 0; k < LP_MAX_STRAINS; ++k) acc[v][k] = 0.0f;
-    for (int j = lane + 64 * v; j < ns; j += 256) {
+  }
+#pragma unroll
+  for (int m = 0; m < MIX_PRE; ++m) {
+    const int v = m & 3;
+    if (lane + 64 * m < ns) {
+      acc[v][0] = __builtin_fmaf(ntv[m], bv[m].x, acc[v][0]);
+      acc[v][1] = __builtin_fmaf(ntv[m], bv[m].y, acc[v][1]);
+      acc[v][2] = __builtin_fmaf(ntv[m], bv[m].z, acc[v][2]);
+      acc[v][3] = __builtin_fmaf(ntv[m], bv[m].w, acc[v][3]);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    for (int j = lane + 64 * v + 64 * MIX_PRE; j < ns; j += 256) {
       const float nt = NT[j];
       const float4 b = B[j];
       acc[v][0] = __builtin_fmaf(nt, b.x, acc[v][0]);
@@ -2158,28 +2195,25 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
     part[k] = __shfl(x, 0, 64);
   }
   // node-level map, then the count draw with chunks spread over lanes
-  const int64_t* tl = d.node_tally + (size_t)i * NPART;
-  const int32_t pop_t = d.r_pop[(size_t)t * d.n_nodes + i];
   const float popc = (float)(pop_t > 1 ? pop_t : 1);
-  const float season = d.season[(size_t)t * d.n_sims + sim];
   float pk[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
   float ptot = 0.0f;
   bool local_zero = true;
 #pragma unroll
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
     if (k >= K) break;
-    const float bi = (float)tl[P_IB0 + k] * 5.9604644775390625e-08f;
-    const float bm = __builtin_fmaf(bi, d.keep[i], part[k]);
-    const float b2 = (bm * season) * d.r0s[i];
+    const float bi = (float)ibk[k] * 5.9604644775390625e-08f;
+    const float bm = __builtin_fmaf(bi, keep, part[k]);
+    const float b2 = (bm * season) * r0s;
     const float rate = b2 / popc;
     float p = -expm1f_lp(-rate);
     if (!(p > 0.0f)) p = 0.0f;
     pk[k] = p;
     ptot = ptot + p;
-    if (tl[P_IB0 + k] != 0) local_zero = false;
+    if (ibk[k] != 0) local_zero = false;
     if (lane == 0) { d.p_dbg[(size_t)i * K + k] = p; d.beta_mixed[(size_t)i * K + k] = bm; }
   }
-  const float expo = (float)tl[P_SW] * 5.9604644775390625e-08f;
+  const float expo = (float)sw * 5.9604644775390625e-08f;
   const float ex = expo * ptot;
   // mode 0: none, 1: Poisson(ex), 2: NB
   int mode = 0;
@@ -2242,9 +2276,8 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   if (lane != 0) return;
   d.expected_dbg[i] = ex;
   d.total_dbg[i] = total;
-  const int64_t scnt = tl[P_S];
   const int32_t sz = (int32_t)(total < scnt ? total : scnt);
-  const int32_t ta = (sz > 0 && (int64_t)sz >= tl[P_SPOS]) ? 1 : 0;
+  const int32_t ta = (sz > 0 && (int64_t)sz >= spos) ? 1 : 0;
   d.size[i] = sz;
   d.take_all[i] = ta;
   d.m_round[i] = ta ? 0 : sz;
@@ -2669,40 +2702,50 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
 
 __device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_adjust) {
   const int sim = d.node_sim[n];
-  if (sim_stopped(d, sim, t)) return;
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
-  if (!(flags & LP_DAY_TX)) return;
   const size_t N = d.n_nodes, row = (size_t)t * N + n;
   const int K = d.K;
   const int64_t* tl = d.node_tally + (size_t)n * NPART;
-  int32_t ne[LP_MAX_STRAINS] = {0, 0, 0, 0};
-  int32_t ne_tot = 0;
-  if (tx_adjust) {
-    for (int k = 0; k < K; ++k) { ne[k] = d.newexp[n * LP_MAX_STRAINS + k]; ne_tot += ne[k]; }
+  // every input up front (unconditional, in-bounds loads in flight together)
+  const int32_t stop = d.stop_day[sim];
+  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  int32_t ne[LP_MAX_STRAINS], ek[LP_MAX_STRAINS], ik[LP_MAX_STRAINS];
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+    ne[k] = (tx_adjust && k < K) ? d.newexp[n * LP_MAX_STRAINS + k] : 0;
+    ek[k] = k < K ? (int32_t)tl[P_E0 + k] : 0;
+    ik[k] = k < K ? (int32_t)tl[P_I0 + k] : 0;
   }
+  const int32_t s_t = (int32_t)tl[P_S], r_t = (int32_t)tl[P_R];
+  const int32_t potp = d.potp_run[n], pr = d.p_run[n];
+  if (stop < t || !(flags & LP_DAY_TX)) return;
+  int32_t ne_tot = 0;
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) ne_tot += ne[k];
   if ((mask & LP_STAGE_TRANSMISSION) && tx_adjust) {
     d.r_newexp[row] += ne_tot;
-    for (int k = 0; k < K; ++k) d.r_newexp_bs[row * K + k] += ne[k];
+#pragma unroll
+    for (int k = 0; k < LP_MAX_STRAINS; ++k)
+      if (k < K) d.r_newexp_bs[row * K + k] += ne[k];
     // every agent exposed by transmission left S and had a positive weight
     d.node_carry[(size_t)n * 3 + 0] -= ne_tot;
     d.node_carry[(size_t)n * 3 + 1] -= ne_tot;
   }
   if (mask & LP_STAGE_LOG) {
     int32_t e = 0, in = 0;
-    for (int k = 0; k < K; ++k) {
-      const int32_t ek = (int32_t)tl[P_E0 + k] + ne[k];
-      const int32_t ik = (int32_t)tl[P_I0 + k];
-      d.r_Ebs[row * K + k] = ek;
-      d.r_Ibs[row * K + k] = ik;
-      e += ek; in += ik;
+#pragma unroll
+    for (int k = 0; k < LP_MAX_STRAINS; ++k) {
+      if (k >= K) break;
+      d.r_Ebs[row * K + k] = ek[k] + ne[k];
+      d.r_Ibs[row * K + k] = ik[k];
+      e += ek[k] + ne[k]; in += ik[k];
     }
-    d.r_S[row] = (int32_t)tl[P_S] - ne_tot;
+    d.r_S[row] = s_t - ne_tot;
     d.r_E[row] = e;
     d.r_I[row] = in;
     if (e + in > 0) d.sim_cases[sim] = 1;   // for the next day's early-stop rule
-    d.r_R[row] += (int32_t)tl[P_R];
-    d.r_potp[row] = d.potp_run[n];
-    d.r_p[row] = d.p_run[n];
+    d.r_R[row] += r_t;
+    d.r_potp[row] = potp;
+    d.r_p[row] = pr;
   }
 }
 
