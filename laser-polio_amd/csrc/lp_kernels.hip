@@ -663,10 +663,12 @@ __device__ __forceinline__ void wave_append_entry(const Dev& d, int b, int node,
 // The entry of an agent exposed on day t with strain k, whose r0 scalar is rk (its
 // timers, potentially_paralyzed and infectivity are the record's, untouched while
 // it was S).
-__device__ __forceinline__ uint4 exposed_entry(const Dev& d, int64_t slot, int k, int32_t t, float rk) {
-  const uint4 r = d.rec[slot];
+__device__ __forceinline__ uint4 exposed_entry_rec(int64_t slot, const uint4& r, int k, int32_t t, float rk) {
   return entry_e((uint32_t)slot, k, r.x & 0xFFFFu, r.x >> 16, r.y & 0xFFFFu, (int8_t)(r.y >> 24), t + 1,
                  weight_q24(__uint_as_float(r.z) * rk));
+}
+__device__ __forceinline__ uint4 exposed_entry(const Dev& d, int64_t slot, int k, int32_t t, float rk) {
+  return exposed_entry_rec(slot, d.rec[slot], k, t, rk);
 }
 
 // NodeInfo of every node (one thread per node).
@@ -2330,15 +2332,14 @@ __device__ __forceinline__ int64_t ld64(const int64_t* p) {
   return ATOMIC_IO ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
 }
 
-// The agent hit by target T inside span s whose prefix (weight before it) is
-// sp_base: the chunk by a wave scan of the span's chunk weights, the agent by a
-// wave scan of the chunk's agent weights (MARKED_AS_S as for resolve_probe).
-// Returns the slot, or -1 if the weights were inconsistent.
+// The agent hit by target T inside the span of `ntiles` tiles from tile `tile0`,
+// whose prefix (weight before it) is sp_base: the chunk by a wave scan of the span's
+// chunk weights, the agent by a wave scan of the chunk's agent weights (MARKED_AS_S
+// as for resolve_probe).  Returns the slot, or -1 if the weights were inconsistent.
 template <bool ATOMIC_IO, bool MARKED_AS_S>
-__device__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane) {
-  const int4 span = d.spans[s];
-  const int64_t c0 = (int64_t)span.y * (LP_TILE / LP_CHUNK);
-  const int nc = span.z * (LP_TILE / LP_CHUNK);
+__device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int64_t sp_base, int64_t T, int lane) {
+  const int64_t c0 = tile0 * (LP_TILE / LP_CHUNK);
+  const int nc = ntiles * (LP_TILE / LP_CHUNK);
   const int64_t cw = lane < nc ? ld64<ATOMIC_IO>(d.chunk_sum + c0 + lane) : 0;
   const int64_t cincl = wave_incl_scan_i64(cw, lane);
   const uint64_t cb = __ballot(lane < nc && sp_base + cincl > T);
@@ -2370,6 +2371,13 @@ __device__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int
   const int L = __ffsll((unsigned long long)hb) - 1;
   const int ha = __shfl(hit, L, 64);
   return c * LP_CHUNK + (int64_t)L * 4 + ha;
+}
+
+// resolve_in_tiles over span s.
+template <bool ATOMIC_IO, bool MARKED_AS_S>
+__device__ __forceinline__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane) {
+  const int4 span = d.spans[s];
+  return resolve_in_tiles<ATOMIC_IO, MARKED_AS_S>(d, span.y, span.z, sp_base, T, lane);
 }
 
 
@@ -2418,8 +2426,9 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
   unweight_in(d, span_of(d, node, slot), slot, weight_q24(d.risk[slot]));
 }
 
-// S -> E of a selected agent with its strain drawn against the node's cumulative
-// strain thresholds thr[0..K) (already loaded); returns the strain.
+// S -> E of a selected agent with its strain drawn by cumulative strain probability
+// (model.py:1125-1147), against the node's thresholds thr[0..K) (already loaded);
+// returns the strain (the caller counts it).
 __device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* thr, uint64_t seed, int64_t slot_base,
                                                 int64_t slot, int32_t t) {
   const uint4 o = draw(seed, (uint64_t)(slot - slot_base), t, ST_STRAIN);
@@ -2427,21 +2436,6 @@ __device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* th
 #pragma unroll
   for (int s = LP_MAX_STRAINS - 1; s >= 0; --s)
     if (s < d.K && (uint64_t)o.x < thr[s]) k = s;   // the first strain whose threshold exceeds the draw
-  d.state[slot] = 1;
-  *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
-  *rec_byte(d, slot, RB_STATE) = 1;
-  return k;
-}
-
-// S -> E of a selected agent with its strain drawn by cumulative strain
-// probability (model.py:1125-1147); returns the strain (the caller counts it).
-__device__ __forceinline__ int expose_agent_nocount(const Dev& d, int node, uint64_t seed, int64_t slot_base,
-                                                    int64_t slot, int32_t t) {
-  const uint4 o = draw(seed, (uint64_t)(slot - slot_base), t, ST_STRAIN);
-  int k = 0;
-  for (int s = 0; s < d.K; ++s) {
-    if ((uint64_t)o.x < d.thr_strain[(size_t)node * LP_MAX_STRAINS + s]) { k = s; break; }
-  }
   d.state[slot] = 1;
   *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
   *rec_byte(d, slot, RB_STATE) = 1;
@@ -2493,7 +2487,7 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int64_t per = (M + nw - 1) / nw;
   const int64_t p0 = (first + (threadIdx.x >> 6)) * per;
   const int64_t p1 = p0 + per < M ? p0 + per : M;
-  int node = -1, ns = 0;
+  int node = -1, ns = 0, stile = 0, sntile = 0;   // lane l: span l of the node (first tile, tiles)
   int64_t node_lo = 0, node_hi = 0, W = 0, s0 = 0, spx = INT64_MAX;
   uint64_t seed = 0;
   uint32_t lnode = 0;
@@ -2508,7 +2502,11 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
       W = d.W_round[node];
       s0 = ni.span_begin;
       ns = ni.n_spans;
-      spx = (ns <= 64 && lane < ns) ? d.span_prefix[s0 + lane] : INT64_MAX;
+      const bool mine = ns <= 64 && lane < ns;
+      spx = mine ? d.span_prefix[s0 + lane] : INT64_MAX;
+      const int4 sp4 = mine ? d.spans[s0 + lane] : make_int4(0, 0, 0, 0);
+      stile = sp4.y;
+      sntile = sp4.z;
     }
     const int64_t j = p - node_lo;
     const uint4 o = philox((uint32_t)j, lnode, (uint32_t)t, ST_PROBE | (1u << 8), seed);
@@ -2520,7 +2518,8 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
         const uint64_t ok = __ballot(spx <= T);      // monotone over the node's spans
         if (ok) {
           const int L = 63 - __clzll((unsigned long long)ok);
-          slot = resolve_in_span<false, true>(d, s0 + L, __shfl(spx, L, 64), T, lane);
+          slot = resolve_in_tiles<false, true>(d, __shfl(stile, L, 64), __shfl(sntile, L, 64), __shfl(spx, L, 64), T,
+                                               lane);
         }
       } else {
         slot = resolve_probe<false, true>(d, node, T, d.span_prefix, lane);
@@ -2547,6 +2546,8 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
     const int64_t p = p0 + lane;
     int node = -1, k = -1;
     int64_t slot = 0;
+    uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+    float r0v[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (p < M) {
       const uint64_t e = d.sel_list[base + p];
       if (e != SEL_INVALID) {
@@ -2562,7 +2563,12 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           const int64_t sp_i = ni.span_begin + (slot / LP_TILE - lo / LP_TILE) / d.tpw;
           uint64_t thr[LP_MAX_STRAINS];
 #pragma unroll
-          for (int q = 0; q < LP_MAX_STRAINS; ++q) thr[q] = d.thr_strain[(size_t)nd * LP_MAX_STRAINS + q];
+          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+            thr[q] = d.thr_strain[(size_t)nd * LP_MAX_STRAINS + q];
+            r0v[q] = ni.r0k[q];
+          }
+          // the record fields of the list entry do not change when an agent is exposed
+          rec = d.rec[slot];
           if (slot >= hi || slot < lo) {
             atomicOr(d.err, ERR_SAMPLING);
           } else if (claim(d.state, slot)) {         // the first probe of an agent claims it
@@ -2584,14 +2590,13 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
         if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
-        wave_append_entry(d, nxt, n0,
-                          node >= 0 ? exposed_entry(d, slot, k, t, d.ninfo[n0].r0k[k]) : make_uint4(0u, 0u, 0u, 0u),
-                          node >= 0, lane);
+        const uint4 ent = node >= 0 ? exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)) : make_uint4(0u, 0u, 0u, 0u);
+        wave_append_entry(d, nxt, n0, ent, node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
       atomicAdd(d.n_uniq + node, 1);
-      append_entry(d, nxt, node, exposed_entry(d, slot, k, t, d.ninfo[node].r0k[k]));
+      append_entry(d, nxt, node, exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)));
     }
   }
 }
@@ -2613,6 +2618,13 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const NodeInfo& ni = d.ninfo[n];
   if (tid < LP_MAX_STRAINS) s_ne[tid] = 0;
   if (tid == 0) s_claimed = 0;
+  uint64_t thr[LP_MAX_STRAINS];   // the node's cumulative strain thresholds, loaded once
+  float r0v[LP_MAX_STRAINS];
+#pragma unroll
+  for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+    thr[q] = d.thr_strain[(size_t)n * LP_MAX_STRAINS + q];
+    r0v[q] = ni.r0k[q];
+  }
   __syncthreads();
   const int nxt = (t + 1) & 1;
   if (take_all) {
@@ -2621,14 +2633,16 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       const int64_t i = i0 + tid;
       bool ex = false;
       int kx = 0;
+      uint4 rec = make_uint4(0u, 0u, 0u, 0u);
       if (i < hi && d.state[i] == 0 && weight_q24(d.risk[i]) > 0) {
+        rec = d.rec[i];
         unweight(d, n, i);
-        kx = expose_agent_nocount(d, n, ni.seed, ni.slot_base, i, t);
+        kx = expose_agent_thr(d, thr, ni.seed, ni.slot_base, i, t);
         atomicAdd(&s_ne[kx], 1);
         ex = true;
       }
-      wave_append_entry(d, nxt, n, ex ? exposed_entry(d, i, kx, t, r0_sel(ni.r0k, kx)) : make_uint4(0u, 0u, 0u, 0u), ex,
-                        lane);
+      wave_append_entry(d, nxt, n, ex ? exposed_entry_rec(i, rec, kx, t, r0_sel(r0v, kx)) : make_uint4(0u, 0u, 0u, 0u),
+                        ex, lane);
     }
   } else {
     const int s0 = ni.span_begin, s1 = ni.span_begin + ni.n_spans;
@@ -2684,11 +2698,13 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
         const bool ex = e != SEL_INVALID;
         int kx = 0;
+        uint4 rec = make_uint4(0u, 0u, 0u, 0u);
         if (ex) {
-          kx = expose_agent_nocount(d, n, ni.seed, ni.slot_base, slot, t);
+          rec = d.rec[slot];
+          kx = expose_agent_thr(d, thr, ni.seed, ni.slot_base, slot, t);
           atomicAdd(&s_ne[kx], 1);
         }
-        wave_append_entry(d, nxt, n, ex ? exposed_entry(d, slot, kx, t, r0_sel(ni.r0k, kx)) : make_uint4(0u, 0u, 0u, 0u),
+        wave_append_entry(d, nxt, n, ex ? exposed_entry_rec(slot, rec, kx, t, r0_sel(r0v, kx)) : make_uint4(0u, 0u, 0u, 0u),
                           ex, lane);
       }
       __threadfence();
