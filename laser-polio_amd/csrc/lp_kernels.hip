@@ -349,13 +349,13 @@ __device__ __forceinline__ bool progress_agent(int a, int8_t& s, int strain, uin
 //   x: exposure_timer (bits 0-15) | infection_timer (16-31)
 //   y: paralysis_timer (0-15) | strain (16-23) | potentially_paralyzed (24-31)
 //   z: daily_infectivity (float bits)
-//   w: ipv_protected (0-7) | paralyzed (8-15) | disease_state (16-23, a copy
-//      kept exact for E and I agents: every writer of their state writes both)
+//   w: ipv_protected (0-7) | paralyzed (8-15) | unused (16-31; the disease state
+//      lives in its column and, for E and I agents, in their list entries)
 // k_pack fills the records from the bound columns after a RECOUNT pass and
 // k_unpack writes them back (lp_sync); in between the records are the state of
 // these fields.  Single fields are written with byte stores (RB_*), so writers
 // of different fields of one record never overwrite each other.
-enum : int { RB_STRAIN = 6, RB_PP = 7, RB_IPV = 12, RB_PARALYZED = 13, RB_STATE = 14 };
+enum : int { RB_STRAIN = 6, RB_PP = 7, RB_IPV = 12, RB_PARALYZED = 13 };
 
 __device__ __forceinline__ int8_t* rec_byte(const Dev& d, int64_t i, int b) {
   return reinterpret_cast<int8_t*>(d.rec + i) + b;
@@ -611,7 +611,6 @@ __device__ __forceinline__ uint32_t ei_quad(const Dev& d, int64_t i0, int64_t li
     if (r[a].x != r0.x) rw[0] = r[a].x;
     if (r[a].y != r0.y) rw[1] = r[a].y;
     if (to_par) *rec_byte(d, i, RB_PARALYZED) = 1;
-    if (sa != byte_of(wei, a)) *rec_byte(d, i, RB_STATE) = sa;
     w = (w & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)sa << (8 * a));
     if (sa == 1 || sa == 2) {                            // into tomorrow's list, timers as of tomorrow
       surv |= 1u << a;
@@ -703,8 +702,7 @@ __global__ void __launch_bounds__(WG) k_pack(Dev d) {
                             (uint32_t)d.par_timer[i] | ((uint32_t)(uint8_t)d.strain[i] << 16) |
                                 ((uint32_t)(uint8_t)d.pp[i] << 24),
                             __float_as_uint(d.infect[i]),
-                            (uint32_t)(uint8_t)d.ipv[i] | ((uint32_t)(uint8_t)d.paralyzed[i] << 8) |
-                                ((uint32_t)(uint8_t)d.state[i] << 16));
+                            (uint32_t)(uint8_t)d.ipv[i] | ((uint32_t)(uint8_t)d.paralyzed[i] << 8));
     }
   }
 }
@@ -970,7 +968,7 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
           if (s0) hw[2] = (uint16_t)timer_left(p_day, t + 1);   // stops after today's count-down
         }
         if (to_par) *rec_byte(d, slot, RB_PARALYZED) = 1;
-        if (sa != s_in) { d.state[slot] = (int8_t)sa; *rec_byte(d, slot, RB_STATE) = (int8_t)sa; }
+        if (sa != s_in) d.state[slot] = (int8_t)sa;
         if (sa == 2) e = make_uint4(slot, ent_flags(2, k, pend, p_day), (uint32_t)i_end, e.w);
       }
       if (sa != 1 && sa != 2) continue;
@@ -1124,7 +1122,6 @@ __global__ void __launch_bounds__(WG) k_ri_list(Dev d, int mask) {
           unweight_in(d, span_of(d, n, slot), slot, w);
           d.state[slot] = 1;
           *rec_byte(d, slot, RB_STRAIN) = (int8_t)rk;
-          *rec_byte(d, slot, RB_STATE) = 1;
           c_rip += 1;
           c_spos += (w > 0);
           expo |= 1u << q;
@@ -1429,7 +1426,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                   atomicAdd(&lc[L_RIV], 1);
                   if (s == 0) {
                     s = 1; k = sp.ri_strain;
-                    *rec_byte(d, i, RB_STRAIN) = (int8_t)k; *rec_byte(d, i, RB_STATE) = 1;
+                    *rec_byte(d, i, RB_STRAIN) = (int8_t)k;
                     atomicAdd(&lc[L_RIP], 1);
                   }
                 }
@@ -1451,7 +1448,6 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
                     s = 1;
                     k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
                     *rec_byte(d, i, RB_STRAIN) = (int8_t)k;
-                    *rec_byte(d, i, RB_STATE) = 1;
                     atomicAdd(&lc[L_SIAPK0 + k], 1);
                     if (last) atomicAdd(&lc[L_SIAP], 1);
                   }
@@ -2438,7 +2434,6 @@ __device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* th
     if (s < d.K && (uint64_t)o.x < thr[s]) k = s;   // the first strain whose threshold exceeds the draw
   d.state[slot] = 1;
   *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
-  *rec_byte(d, slot, RB_STATE) = 1;
   return k;
 }
 
