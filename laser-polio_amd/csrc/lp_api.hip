@@ -461,7 +461,7 @@ int lp_prepare(lp_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(nspan_d, nspan.data(), (N + 1) * 4, hipMemcpyHostToDevice, c->stream));
   d.spans = spans_d; d.node_span_begin = nspan_d;
   const int64_t nch = g.capacity / LP_CHUNK + 1;
-  if ((rc = alloc(c, &d.day, 4))) return rc;
+  if ((rc = alloc(c, &d.day, 8))) return rc;   // t, until, -, -, then the day record (16-byte aligned)
   if ((rc = alloc(c, &d.stop_day, S))) return rc;
   if ((rc = alloc(c, &d.err, 4))) return rc;
   if ((rc = alloc(c, &d.partials, (size_t)c->n_spans * lp::NPART))) return rc;

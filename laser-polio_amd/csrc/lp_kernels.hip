@@ -90,7 +90,8 @@ struct Dev {
           *r_births, *r_deaths, *r_newexp, *r_newexp_bs, *r_riv, *r_rip, *r_ri_newexp_bs, *r_ipv,
           *r_siav, *r_siap, *r_sia_newexp_bs;
   // scratch
-  int32_t* day; int32_t* stop_day; int32_t* err;
+  int32_t* day;   // [8] tick, end of the lp_run_days call, unused x2, the day record (today())
+  int32_t* stop_day; int32_t* err;
   const int4* spans; const int32_t* node_span_begin;
   int64_t* partials; int64_t* chunk_sum;
   int64_t* nb_begin; int32_t* births;
@@ -119,6 +120,22 @@ struct Dev {
 
 __device__ __forceinline__ bool sim_stopped(const Dev& d, int sim, int t) { return d.stop_day[sim] < t; }
 
+// The current day's record at day + 4: {t, day_flags[t], ri_cnt[t], ri_cnt[t + 1]}, so
+// a kernel reads its day, flags and RI counts with one 16-byte load instead of a
+// load that depends on the day counter.  Written where the day is set (k_set_day)
+// and where it advances (k_day_end).
+__device__ __forceinline__ int4 today(const Dev& d) { return *reinterpret_cast<const int4*>(d.day + 4); }
+__device__ __forceinline__ int4 day_record(const Dev& d, int32_t t) {
+  const int32_t f = t < d.nt ? d.day_flags[t] : 0;
+  const int32_t r0 = t <= d.nt ? d.ri_cnt[t] : 0;          // ri_cnt has nt + 1 entries
+  const int32_t r1 = t + 1 <= d.nt ? d.ri_cnt[t + 1] : r0;
+  return make_int4(t, f, r0, r1);
+}
+__device__ __forceinline__ void set_today(const Dev& d, const int4& rec) {
+  d.day[0] = rec.x;
+  *reinterpret_cast<int4*>(d.day + 4) = rec;
+}
+
 __device__ __forceinline__ int32_t stage_day_flags(int32_t flags, int mask) {
   int32_t f = 0;
   if (mask & LP_STAGE_VITAL) f |= flags & (LP_DAY_VD_STEP | LP_DAY_VD_COPY);
@@ -132,7 +149,10 @@ __device__ __forceinline__ int32_t stage_day_flags(int32_t flags, int mask) {
 // ---------------------------------------------------------------------------
 // day[0] = current tick; day[1] = end of the current lp_run_days call (0 outside one).
 __global__ void k_set_day(Dev d, int32_t t, int32_t until) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) { d.day[0] = t; d.day[1] = until; }
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    set_today(d, day_record(d, t));
+    d.day[1] = until;
+  }
 }
 
 // Kaplan-Meier style age at death (days) for an agent of `age` days, from a
@@ -783,11 +803,12 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
     const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
     st[u] = (u < span.z) ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
   });
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int node = span.x, sim = span.w;
   const int64_t live_hi = d.live_end[node];
   const int32_t stop_day = d.stop_day[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   if (stop_day < t) return;
   if (!is_plain_day(flags)) {
     if (tid < NPART && tid != P_SW) d.partials[(size_t)blockIdx.x * NPART + tid] = 0;
@@ -869,12 +890,13 @@ __device__ __forceinline__ int find_piece_node(const int32_t* off, int N, int32_
 __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
   // the piece offsets in LDS (when they fit) for the node lookup of each piece
   __shared__ int32_t s_off[EI_LDS_NODES + 1];
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int cur = t & 1, nxt = cur ^ 1;
   const int N = d.n_nodes, K = d.K;
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * (WG / 64) + (threadIdx.x >> 6), nw = gridDim.x * (WG / 64);
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   const bool do_ds = flags & LP_DAY_DS;
   const bool do_tally = (flags & LP_DAY_TX) != 0;
   const bool vd = flags & LP_DAY_VD_STEP;   // agents due to die today leave before DiseaseState
@@ -1058,22 +1080,23 @@ __global__ void __launch_bounds__(WG) k_ei_actual(Dev d, int b, int32_t t_next) 
 // to node_acc (k_node_reduce adds them to the node tallies) and to the chunk /
 // span weights.  Runs after k_ei (DiseaseState reads ipv_protected before RI).
 __global__ void __launch_bounds__(WG) k_ri_list(Dev d, int mask) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int n = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63;
   const int sim = d.node_sim[n];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   if (sim_stopped(d, sim, t) || !(flags & LP_DAY_RI)) return;
   if (n == 0 && blockIdx.y == 0 && tid == 0) d.ri_ref[1] = t;
   const lp_sim_params& sp = d.sims[sim];
   const int32_t step = sp.ri_step;
-  const int32_t sh_today = step * (d.ri_cnt[t + 1] - d.ri_cnt[d.ri_ref[0] + 1]);
+  const int32_t sh_today = step * (dy.w - d.ri_cnt[d.ri_ref[0] + 1]);
   const uint64_t thr_ri = d.thr_ri[n], thr_ipv = d.thr_ipv[n];
   const uint64_t seed = sp.seed;
   const int64_t sbase = sp.slot_base;
   const int rk = sp.ri_strain & (LP_MAX_STRAINS - 1);
   const bool do_tally = (flags & LP_DAY_TX) != 0;
-  const int A = d.ri_cnt[t] & 1, B = A ^ 1, nxt = (t + 1) & 1;
+  const int A = dy.z & 1, B = A ^ 1, nxt = (t + 1) & 1;
   const int32_t cnt = *ri_cnt_of(d, A, n);
   const uint32_t* L = ri_list_of(d, A, n);
   uint32_t* LB = ri_list_of(d, B, n);
@@ -1210,11 +1233,12 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   // earliest death date of each of the span's chunks (<= 64): lane l holds chunk l
   const int32_t cm_all = (VD && lane < span0.z * (LP_TILE / LP_CHUNK))
                              ? d.chunk_min_dod[(int64_t)span0.y * (LP_TILE / LP_CHUNK) + lane] : INT_MAX;
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int node = span0.x, sim = span0.w;
   const int64_t live_hi = d.live_end[node];
   const int32_t stop_day = d.stop_day[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   if (stop_day < t) return;
   const int want = ((flags & LP_DAY_VD_STEP) ? FULL_VD : 0) | ((flags & LP_DAY_RI) ? FULL_RI : 0) |
                    ((flags & LP_DAY_SIA) ? FULL_SIA : 0);
@@ -1253,8 +1277,8 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   // ri_ref[0]: RI's decrements since then are applied by arithmetic (sh_prev: RI days
   // before today, for births and deaths; sh_today: through today, for RI)
   const int32_t cnt_ref = d.ri_cnt[d.ri_ref[0] + 1];
-  const int32_t sh_prev = sp.ri_step * (d.ri_cnt[t] - cnt_ref);
-  const int32_t sh_today = sp.ri_step * (d.ri_cnt[t + 1] - cnt_ref);
+  const int32_t sh_prev = sp.ri_step * (dy.z - cnt_ref);
+  const int32_t sh_today = sp.ri_step * (dy.w - cnt_ref);
   if (RI && blockIdx.x == 0 && tid == 0) d.ri_ref[1] = t;
   // does any of today's campaigns target this node?  (uniform over the workgroup)
   bool sia_here = false;
@@ -1471,7 +1495,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
       if (!VD) w1 = w0;
       if (u < ng) s_sv[g0 + u][tid] = (uint8_t)cnew;
       if (RI) {                                           // candidates for later RI days
-        const int b = (d.ri_cnt[t + 1]) & 1;
+        const int b = (dy.w) & 1;
         wave_append_quad_to(ri_list_of(d, b, node), ri_cnt_of(d, b, node), i0, u < ng ? rcand : 0u, lane);
       }
       m[u] = __ballot(q && need_ei);
@@ -1559,7 +1583,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
     const int64_t lo = nb_lo > s_lo ? nb_lo : s_lo, hi = live_hi < s_hi ? live_hi : s_hi;
     if (!RI && hi > lo) {
       __shared__ int32_t s_base;
-      const int b = d.ri_cnt[t] & 1;
+      const int b = dy.z & 1;
       if (tid == 0) s_base = atomicAdd(ri_cnt_of(d, b, node), (int32_t)(hi - lo));
       __syncthreads();
       uint32_t* L = ri_list_of(d, b, node);
@@ -1584,11 +1608,12 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
   constexpr bool RECOUNT = KIND == KIND_RECOUNT;
   const int4 span0 = d.spans[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int node = span0.x, sim = span0.w;
   const int64_t live_hi = d.live_end[node];
   const int32_t stop_day = d.stop_day[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   if (stop_day < t) return;
   if (!RECOUNT && is_plain_day(flags)) {
     if (tid < NPART && tid != P_SW) d.partials[(size_t)blockIdx.x * NPART + tid] = 0;
@@ -1893,7 +1918,7 @@ __global__ void __launch_bounds__(WG) k_agents(Dev d, int mask) {
             if (i0 + a < live_hi && byte_of(wf, a) >= 0 && byte_of(mm, a) != 1 && tm > -sp.ri_step) rb |= 1u << a;
           }
         }
-        const int rbuf = d.ri_cnt[t + 1] & 1;
+        const int rbuf = dy.w & 1;
         wave_append_quad_to(ri_list_of(d, rbuf, node), ri_cnt_of(d, rbuf, node), i0, rb, lane);
       }
       // chunk weight sums: rewritten when recounting, adjusted by their delta otherwise
@@ -1954,7 +1979,8 @@ __device__ int64_t node_span_scan(int64_t* partials, int64_t* prefix, int s0, in
 
 // Per node: reduce span partials, span prefix, results rows of VD / DS / RI / SIA.
 __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount, int list) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int n = blockIdx.x;
   const int tid = threadIdx.x;
   const size_t N = d.n_nodes, row = (size_t)t * N;
@@ -1967,7 +1993,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   const int32_t births_n = tid == 0 ? d.births[n] : 0;
   if (sim_stopped(d, sim, t)) return;
   const lp_sim_params& sp = d.sims[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(dy.y, mask);
   __shared__ int64_t tally[NPART];
   __shared__ int64_t grp[WG / NPART][NPART];
   {
@@ -2005,7 +2031,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   }
   __syncthreads();
   if (tid < NPART) d.node_tally[(size_t)n * NPART + tid] = tally[tid];
-  if (tid == 0 && (flags & LP_DAY_RI)) *ri_cnt_of(d, d.ri_cnt[t] & 1, n) = 0;   // consumed by today's RI
+  if (tid == 0 && (flags & LP_DAY_RI)) *ri_cnt_of(d, dy.z & 1, n) = 0;   // consumed by today's RI
   if (tid < LP_MAX_STRAINS)   // infectious weight per strain as the network mix reads it
     d.beta_f[(size_t)n * LP_MAX_STRAINS + tid] = tid < d.K ? (float)tally[P_IB0 + tid] * 5.9604644775390625e-08f : 0.0f;
   if (flags & LP_DAY_TX) {
@@ -2290,13 +2316,14 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
 
 // Per node: network mix, FOI, count draw.  One wavefront per node.
 __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int i = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i < d.n_nodes) {
     const int sim = d.node_sim[i];
     const lp_sim_params& sp = d.sims[sim];
-    const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+    const int32_t flags = stage_day_flags(dy.y, mask);
     if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
       if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
     } else {
@@ -2473,7 +2500,8 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int64_t first = (int64_t)blockIdx.x * (WG / 64);
   if (first >= M) return;
   const int64_t* off = lds ? s_off : d.probe_off;
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int lane = threadIdx.x & 63;
   const int64_t base = lds ? 0 : d.round_base[1];
   // each wave takes a contiguous range of probes (probes are ordered by node); the
@@ -2531,12 +2559,16 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
 // it (the round's distinct agents, counted per node in n_uniq); a claimed agent
 // loses its weight and is exposed.
 __global__ void __launch_bounds__(WG) k_commit(Dev d) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int nxt = (t + 1) & 1;
   const int64_t M = d.probe_off[d.n_nodes];
   const int64_t base = d.round_base[1];
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * WG;
+  // round 1 is the day's first reservation (base 0): the first entry is loaded with M
+  const int64_t p_first = (int64_t)blockIdx.x * WG + threadIdx.x;
+  const uint64_t e_first = p_first < d.sel_cap ? d.sel_list[p_first] : SEL_INVALID;
   for (int64_t p0 = (int64_t)blockIdx.x * WG + (threadIdx.x & ~63); p0 < M; p0 += stride) {
     const int64_t p = p0 + lane;
     int node = -1, k = -1;
@@ -2544,7 +2576,7 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
     uint4 rec = make_uint4(0u, 0u, 0u, 0u);
     float r0v[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (p < M) {
-      const uint64_t e = d.sel_list[base + p];
+      const uint64_t e = (p == p_first && base == 0) ? e_first : d.sel_list[base + p];
       if (e != SEL_INVALID) {
         slot = (int64_t)(e & ((1ull << 40) - 1));
         const int nd = (int)(e >> 40);
@@ -2600,7 +2632,8 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
 // such agent; otherwise the node's later rounds (2, 3, ...) run here until
 // `size` distinct agents are exposed or no weight is left.
 __global__ void __launch_bounds__(WG) k_finish(Dev d) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const int n = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int64_t sh_base;
@@ -2711,14 +2744,14 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   if (tid == 0 && s_claimed) atomicAdd(d.n_uniq + n, s_claimed);
 }
 
-__device__ void day_end_node(const Dev& d, int n, int32_t t, int mask, int tx_adjust) {
+__device__ void day_end_node(const Dev& d, int n, int32_t t, int32_t day_flags_t, int mask, int tx_adjust) {
   const int sim = d.node_sim[n];
   const size_t N = d.n_nodes, row = (size_t)t * N + n;
   const int K = d.K;
   const int64_t* tl = d.node_tally + (size_t)n * NPART;
   // every input up front (unconditional, in-bounds loads in flight together)
   const int32_t stop = d.stop_day[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  const int32_t flags = stage_day_flags(day_flags_t, mask);
   int32_t ne[LP_MAX_STRAINS], ek[LP_MAX_STRAINS], ik[LP_MAX_STRAINS];
 #pragma unroll
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
@@ -2777,11 +2810,14 @@ __device__ void ei_plan(const Dev& d, int b) {
 // early-stop rule for it; the last workgroup plans the next day's E / I list
 // pieces and moves the day counter.
 __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, int advance) {
-  const int32_t t = d.day[0];
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
   const bool begin_next = advance && t + 1 < d.day[1];
+  // the next day's record, loaded early (thread 0 of the last workgroup stores it)
+  const int4 next = (advance && threadIdx.x == 0) ? day_record(d, t + 1) : make_int4(0, 0, 0, 0);
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < d.n_nodes) {
-    day_end_node(d, n, t, mask, tx_adjust);
+    day_end_node(d, n, t, dy.y, mask, tx_adjust);
     if (begin_next) day_begin_node(d, n, t + 1, mask);
   }
   if (advance && last_block(d.ticket + 2)) {
@@ -2791,7 +2827,7 @@ __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, 
       d.sim_cases[sim] = 0;
     }
     if (begin_next && threadIdx.x == 0) d.sel_count[0] = 0;
-    if (threadIdx.x == 0) d.day[0] = t + 1;
+    if (threadIdx.x == 0) set_today(d, next);
   }
 }
 
