@@ -2630,7 +2630,9 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
 
 // Per node: take-all nodes (size >= #positive-weight susceptibles) expose every
 // such agent; otherwise the node's later rounds (2, 3, ...) run here until
-// `size` distinct agents are exposed or no weight is left.
+// `size` distinct agents are exposed or no weight is left.  A round's probes are
+// kept in LDS up to FIN_LDS of them, else in the selected list.
+constexpr int FIN_LDS = 2048;
 __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int4 dy = today(d);
   const int32_t t = dy.x;
@@ -2638,6 +2640,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int64_t sh_base;
   __shared__ int64_t sh_W;
+  __shared__ uint64_t s_sel[FIN_LDS];        // a round's selected entries when they fit
   __shared__ int32_t s_ne[LP_MAX_STRAINS];   // the node's new exposures by strain (one global add each)
   __shared__ int32_t s_claimed;
   // most nodes have nothing left after round 1: one round of loads decides
@@ -2680,17 +2683,25 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
       if (need <= 0) break;
       const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
       if (W <= 0) break;
+      // a round of up to FIN_LDS probes keeps them in LDS; a larger one reserves space in
+      // the selected list (uniform over the workgroup)
+      const bool in_lds = need <= FIN_LDS;
       if (tid == 0) {
-        const int64_t b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
-        if (b + need > d.sel_cap) {
-          atomicOr(d.err, ERR_OVERFLOW);
-          sh_base = -1;
-        } else sh_base = b;
+        int64_t b = 0;
+        if (!in_lds) {
+          b = atomicAdd(reinterpret_cast<unsigned long long*>(d.sel_count), (unsigned long long)need);
+          if (b + need > d.sel_cap) {
+            atomicOr(d.err, ERR_OVERFLOW);
+            b = -1;
+          }
+        }
+        sh_base = b;
         sh_W = W;
       }
       __syncthreads();
       const int64_t base = sh_base;
       if (base < 0) break;
+      uint64_t* gsel = d.sel_list + base;
       for (int j = wave; j < need; j += WG / 64) {
         const uint4 o = philox((uint32_t)j, (uint32_t)ni.lnode, (uint32_t)t,
                                ST_PROBE | ((uint32_t)round << 8), ni.seed);
@@ -2700,29 +2711,32 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         if (lane == 0) {
           const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
           if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-          __hip_atomic_store(d.sel_list + base + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (in_lds) s_sel[j] = e;
+          else __hip_atomic_store(gsel + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      __threadfence();
+      if (!in_lds) __threadfence();
       __syncthreads();
       for (int j = tid; j < need; j += WG) {
-        const uint64_t e = __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t e = in_lds ? s_sel[j] : __hip_atomic_load(gsel + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (e == SEL_INVALID) continue;
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
         if (claim(d.state, slot)) {
           unweight(d, n, slot);
           atomicAdd(&s_claimed, 1);
+        } else if (in_lds) {
+          s_sel[j] = SEL_INVALID;
         } else {
-          __hip_atomic_store(d.sel_list + base + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(gsel + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      __threadfence();
+      __threadfence();   // the claims and weight updates before the exposures and the next round
       __syncthreads();
       for (int j0 = 0; j0 < need; j0 += WG) {          // uniform trip count: whole waves append
         const int j = j0 + tid;
-        const uint64_t e = j < need ? __hip_atomic_load(d.sel_list + base + j, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT)
-                                    : SEL_INVALID;
+        const uint64_t e = j >= need ? SEL_INVALID
+                           : in_lds ? s_sel[j]
+                                    : __hip_atomic_load(gsel + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int64_t slot = (int64_t)(e & ((1ull << 40) - 1));
         const bool ex = e != SEL_INVALID;
         int kx = 0;

@@ -52,6 +52,13 @@ def make(case, seed=7, **over):
                  r0_scalars=np.array([1.0, 1.5, 2.0]), init_immun=0.2, init_prev=[30, 10, 2],
                  distances=np.array([[0, 1, 2], [1, 0, 1], [2, 1, 0]]), dur_exp=lp.constant(2), dur_inf=lp.constant(3))
         comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
+    elif case == "hotbig":
+        # huge R0 on large nodes: later sampling rounds of more probes than k_finish keeps
+        # in LDS (its selected-list path)
+        p = dict(dur=6, init_pop=np.array([60000, 30000]), cbr=np.array([30.0]), r0=400,
+                 r0_scalars=np.array([1.0, 1.5]), init_immun=0.1, init_prev=[400, 100],
+                 distances=np.array([[0, 1], [1, 0]]), dur_exp=lp.constant(2), dur_inf=lp.constant(3))
+        comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
     elif case == "zinb":
         # infection only in node 0; other nodes get importations through the zero-inflated NB branch
         p = dict(dur=80, init_pop=np.array([40000, 30000, 30000, 20000]), cbr=None, r0=12,
@@ -105,4 +112,4 @@ def make(case, seed=7, **over):
     return sim
 
 
-CASES = ["small3", "sia", "c0mini", "hot", "zinb", "stop", "campaigns", "ri_only"]
+CASES = ["small3", "sia", "c0mini", "hot", "hotbig", "zinb", "stop", "campaigns", "ri_only"]
