@@ -2639,7 +2639,6 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int n = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int64_t sh_base;
-  __shared__ int64_t sh_W;
   __shared__ uint64_t s_sel[FIN_LDS];        // a round's selected entries when they fit
   __shared__ int32_t s_ne[LP_MAX_STRAINS];   // the node's new exposures by strain (one global add each)
   __shared__ int32_t s_claimed;
@@ -2677,11 +2676,26 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
     }
   } else {
     const int s0 = ni.span_begin, s1 = ni.span_begin + ni.n_spans;
+    // up to 64 spans: lane l of every wave holds span l's tile range and, each round,
+    // its prefix of the remaining weights (a wave scan of the span partials)
+    const bool lane_spans = ni.n_spans <= 64;
+    const bool my_span = lane_spans && lane < ni.n_spans;
+    const int4 sp4 = my_span ? d.spans[s0 + lane] : make_int4(0, 0, 0, 0);
     for (int round = 2;; ++round) {
       if (round > MAX_ROUNDS) { if (tid == 0) atomicOr(d.err, ERR_SAMPLING); break; }
       const int32_t need = size - (uniq0 + s_claimed);
       if (need <= 0) break;
-      const int64_t W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
+      int64_t W, spx = INT64_MAX;
+      if (lane_spans) {
+        const int64_t v = my_span ? __hip_atomic_load(d.partials + (size_t)(s0 + lane) * NPART + P_SW, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0;
+        const int64_t incl = wave_incl_scan_i64(v, lane);
+        W = __shfl(incl, 63, 64);
+        if (my_span) spx = incl - v;
+      } else {
+        W = node_span_scan<true>(d.partials, d.fin_prefix, s0, s1);
+      }
       if (W <= 0) break;
       // a round of up to FIN_LDS probes keeps them in LDS; a larger one reserves space in
       // the selected list (uniform over the workgroup)
@@ -2696,7 +2710,6 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           }
         }
         sh_base = b;
-        sh_W = W;
       }
       __syncthreads();
       const int64_t base = sh_base;
@@ -2706,8 +2719,18 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         const uint4 o = philox((uint32_t)j, (uint32_t)ni.lnode, (uint32_t)t,
                                ST_PROBE | ((uint32_t)round << 8), ni.seed);
         const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
-        const int64_t T = (int64_t)mulhi64(u, (uint64_t)sh_W);
-        const int64_t slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
+        const int64_t T = (int64_t)mulhi64(u, (uint64_t)W);
+        int64_t slot = -1;
+        if (lane_spans) {
+          const uint64_t ok = __ballot(spx <= T);         // monotone over the node's spans
+          if (ok) {
+            const int L = 63 - __clzll((unsigned long long)ok);
+            slot = resolve_in_tiles<true, false>(d, __shfl(sp4.y, L, 64), __shfl(sp4.z, L, 64), __shfl(spx, L, 64), T,
+                                                 lane);
+          }
+        } else {
+          slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
+        }
         if (lane == 0) {
           const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
           if (slot < 0) atomicOr(d.err, ERR_SAMPLING);

@@ -59,6 +59,13 @@ def make(case, seed=7, **over):
                  r0_scalars=np.array([1.0, 1.5]), init_immun=0.1, init_prev=[400, 100],
                  distances=np.array([[0, 1], [1, 0]]), dur_exp=lp.constant(2), dur_inf=lp.constant(3))
         comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
+    elif case == "bignode":
+        # a node of more than 64 spans (over 1M slots): the samplers' span search in
+        # global memory instead of one span per lane
+        p = dict(dur=10, init_pop=np.array([1_150_000, 40000]), cbr=np.array([35.0]), r0=25,
+                 r0_scalars=np.array([1.0, 1.0]), init_immun=0.3, init_prev=[3000, 20],
+                 distances=np.array([[0, 20], [20, 0]]))
+        comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
     elif case == "zinb":
         # infection only in node 0; other nodes get importations through the zero-inflated NB branch
         p = dict(dur=80, init_pop=np.array([40000, 30000, 30000, 20000]), cbr=None, r0=12,
@@ -112,4 +119,4 @@ def make(case, seed=7, **over):
     return sim
 
 
-CASES = ["small3", "sia", "c0mini", "hot", "hotbig", "zinb", "stop", "campaigns", "ri_only"]
+CASES = ["small3", "sia", "c0mini", "hot", "hotbig", "bignode", "zinb", "stop", "campaigns", "ri_only"]
