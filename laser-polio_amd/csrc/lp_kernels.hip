@@ -2631,7 +2631,11 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
 // Per node: take-all nodes (size >= #positive-weight susceptibles) expose every
 // such agent; otherwise the node's later rounds (2, 3, ...) run here until
 // `size` distinct agents are exposed or no weight is left.  A round's probes are
-// kept in LDS up to FIN_LDS of them, else in the selected list.
+// kept in LDS up to FIN_LDS of them, else in the selected list.  The workgroup's
+// threads pass values between the phases of a round only through LDS and through
+// L2 (agent-scope atomics, atomic loads and stores), so __syncthreads (workgroup-
+// scope release / acquire) orders them: a device-scope fence would also write back
+// the XCD's whole L2, at a cost of microseconds.
 constexpr int FIN_LDS = 2048;
 __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   const int4 dy = today(d);
@@ -2738,7 +2742,6 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           else __hip_atomic_store(gsel + j, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      if (!in_lds) __threadfence();
       __syncthreads();
       for (int j = tid; j < need; j += WG) {
         const uint64_t e = in_lds ? s_sel[j] : __hip_atomic_load(gsel + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2753,8 +2756,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           __hip_atomic_store(gsel + j, SEL_INVALID, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      __threadfence();   // the claims and weight updates before the exposures and the next round
-      __syncthreads();
+      __syncthreads();   // the claims and weight updates before the exposures and the next round
       for (int j0 = 0; j0 < need; j0 += WG) {          // uniform trip count: whole waves append
         const int j = j0 + tid;
         const uint64_t e = j >= need ? SEL_INVALID
@@ -2772,7 +2774,6 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         wave_append_entry(d, nxt, n, ex ? exposed_entry_rec(slot, rec, kx, t, r0_sel(r0v, kx)) : make_uint4(0u, 0u, 0u, 0u),
                           ex, lane);
       }
-      __threadfence();
       __syncthreads();
     }
   }
