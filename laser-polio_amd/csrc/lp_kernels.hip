@@ -1265,6 +1265,9 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   __shared__ int32_t lc[L_N];
   __shared__ unsigned long long lib[LP_MAX_STRAINS];
   __shared__ int64_t red[WG / 64][4];
+  // VD days: the simulation's Kaplan-Meier table in LDS for the newborns' death dates
+  // (a binary search of dependent loads per newborn)
+  __shared__ int64_t s_km[VD ? LP_KM_LEN : 1];
 
   const lp_sim_params& sp = d.sims[sim];
   const int64_t nb_lo = VD ? d.nb_begin[node] : live_hi;
@@ -1287,6 +1290,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
   const bool camp = RI || sia_here;
   if (tid < L_N) lc[tid] = 0;
   if (tid < LP_MAX_STRAINS) lib[tid] = 0;
+  if (VD && tid < LP_KM_LEN) s_km[tid] = d.km_cd[(size_t)sim * LP_KM_LEN + tid];
   __syncthreads();
 
   int32_t c_s = 0, c_spos = 0, c_r = 0;
@@ -1389,7 +1393,7 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
           const uint32_t m16 = 0xffffu << sh16;
           if (VD) {
             if (i >= nb_lo) {                             // newborn (model.py:1719-1732)
-              const int32_t dod = t + km_age_at_death(d.km_cd + (size_t)sim * LP_KM_LEN, seed, slot, t, 0);
+              const int32_t dod = t + km_age_at_death(s_km, seed, slot, t, 0);
               d.dob[i] = t;
               d.dod[i] = dod;
               dmin = dod < dmin ? dod : dmin;
