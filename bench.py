@@ -226,7 +226,7 @@ def main():
         }
         if tr:
             roofline["traffic_over_algorithmic"] = round(tr["bytes_per_launch"] / (bytes_alg / launches), 3)
-        log(f"[bench] k_agents: {launches} launches, avg {kernel_ms * 1e3 / launches:.1f} us, {achieved:.0f} GB/s")
+        log(f"[bench] agent pass: {launches} launches, avg {kernel_ms * 1e3 / launches:.1f} us, {achieved:.0f} GB/s")
 
     cpu = None
     if oracle_sim is not None:
