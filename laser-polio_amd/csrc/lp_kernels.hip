@@ -780,7 +780,8 @@ __device__ __forceinline__ void write_partial(const Dev& d, int tid, const int64
   int64_t* slot = d.partials + (size_t)blockIdx.x * NPART + tid;
   if (tid != P_SW) *slot = v;
   else if (recount) *slot = v;
-  else if (sw_delta && v != 0) *slot += v;
+  else if (sw_delta && v != 0)   // the carried weight: a delta, added without a dependent load
+    atomicAdd(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)v);
 }
 
 // ---------------------------------------------------------------------------
