@@ -1,9 +1,9 @@
 """Host-side logic and the C-ABI library, without a GPU.
 
 No entry point that launches work is called here: the library is loaded, its
-exported symbols are checked against include/lp_step.h, and the two pure
-queries (ABI version, struct size, bind names) are compared with the host
-mirror.
+exported symbols are checked against include/lp_step.h, the pure queries (ABI
+version, struct size, bind names) are compared with the host mirror, and
+lp_create is called only with configurations it rejects before any HIP call.
 """
 
 import ctypes
@@ -67,6 +67,21 @@ def test_abi_version_and_struct_layout():
     text = open(HEADER).read()
     for name, val in (("LP_TILE", 1024), ("LP_CHUNK", 256), ("LP_MAX_STRAINS", em.MAX_STRAINS)):
         assert re.search(rf"#define {name} {val}\b", text), name
+
+
+def test_create_rejects_out_of_range_configs():
+    """lp_create checks its limits before any HIP call: capacity below 2^32 and a multiple
+    of LP_TILE, at most 32767 nodes, 1-4 strains, 1 <= nt <= 2^23 (list entries hold event
+    days in 24 bits)."""
+    lib = _lib()
+    good = dict(capacity=1 << 20, n_nodes=4, n_sims=1, n_strains=1, nt=366, n_sia_events=0, mix_mode=0)
+    bad = [dict(nt=(1 << 23) + 1), dict(nt=0), dict(capacity=1 << 32), dict(capacity=1000), dict(n_nodes=32768),
+           dict(n_strains=5), dict(n_strains=0), dict(n_sims=0), dict(mix_mode=1)]
+    for over in bad:
+        cfg = em.LPConfig(**{**good, **over})
+        ctx = ctypes.c_void_p()
+        assert lib.lp_create(ctypes.byref(cfg), ctypes.byref(ctx)) == -1, over   # LP_ERR_ARG
+        assert not ctx.value
 
 
 def test_bind_names_cover_host_columns_and_results():
