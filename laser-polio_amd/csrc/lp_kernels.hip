@@ -1996,6 +1996,9 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   const int64_t carry0 = tid < 3 ? d.node_carry[(size_t)n * 3 + tid] : 0;
   const int32_t pop_prev = (tid == 0 && t >= 1) ? d.r_pop[row - N + n] : 0;
   const int32_t births_n = tid == 0 ? d.births[n] : 0;
+  // up to 64 spans: wave 0 holds the carried span weights for the span prefix (lane = span)
+  const bool lane_spans = s1 - s0 <= 64;
+  const int64_t psw = (lane_spans && tid < 64 && s0 + tid < s1) ? d.partials[(size_t)(s0 + tid) * NPART + P_SW] : 0;
   if (sim_stopped(d, sim, t)) return;
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(dy.y, mask);
@@ -2040,8 +2043,17 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   if (tid < LP_MAX_STRAINS)   // infectious weight per strain as the network mix reads it
     d.beta_f[(size_t)n * LP_MAX_STRAINS + tid] = tid < d.K ? (float)tally[P_IB0 + tid] * 5.9604644775390625e-08f : 0.0f;
   if (flags & LP_DAY_TX) {
-    const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
-    if (tid == 0) d.W_round[n] = W;
+    if (lane_spans) {
+      if (tid < 64) {
+        const int64_t incl = wave_incl_scan_i64(psw, tid);
+        if (s0 + tid < s1) d.span_prefix[s0 + tid] = incl - psw;
+        const int64_t W = __shfl(incl, 63, 64);
+        if (tid == 0) d.W_round[n] = W;
+      }
+    } else {
+      const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
+      if (tid == 0) d.W_round[n] = W;
+    }
   }
   if (tid != 0) return;
   const size_t K4 = d.K;
