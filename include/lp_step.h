@@ -54,7 +54,8 @@ enum {
 };
 
 enum lp_dtype { LP_I8 = 1, LP_U8 = 2, LP_I16 = 3, LP_U16 = 4, LP_I32 = 5, LP_U32 = 6,
-                LP_I64 = 7, LP_U64 = 8, LP_F32 = 9 };
+                LP_I64 = 7, LP_U64 = 8, LP_F32 = 9,
+                LP_F64 = 10 /* held as bit patterns: no kernel does float64 arithmetic */ };
 
 /* Stages (bit mask) -- one per reference component, plus the log tally. */
 enum {
@@ -126,8 +127,15 @@ const char* lp_bind_names(void);
  * allocate scratch.  Must follow the last lp_bind and precede any step. */
 int lp_prepare(lp_ctx* ctx);
 
-/* Use this HIP stream (hipStream_t) instead of the context's own. */
+/* Use this HIP stream (hipStream_t, owned by the caller; NULL = the default
+ * stream) instead of the context's own.  Waits for the work queued on the
+ * current stream, destroys the context's own stream if it still has it, and
+ * never destroys the caller's.  Must precede the first graph capture.  On the
+ * default stream lp_run_days launches the day kernels directly (a hipGraph is
+ * not captured there). */
 int lp_set_stream(lp_ctx* ctx, void* hip_stream);
+/* 1 if the context still uses (and will destroy) its own stream, else 0. */
+int lp_owns_stream(lp_ctx* ctx);
 
 /* Fused days: runs ticks [t_begin, t_end) with every stage the schedule enables,
  * in reference order (VitalDynamics, DiseaseState, RI, SIA, Transmission, log).
@@ -159,7 +167,9 @@ int lp_stop_days(lp_ctx* ctx, int32_t* out, int32_t n_sims);
 
 /* Diagnostics of the last transmission step (device pointers of library scratch):
  * "p" float[n_nodes*n_strains], "expected" float[n_nodes], "total" int64[n_nodes],
- * "beta_mixed" float[n_nodes*n_strains].  Returns NULL for unknown names. */
+ * "beta_mixed" float[n_nodes*n_strains]; of the last VitalDynamics step:
+ * "birth_expected" int64[n_nodes][2] = floor and Bernoulli threshold ceil(frac * 2^32)
+ * of the expected births.  Returns NULL for unknown names. */
 const void* lp_debug_buffer(lp_ctx* ctx, const char* name);
 
 /* Timing of the agent-pass kernel: with timing on, lp_run_days launches the day

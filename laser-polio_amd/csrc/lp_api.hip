@@ -45,7 +45,7 @@ const BindSpec kBinds[] = {
   {"node_sim", LP_I32, L_NODES, true, F(node_sim)},
   {"r0_scalar", LP_F32, L_NODES, true, F(r0s)},
   {"keep", LP_F32, L_NODES, true, F(keep)},
-  {"birth_rate_step", LP_F32, L_NODES, true, F(birth_rate_step)},
+  {"birth_rate_step", LP_F64, L_NODES, true, F(birth_rate_step)},
   {"thr_ri", LP_U64, L_NODES, true, F(thr_ri)},
   {"thr_ipv", LP_U64, L_NODES, true, F(thr_ipv)},
   {"sim_params", LP_U8, L_SIMPAR, true, F(sims)},
@@ -379,9 +379,18 @@ int lp_set_stream(lp_ctx* c, void* s) {
   if (!c) return LP_ERR_ARG;
   for (int k = 0; k < lp::N_KINDS; ++k)
     if (c->graph_ready[k]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
-  c->stream = reinterpret_cast<hipStream_t>(s);
+  const hipStream_t ns = reinterpret_cast<hipStream_t>(s);
+  if (ns == c->stream) return LP_OK;
+  // work queued on the current stream completes before anything is queued on the new one
+  if (c->stream) HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the library's own stream is released; the caller's stream stays the caller's
+  if (c->own_stream && c->stream) HIPCHK(c, hipStreamDestroy(c->stream));
+  c->own_stream = false;
+  c->stream = ns;
   return LP_OK;
 }
+
+int lp_owns_stream(lp_ctx* c) { return c ? (c->own_stream ? 1 : 0) : LP_ERR_ARG; }
 
 int lp_prepare(lp_ctx* c) {
   if (!c) return LP_ERR_ARG;
@@ -493,6 +502,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.beta_mixed, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.expected_dbg, N))) return rc;
   if ((rc = alloc(c, &d.total_dbg, N))) return rc;
+  if ((rc = alloc(c, &d.birth_dbg, 2 * (size_t)N))) return rc;
   if ((rc = alloc(c, &d.size, N))) return rc;
   if ((rc = alloc(c, &d.take_all, N))) return rc;
   if ((rc = alloc(c, &d.n_uniq, N))) return rc;
@@ -559,7 +569,7 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
     if (list_kind(kind)) c->rec_actual = false;
     else enqueue_actualize(c);
     c->list_tick = t + 1;   // every day kind run here completes the next tick's list
-    if (use_graph && !c->timing && !first) {
+    if (use_graph && !c->timing && !first && c->stream != nullptr) {
       if (!c->graph_ready[kind]) {
         hipGraph_t graph;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -647,6 +657,7 @@ const void* lp_debug_buffer(lp_ctx* c, const char* name) {
   if (!std::strcmp(name, "total")) return c->d.total_dbg;
   if (!std::strcmp(name, "beta_mixed")) return c->d.beta_mixed;
   if (!std::strcmp(name, "size")) return c->d.size;
+  if (!std::strcmp(name, "birth_expected")) return c->d.birth_dbg;
   return nullptr;
 }
 

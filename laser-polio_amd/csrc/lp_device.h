@@ -48,6 +48,57 @@ __device__ __forceinline__ uint64_t thr_from_prob(float p) {
   return (uint64_t)ceilf(s);
 }
 
+// The IEEE-754 binary64 product y = fl(x * n) (round to nearest, ties to even)
+// of a non-negative double x, given by its bit pattern, and an integer n >= 0,
+// computed with integer arithmetic only: returns floor(y) and sets *thr =
+// ceil((y - floor(y)) * 2^32), the Bernoulli threshold of its fractional part.
+// x * n is exact in 128 bits (53 + 32 bits); it is rounded to 53 significant
+// bits as the floating-point multiply would round it.  Negative, infinite and
+// NaN x give 0.  Values below 2^-32 (subnormal results included) have floor 0
+// and threshold 1, which no rounding below that scale changes.
+__device__ __forceinline__ int64_t f64_mul_floor(uint64_t xbits, uint32_t n, uint64_t* thr) {
+  *thr = 0;
+  const uint32_t ef = (uint32_t)(xbits >> 52) & 0x7ffu;
+  uint64_t m = xbits & ((1ull << 52) - 1);
+  if ((xbits >> 63) || ef == 0x7ffu || n == 0) return 0;
+  int32_t e2;
+  if (ef == 0) e2 = -1074;
+  else { m |= 1ull << 52; e2 = (int32_t)ef - 1075; }
+  if (m == 0) return 0;
+  const unsigned __int128 P = (unsigned __int128)m * n;
+  const uint64_t hi = (uint64_t)(P >> 64), lo = (uint64_t)P;
+  const int L = hi ? 128 - __clzll(hi) : 64 - __clzll(lo);   // bit length of P
+  uint64_t M;
+  int32_t ex = e2;
+  if (L > 53) {
+    int s = L - 53;
+    M = (uint64_t)(P >> s);
+    const unsigned __int128 R = P & ((((unsigned __int128)1) << s) - 1);
+    const unsigned __int128 half = ((unsigned __int128)1) << (s - 1);
+    if (R > half || (R == half && (M & 1))) ++M;
+    if (M == (1ull << 53)) { M >>= 1; ++s; }
+    ex += s;
+  } else {
+    M = lo;
+  }
+  const int lm = 64 - __clzll(M);                               // bit length of M
+  if (ex + lm - 1 < -32) { *thr = 1; return 0; }               // 0 < y < 2^-32
+  if (ex >= 0) return ex >= 10 ? INT64_MAX : (int64_t)(M << ex);
+  const int sh = -ex;
+  if (sh >= 64) {                                               // floor 0, y = M 2^-sh, sh - 32 < 64 here
+    const int k = sh - 32;
+    *thr = (M >> k) + ((M & ((1ull << k) - 1)) != 0);
+    return 0;
+  }
+  const uint64_t ip = M >> sh, fb = M & ((1ull << sh) - 1);
+  if (sh <= 32) *thr = fb << (32 - sh);
+  else {
+    const int k = sh - 32;
+    *thr = (fb >> k) + ((fb & ((1ull << k) - 1)) != 0);
+  }
+  return (int64_t)ip;
+}
+
 __device__ __forceinline__ float u24(uint32_t r) { return (float)(r >> 8) * 5.9604644775390625e-08f; }
 
 // ---- float32 elementary functions --------------------------------------------

@@ -77,7 +77,8 @@ struct Dev {
   uint4* rec;   // [capacity] agent records: the DiseaseState / tally fields of each slot in 16 B
   // node tables
   const int64_t* seg_begin; int64_t* live_end; const int64_t* seg_end; const int32_t* node_sim;
-  const float* r0s; const float* keep; const float* birth_rate_step;
+  const float* r0s; const float* keep;
+  const uint64_t* birth_rate_step;   // step * birth rate per node, float64 bit patterns
   const uint64_t* thr_ri; const uint64_t* thr_ipv;
   // sim tables
   const lp_sim_params* sims; const float* network_t; const int64_t* km_cd;
@@ -110,6 +111,7 @@ struct Dev {
   uint32_t* ri_list;       // [2][capacity] per-node lists of RI candidates (slots)
   int32_t* ri_lcnt;        // [2][nodes] their lengths
   float* p_dbg; float* expected_dbg; int64_t* total_dbg; float* beta_mixed;
+  int64_t* birth_dbg;      // [nodes][2] integer part and fraction threshold of the last VD step's expected births
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
   int64_t* probe_off; int64_t* round_base; uint64_t* sel_list; int64_t* sel_count; int64_t sel_cap;
@@ -193,18 +195,20 @@ __device__ void day_begin_node(const Dev& d, int n, int32_t t, int mask) {
   d.ei_cnt[(size_t)((t + 1) & 1) * d.n_nodes + n] = 0;
   if (sim_stopped(d, sim, t)) return;
   if ((flags & LP_DAY_VD_STEP) && t >= 1) {
+    // expected = step * rate * pop[t-1] in float64 as the reference computes it
+    // (model.py:1712-1716), reproduced exactly in integer arithmetic
     const int32_t pop_prev = d.r_pop[(size_t)(t - 1) * d.n_nodes + n];
-    const float e = d.birth_rate_step[n] * (float)pop_prev;
-    int32_t b = 0;
-    if (e > 0.0f) {
-      const int32_t bi = (int32_t)e;
-      const float frac = e - (float)bi;
+    uint64_t thr = 0;
+    int64_t b = pop_prev > 0 ? f64_mul_floor(d.birth_rate_step[n], (uint32_t)pop_prev, &thr) : 0;
+    d.birth_dbg[2 * n] = b;
+    d.birth_dbg[2 * n + 1] = (int64_t)thr;
+    if (thr) {
       const uint4 o = philox(0u, (uint32_t)(n - sp.node_begin), (uint32_t)t, ST_BIRTH, sp.seed);
-      b = bi + ((uint64_t)o.x < thr_from_prob(frac) ? 1 : 0);
+      b += (uint64_t)o.x < thr ? 1 : 0;
     }
     const int64_t room = d.seg_end[n] - d.live_end[n];
-    if (b > room) { b = (int32_t)room; atomicOr(d.err, ERR_CAPACITY); }
-    d.births[n] = b;
+    if (b > room) { b = room; atomicOr(d.err, ERR_CAPACITY); }
+    d.births[n] = (int32_t)b;
     d.live_end[n] += b;
   }
 }

@@ -12,7 +12,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblpstep.so")
 
-LP_I8, LP_U8, LP_I16, LP_U16, LP_I32, LP_U32, LP_I64, LP_U64, LP_F32 = range(1, 10)
+LP_I8, LP_U8, LP_I16, LP_U16, LP_I32, LP_U32, LP_I64, LP_U64, LP_F32, LP_F64 = range(1, 11)
 _DTYPE_CODE = {
     np.dtype(np.int8): LP_I8,
     np.dtype(np.uint8): LP_U8,
@@ -23,6 +23,7 @@ _DTYPE_CODE = {
     np.dtype(np.int64): LP_I64,
     np.dtype(np.uint64): LP_U64,
     np.dtype(np.float32): LP_F32,
+    np.dtype(np.float64): LP_F64,
 }
 # torch has no general uint16/uint32/uint64 kernels: keep those buffers as signed views.
 _TORCH_VIEW = {np.dtype(np.uint16): np.int16, np.dtype(np.uint32): np.int32, np.dtype(np.uint64): np.int64}
@@ -99,6 +100,7 @@ def load_library(path=LIB_PATH):
     lib.lp_bind_names.restype = ctypes.c_char_p
     lib.lp_prepare.argtypes = [vp]
     lib.lp_set_stream.argtypes = [vp, vp]
+    lib.lp_owns_stream.argtypes = [vp]
     lib.lp_run_days.argtypes = [vp, i32, i32, i32]
     lib.lp_step_stage.argtypes = [vp, i32, i32]
     for f in ("lp_step_vital", "lp_step_disease", "lp_step_ri", "lp_step_sia", "lp_step_transmission", "lp_log"):
@@ -198,6 +200,15 @@ class Engine:
         if rc != 0:
             msg = self.lib.lp_last_error(self.ctx).decode() if self.ctx else ""
             raise LPError(f"liblpstep error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+    def set_stream(self, stream):
+        """Queue the library's work on `stream` (a torch.cuda.Stream or a raw hipStream_t
+        handle) instead of its own stream; the caller keeps ownership of it."""
+        handle = getattr(stream, "cuda_stream", stream)
+        self._check(self.lib.lp_set_stream(self.ctx, ctypes.c_void_p(int(handle) if handle else None)))
+
+    def owns_stream(self):
+        return bool(self.lib.lp_owns_stream(self.ctx))
 
     def run_days(self, t_begin, t_end, use_graph=True):
         self._check(self.lib.lp_run_days(self.ctx, int(t_begin), int(t_end), 1 if use_graph else 0))

@@ -295,10 +295,11 @@ class SEIR_ABM:
             keep = np.ones(N, np.float32)
             network_t = np.zeros(N * N, np.float32)
         if vd is not None:
-            brs = (np.float64(vd.step_size) * vd.birth_rate).astype(np.float32)
+            # the reference's step_size * birth_rate (model.py:1712), kept in float64
+            brs = vd.step_size * np.asarray(vd.birth_rate, dtype=np.float64)
             km_cd = vd.km_cd
         else:
-            brs = np.zeros(N, np.float32)
+            brs = np.zeros(N, np.float64)
             km_cd = create_cumulative_deaths(int(pars.init_pop.sum()), 100)
         ri_on = ri is not None and pars.vx_prob_ri is not None
         thr_ri = engmod.thr_from_prob64(np.broadcast_to(np.asarray(pars.vx_prob_ri if ri_on else 0.0, dtype=np.float64), (N,)))

@@ -214,6 +214,20 @@ inline int32_t km_age_at_death(const int64_t* cd, int32_t km_len, uint64_t seed,
   return lo_d + (int32_t)(((uint64_t)o[2] * (uint64_t)(hi_d - lo_d)) >> 32);
 }
 
+// expected_births = step_size * birth_rate * pop[t-1] in float64 (model.py:1712; brs is
+// step_size * birth_rate, the product's first factor): returns its integer part
+// (astype(int32), model.py:1713) and sets *thr to the Bernoulli threshold of the
+// fractional part (model.py:1714-1715), ceil(frac * 2^32) (R1).
+int32_t birth_expected(double brs, int32_t pop, uint64_t* thr) {
+  *thr = 0;
+  const double e = brs * (double)pop;
+  if (!(e > 0.0)) return 0;
+  const int32_t bi = (int32_t)e;
+  const double frac = e - (double)bi;
+  if (frac > 0.0) *thr = (uint64_t)std::ceil(frac * 4294967296.0);
+  return bi;
+}
+
 }  // namespace
 
 extern "C" {
@@ -228,7 +242,7 @@ struct orc_model {
   int8_t* pp; int8_t* paralyzed; int8_t* ipv; uint8_t* missed;
   float* risk; float* infect; int32_t* dob; int32_t* dod; int16_t* ri_timer;
   int64_t* seg_begin; int64_t* live_end; int64_t* seg_end;
-  float* r0s; float* keep; float* birth_rate_step; float* NT;
+  float* r0s; float* keep; double* birth_rate_step; float* NT;
   uint64_t* thr_ri; uint64_t* thr_ipv;
   int64_t* km_cd; int32_t km_len; int32_t newborn_ri_timer;
   float strain_r0[4];
@@ -237,6 +251,7 @@ struct orc_model {
 };
 
 int64_t orc_struct_size(void) { return (int64_t)sizeof(orc_model); }
+int32_t orc_birth_expected(double brs, int32_t pop, uint64_t* thr) { return birth_expected(brs, pop, thr); }
 int32_t orc_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_philox(const uint32_t* ctr, uint64_t seed, uint32_t* out) {
@@ -275,14 +290,12 @@ int32_t orc_births(const orc_model* m, int32_t t, const int32_t* pop_prev, int32
   int32_t overflow = 0;
   std::vector<int64_t> nb_begin(N);
   for (int n = 0; n < N; ++n) {
-    const float e = m->birth_rate_step[n] * (float)pop_prev[n];
-    int32_t b = 0;
-    if (e > 0.0f) {
-      const int32_t bi = (int32_t)e;
-      const float frac = e - (float)bi;
+    uint64_t thr = 0;
+    int32_t b = birth_expected(m->birth_rate_step[n], pop_prev[n], &thr);
+    if (thr) {
       uint32_t o[4];
       o[0] = 0; o[1] = (uint32_t)n; o[2] = (uint32_t)t; o[3] = ST_BIRTH; philox(o, m->seed);
-      b = bi + ((uint64_t)o[0] < thr_from_prob(frac) ? 1 : 0);
+      b += (uint64_t)o[0] < thr ? 1 : 0;
     }
     const int64_t room = m->seg_end[n] - m->live_end[n];
     if (b > room) { b = (int32_t)room; overflow += 1; }

@@ -22,8 +22,14 @@ Parity status of the oracle's functions (how each is pinned, see tests/):
   disease_state_step     reference KATs test_progression_without_transmission,
                          test_disease_timers_with_trans_explicit,
                          test_paralysis_progression_manual; brute force
-  get_deaths / births    bookkeeping KATs of tests/test_vital_dynamics.py;
-                         birth counts against the closed form
+  get_deaths            bookkeeping KATs of tests/test_vital_dynamics.py
+  births                 bookkeeping KATs; expected births computed in float64
+                         exactly as model.py:1712-1716 and checked against numpy's
+                         evaluation of that expression over a sweep of rates and
+                         populations (tests/test_oracle_math.py)
+  seed_schedule          reference test_seed_schedule (tests/test_diseasestate_abm.py:
+                         398-457) ported with synthetic nodes; seeded counts and
+                         timers checked against the definition
   fast_ri / fast_sia     test_ri_no_effect_on_non_susceptibles, RI windows of
                          test_ri_zero, SIA date/node/age targeting; binomial rates
   tx_step_prep / count   brute-force numpy bincount
@@ -102,6 +108,8 @@ def lib():
         L.orc_negbin.restype = i64
         L.orc_km_age_at_death.argtypes = [vp, i32, u64, u64, i32, i32]
         L.orc_km_age_at_death.restype = i32
+        L.orc_birth_expected.argtypes = [ctypes.c_double, i32, vp]
+        L.orc_birth_expected.restype = i32
         M = ctypes.POINTER(OrcModel)
         L.orc_get_deaths.argtypes = [M, i32, vp]
         L.orc_births.argtypes = [M, i32, vp, vp]
@@ -139,10 +147,36 @@ RESULT_SHAPES = {
 }
 
 
+def parse_seed_schedule(seeding):
+    """tick -> [(node, value)] from seed_schedule entries (model.py:535-545): an entry with
+    "date" and "dot_name" seeds the node whose node_lookup dot_name matches, on the tick
+    (date - start_date).days (entries naming no node are dropped); one with "timestep"
+    and "node_id" is taken as it is."""
+    import datetime
+
+    out = {}
+    lookup = seeding.get("node_lookup") or {}
+    start = seeding["start_date"]
+    for entry in seeding.get("seed_schedule") or []:
+        if "date" in entry and "dot_name" in entry:
+            d = entry["date"]
+            if isinstance(d, str):
+                d = datetime.date.fromisoformat(d)
+            node = next((k for k, v in lookup.items() if v["dot_name"] == entry["dot_name"]), None)
+            if node is not None:
+                out.setdefault((d - start).days, []).append((node, entry["prevalence"]))
+        elif "timestep" in entry and "node_id" in entry:
+            out.setdefault(entry["timestep"], []).append((entry["node_id"], entry["prevalence"]))
+    return out
+
+
 class OracleSim:
     """One simulation on the CPU oracle, from the product's exported SimTables."""
 
-    def __init__(self, tables, pop0, n_threads=0, results0=None):
+    def __init__(self, tables, pop0, n_threads=0, results0=None, seeding=None):
+        """seeding: None, or a dict with the raw ``seed_schedule`` entries, ``node_lookup``,
+        ``start_date``, the numpy Generator ``rng`` the seeds are drawn from and
+        ``dur_inf(rng, n)``, the infectious-duration sampler (see _seed)."""
         tb = tables
         self.tb = tb
         self.cols = {k: np.array(v, copy=True) for k, v in tb.cols.items()}
@@ -177,7 +211,7 @@ class OracleSim:
         self.tables = {
             "r0s": np.ascontiguousarray(tb.r0s, np.float32),
             "keep": np.ascontiguousarray(tb.keep, np.float32),
-            "brs": np.ascontiguousarray(tb.birth_rate_step, np.float32),
+            "brs": np.ascontiguousarray(tb.birth_rate_step, np.float64),
             "NT": np.ascontiguousarray(tb.network_t, np.float32),
             "thr_ri": np.ascontiguousarray(tb.thr_ri, np.uint64),
             "thr_ipv": np.ascontiguousarray(tb.thr_ipv, np.uint64),
@@ -198,6 +232,8 @@ class OracleSim:
         m.ri_step = tb.ri_step
         m.zi = float(tb.zi)
         self.m = m
+        self.seeding = seeding
+        self.seed_schedule = parse_seed_schedule(seeding) if seeding else {}
         self.node_map_log = {}
         self.record_node_map = False
 
@@ -238,6 +274,8 @@ class OracleSim:
             L.orc_disease_state_step(m, t, _p(a), _p(b))
             r["new_potentially_paralyzed"][t] = a
             r["new_paralyzed"][t] = b
+        if (f & DAY_DS) and t in self.seed_schedule:
+            self._seed(t)
         if f & DAY_RI:
             a, b, c = (np.zeros(N, np.int32) for _ in range(3))
             L.orc_fast_ri(m, t, _p(a), _p(b), _p(c))
@@ -268,6 +306,32 @@ class OracleSim:
         if f & DAY_TX:
             self._transmission(t)
             self._count(t)
+
+    def _seed(self, t):
+        """seed_schedule injections at the end of DiseaseState_ABM.step (model.py:759-786).
+        Candidates are the node's live agents in slot order (the reference's index order
+        within the node); a float value seeds int(len * value) of them, an int
+        min(value, len); they are chosen uniformly without replacement, become I whatever
+        their state, and those whose infection timer is 0 get a fresh infectious
+        duration, truncated to an integer and clipped to [0, 127] as at initialisation
+        (model.py:575-579)."""
+        state, itimer = self.cols["disease_state"], self.cols["infection_timer"]
+        rng, dur_inf = self.seeding["rng"], self.seeding["dur_inf"]
+        for node, value in self.seed_schedule[t]:
+            lo, hi = int(self.seg_begin[node]), int(self.live_end[node])
+            cand = lo + np.nonzero(state[lo:hi] >= 0)[0]
+            if isinstance(value, float):
+                n_seed = int(len(cand) * value)
+            elif isinstance(value, (int, np.integer)):
+                n_seed = min(int(value), len(cand))
+            else:
+                raise ValueError(f"Unsupported seed value type: {type(value)}")
+            if n_seed <= 0:
+                continue
+            sel = rng.choice(cand, size=n_seed, replace=False)
+            state[sel] = 2
+            zero = sel[itimer[sel] == 0]
+            itimer[zero] = np.clip(np.trunc(np.asarray(dur_inf(rng, len(zero)), np.float64)), 0, 127).astype(np.uint16)
 
     def _transmission(self, t):
         L, m, N, K = lib(), ctypes.byref(self.m), self.N, self.K

@@ -110,6 +110,34 @@ def make(case, seed=7, **over):
                  r0_scalars=rng.lognormal(0, 0.3, n), init_immun=0.2, init_prev=[5] * 50 + [0] * (n - 50),
                  distances=d)
         comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
+    elif case == "seeds":
+        # seed_schedule injections on a plain day (float prevalence by date + dot_name), on
+        # day 2 (int count by timestep), on a VD step day (t=7), on a VD + RI day (t=14,
+        # more than the node's live agents) and on an SIA day (t=10): each is a
+        # VD + DS stage call, host seeding, then a RECOUNT stage call of RI + SIA + TX + log
+        lookup = {i: {"dot_name": f"AFRO:TESTLAND:REGION:N{i}", "lat": 0.0, "lon": float(i)} for i in range(4)}
+        p = dict(dur=30, init_pop=np.array([9000, 6000, 4000, 2500]), cbr=np.array([40.0]), r0=6,
+                 r0_scalars=np.array([1.0, 1.1, 0.9, 1.2]), init_immun=0.5, init_prev=[10, 0, 0, 0],
+                 distances=np.array([[0, 40, 70, 90], [40, 0, 40, 60], [70, 40, 0, 30], [90, 60, 30, 0]]),
+                 vx_prob_ri=0.6, vx_prob_ipv=0.3, ri_newborn_timer=14, ri_age_days=(14, 28), vx_prob_sia=[0.7] * 4,
+                 sia_schedule=[{"date": "2019-01-11", "nodes": [1, 3], "age_range": (0, 5 * 365), "vaccinetype": "nOPV2"}],
+                 node_lookup=lookup,
+                 seed_schedule=[{"date": "2019-01-02", "dot_name": "AFRO:TESTLAND:REGION:N2", "prevalence": 0.02},
+                                {"timestep": 2, "node_id": 3, "prevalence": 15},
+                                {"timestep": 7, "node_id": 1, "prevalence": 0.05},
+                                {"timestep": 10, "node_id": 0, "prevalence": 40},
+                                {"timestep": 14, "node_id": 3, "prevalence": 10**6},
+                                {"timestep": 14, "node_id": 2, "prevalence": 0.01}])
+    elif case == "stopseed":
+        # the epidemic dies out and the early-stop rule waits for seeds still ahead (t=60
+        # and t=62; a seed on the day of the check itself does not count as ahead,
+        # model.py:792), the seeds re-ignite transmission, and the run stops after it dies out again
+        p = dict(dur=200, init_pop=np.array([3000, 2000]), cbr=np.array([20.0]), r0=0.5,
+                 r0_scalars=np.array([1.0, 1.0]), init_immun=0.0, init_prev=[5, 0], dur_inf=lp.constant(4),
+                 distances=np.array([[0, 10], [10, 0]]), stop_if_no_cases=True,
+                 seed_schedule=[{"timestep": 60, "node_id": 1, "prevalence": 20},
+                                {"timestep": 62, "node_id": 0, "prevalence": 0.01}])
+        comps = [lp.VitalDynamics_ABM, lp.DiseaseState_ABM, lp.Transmission_ABM]
     else:
         raise KeyError(case)
     base.update(p)

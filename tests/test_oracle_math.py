@@ -107,3 +107,59 @@ def test_probability_thresholds():
 
     t = engine.thr_from_prob64(np.array([0.0, 1.0, 0.5, 2**-32, 1e-300, -0.1]))
     assert t.tolist() == [0, 2**32, 2**31, 1, 1, 0]
+
+
+def test_expected_births_match_float64_reference_expression():
+    """VitalDynamics_ABM births (model.py:1712-1716): expected = step_size * birth_rate *
+    pop[t-1] evaluated by numpy in float64 (step_size an int, birth_rate = cbr / (365 * 1000)
+    float64, pop int32), integer part by astype(int32), Bernoulli on expected - integer.
+    The oracle's integer part must be numpy's, and its threshold ceil(frac * 2^32)."""
+    rng = np.random.default_rng(11)
+    cbr = np.concatenate([rng.uniform(0.0, 80.0, 3000), [0.0, 35.0, 37.0, 41.0, 1e-9, 365.0, 1000.0]])
+    pop = np.concatenate([rng.integers(0, 3_000_000, 3000), [1, 7, 52143, 2**31 - 1, 1000, 365000, 3]]).astype(np.int32)
+    birth_rate = cbr / (365 * 1000)
+    for step in (1, 7, 14):
+        expected = step * birth_rate * pop
+        want_int = expected.astype(np.int32)
+        frac = expected - want_int
+        want_thr = np.ceil(frac * 4294967296.0).astype(np.uint64)
+        L = orc.lib()
+        thr = np.zeros(1, np.uint64)
+        brs = step * birth_rate
+        for i in range(len(pop)):
+            got = L.orc_birth_expected(float(brs[i]), int(pop[i]), thr.ctypes.data)
+            assert got == want_int[i], (step, cbr[i], pop[i])
+            assert int(thr[0]) == int(want_thr[i]), (step, cbr[i], pop[i])
+
+
+def test_km_age_at_death_brute_force_on_a_tiny_table():
+    """Age at death on a 4-year table where every year gets many draws: the year of death
+    follows deaths per year conditioned on the current year of age (an off-by-one in the
+    conditioning moves a whole year's mass), the day is uniform over the part of that
+    year after the current age, and every draw is later than the current age."""
+    cd = np.ascontiguousarray([100, 300, 600, 1000], np.int64)
+    L = orc.lib()
+    n = 40000
+    for age in (0, 400, 800, 1000):
+        ay = age // 365
+        draws = np.array([L.orc_km_age_at_death(cd.ctypes.data, len(cd), 9, s, 5, age) for s in range(n)])
+        assert (draws > age).all()
+        yod = draws // 365
+        lo = 0 if ay == 0 else int(cd[ay - 1])
+        p = np.diff(np.concatenate([[lo], cd[ay:]])) / (cd[-1] - lo)
+        obs = np.bincount(yod - ay, minlength=len(p))
+        assert len(obs) == len(p), "a year outside [current year, last year] was drawn"
+        exp = p * n
+        assert (exp >= 5).all()
+        chi = (((obs - exp) ** 2) / exp).sum()
+        assert stats.chi2.sf(chi, len(p) - 1) > 1e-3, (age, obs, exp)
+        for y in range(ay, len(cd)):
+            d = draws[yod == y]
+            lo_d, hi_d = max(y * 365, age + 1), (y + 1) * 365
+            assert d.min() >= lo_d and d.max() < hi_d
+            width = hi_d - lo_d
+            if len(d) >= 5 * width:
+                obs_d = np.bincount(d - lo_d, minlength=width)
+                e = len(d) / width
+                chi_d = (((obs_d - e) ** 2) / e).sum()
+                assert stats.chi2.sf(chi_d, width - 1) > 1e-4, (age, y)
