@@ -37,9 +37,9 @@ def test_foi_on_national_geography_matches_oracle_and_fp64():
         eng.run_days(t, t + 1)
         if t == 0:
             continue
-        got = {"p": eng.debug("p", N * 4, np.float32).reshape(N, 4)[:, :K],
+        got = {"p": eng.debug("p", N * K, np.float32).reshape(N, K),
                "expected": eng.debug("expected", N, np.float32),
-               "beta_mixed": eng.debug("beta_mixed", N * 4, np.float32).reshape(N, 4)[:, :K],
+               "beta_mixed": eng.debug("beta_mixed", N * K, np.float32).reshape(N, K),
                "total": eng.debug("total", N, np.int64)}
         L = ref.node_map_log[t]
         for name in ("p", "expected", "beta_mixed", "total"):
