@@ -61,6 +61,19 @@ int lp_mf_adjoint_days(const lp_mf_model* model, int32_t t0, int32_t t1, const f
                        float* aI, float* r0_bar, float* net_bar, float* season_part, float* work,
                        void* hip_stream);
 
+/* lp_mf_adjoint_days without the `work` argument: the same kernels, with their scratch
+ * in a fixed device array of the library (2 * LP_MF_NOALLOC_MAX_NODES floats, 256 KiB
+ * per GPU, held until the process ends).  It allocates nothing (no hipMalloc, no
+ * library allocation), for callers that cannot: code under hipGraph capture, tight
+ * calibration loops, harnesses that forbid allocation.  No counterpart in the
+ * reference.  Returns LP_ERR_ARG if model->n > LP_MF_NOALLOC_MAX_NODES.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialised by the caller. */
+#define LP_MF_NOALLOC_MAX_NODES 32768
+int lp_mf_adjoint_days_noalloc(const lp_mf_model* model, int32_t t0, int32_t t1, const float* S, const float* I,
+                               const float* m_save, const float* g_save, const float* a_inc, float* aS, float* aE,
+                               float* aI, float* r0_bar, float* net_bar, float* season_part, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

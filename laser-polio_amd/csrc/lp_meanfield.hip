@@ -3,10 +3,35 @@
 // matvecs (the transposed network row is read coalesced), fixed-order LDS
 // reductions, elementwise kernels for the compartment updates.
 #include "../../include/lp_meanfield.h"
+#include <mutex>
 
 namespace lp_mf {
 
 constexpr int WG = 256;
+
+// Scratch of lp_mf_adjoint_days_noalloc: the adjoint's two per-node work rows
+// (mixed-weight adjoint, carried I adjoint), 2 * LP_MF_NOALLOC_MAX_NODES floats
+// = 256 KiB held by every GPU the library runs on until the process ends.
+static __device__ __align__(256) float scratch[2 * LP_MF_NOALLOC_MAX_NODES];
+
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each device
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per device
+// (each device holds its own copy of the array at its own address).
+int scratch_of_current_device(float** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return LP_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return LP_ERR_HIP;
+    g_scratch[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return LP_OK;
+}
 
 // Fixed-order sum of the 256 per-thread partials of a workgroup.
 __device__ __forceinline__ float wg_sum(float v, float* sh) {
@@ -102,6 +127,19 @@ int lp_mf_days(const lp_mf_model* model, int32_t t0, int32_t t1, float* S, float
                        S + r1, E + r1, I + r1, inc + r0, m_save ? m_save + r0 : nullptr, g_save ? g_save + r0 : nullptr);
   }
   return hipGetLastError() == hipSuccess ? LP_OK : LP_ERR_HIP;
+}
+
+int lp_mf_adjoint_days_noalloc(const lp_mf_model* model, int32_t t0, int32_t t1, const float* S, const float* I,
+                               const float* m_save, const float* g_save, const float* a_inc, float* aS, float* aE,
+                               float* aI, float* r0_bar, float* net_bar, float* season_part, void* hip_stream) {
+  if (!model || model->n <= 0 || model->n > LP_MF_NOALLOC_MAX_NODES) return LP_ERR_ARG;
+  float* work = nullptr;
+  const int rc = lp_mf::scratch_of_current_device(&work);
+  if (rc) return rc;
+  // every work row is written (k_mf_adj_a) before it is read (k_mf_adj_b), so the
+  // previous call's contents need no clearing
+  return lp_mf_adjoint_days(model, t0, t1, S, I, m_save, g_save, a_inc, aS, aE, aI, r0_bar, net_bar,
+                            season_part, work, hip_stream);
 }
 
 int lp_mf_adjoint_days(const lp_mf_model* model, int32_t t0, int32_t t1, const float* S, const float* I,
