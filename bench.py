@@ -1,22 +1,31 @@
 """Benchmark: agent-days/s of the laser-polio daily step on one MI355X.
 
 Workload (the national-scale configuration): 774 nodes, populations
-LogNormal(ln 2.5e5, 0.6) -> about 2.3e8 agents, gravity network, RI + SIA
-campaigns, 2555 simulated days.  A "step" is one simulated day of the whole
-population.  The host initialises the population (not timed), uploads it once,
-runs ``--warmup`` days untimed, then times exactly ``--steps`` days bracketed by
-torch.cuda.synchronize().  value = sum over timed days of live agents / seconds.
+LogNormal(ln 2.5e5, 0.6) -> 2.27e8 agents, gravity network, RI + SIA campaigns,
+2555 simulated days.  A "step" is one simulated day of the whole population.
+The host initialises the population (not timed) and uploads it once.  The days
+before the timed window run first (replayed hipGraphs), the last ``--warmup`` of
+them as warm-up; then exactly ``--steps`` days starting at ``--start-day`` are
+timed, bracketed by torch.cuda.synchronize(), with all state resident in HBM.
+value = sum over the timed days of live agents / seconds.  The write-back of the
+device-held agent records into the bound columns (lp_sync) is not part of a day
+and is reported on its own (writeback_ms).
 
 Also reported on the same JSON line:
-  roofline      the agent pass (k_ei every day, plus k_full / k_ri_list on VD / RI / SIA days):
-                algorithmic bytes per day (agent_pass_bytes below) over its
-                HIP-event-timed duration, against MI355X's 8 TB/s HBM peak
+  full_run      every simulated day of the run timed, as agent-days/s
+  roofline      the agent pass (k_ei every day, plus k_full / k_ri_list on VD / RI /
+                SIA days): algorithmic bytes (agent_pass_bytes below) over its
+                HIP-event-timed duration against MI355X's 8 TB/s HBM peak; its HBM
+                traffic per launch from the committed rocprofv3 per-kernel record;
+                whole_day: the day's algorithmic bytes over the whole day time;
+                survey_byte_model: the bandwidth a dense day would need
   cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
-                threads) on the same initial population for the first --cpu-days days,
-                plus one further day on a single thread
-The HBM traffic of the agent pass (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch,
-calibrated on a plain copy) is read from profiles/agent_pass_national_traffic.md
-when that file was measured on the same workload and window; otherwise null.
+                threads) on the same initial population for the first --cpu-days
+                days, plus the next day on a single thread
+  parity        the device state after that day (every results row, agent column
+                and live end) against the oracle's, bit for bit; a mismatch makes
+                the exit status 1
+  cpu_baseline_spatial14   the CPU oracle over the whole 14-node workload
 """
 
 import argparse
@@ -81,19 +90,19 @@ def agent_pass_bytes(res, flags, t0, t1):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "agent_pass_national_traffic.md")
 
 
-def workload_key(scale, warmup, steps, tpw):
-    return f"scale={scale:g},warmup={warmup},steps={steps},tiles_per_block={tpw}"
-
-
-def measured_traffic(key, path=TRAFFIC_PROFILE):
-    """HBM bytes per agent-pass launch from the committed rocprofv3 summary, for this workload only."""
+def measured_traffic(scale, tpw, path=TRAFFIC_PROFILE):
+    """HBM bytes per dispatch of each agent-pass kernel (calibrated rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, averaged over a whole run) from the committed record, when it was measured
+    on this population scale and span size; else None."""
     if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic: (\{.*\})$", open(path).read(), re.M)
+    m = re.search(r"^traffic_per_kernel: (\{.*\})$", open(path).read(), re.M)
     if not m:
         return None
     rec = json.loads(m.group(1))
-    return rec if rec.get("workload") == key else None
+    if rec.get("workload") != f"scale={scale:g},tiles_per_block={tpw}":
+        return None
+    return rec["kernels"]
 
 
 def dump_days(path, res, flags, t0, t1):
@@ -115,12 +124,18 @@ def dump_days(path, res, flags, t0, t1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2550)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0, help="population scale (1.0 = ~2.3e8 agents)")
-    ap.add_argument("--cpu-days", type=int, default=24, help="days of the CPU oracle baseline sample (0 = skip)")
+    ap.add_argument("--start-day", type=int, default=1001,
+                    help="first timed day (the days before it run untimed for the window, timed for the whole-run figure)")
+    ap.add_argument("--cpu-days", type=int, default=24,
+                    help="days of the CPU oracle baseline sample; the GPU state after the next day is compared with "
+                         "the oracle's bit for bit (0 = skip both)")
+    ap.add_argument("--cpu-spatial14", type=int, default=1, help="1: also time the CPU oracle on the 14-node workload")
     ap.add_argument("--dump-days", default="", help="write per-day N_live / N_EI / N_draw / flags to this CSV")
-    ap.add_argument("--graph", type=int, default=0, help="1: replay one hipGraph per day (no per-kernel events)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="1: the timed window replays hipGraphs too (no per-kernel events, no roofline)")
     ap.add_argument("--tiles-per-block", type=int, default=0)
     ap.add_argument("--workload", choices=["national", "replicates", "adjoint"], default="national",
                     help="national: one 774-node simulation (default); replicates: --sims independent "
@@ -157,61 +172,214 @@ def main():
         adjoint_workload(args)
         return
 
+    national(args)
+
+
+RESULT_NAMES = [
+    "S", "E", "I", "R", "E_by_strain", "I_by_strain", "potentially_paralyzed", "paralyzed",
+    "new_potentially_paralyzed", "new_paralyzed", "pop", "births", "deaths", "new_exposed",
+    "new_exposed_by_strain", "ri_vaccinated", "ri_protected", "ri_new_exposed_by_strain", "ipv_vaccinated",
+    "sia_vaccinated", "sia_protected", "sia_new_exposed_by_strain",
+]
+
+
+def compare_with_oracle(sim, ref, t_last):
+    """Differences between the device state after tick t_last (pulled to the host) and the
+    oracle's: results rows 0..t_last of every array, every agent column over the whole
+    capacity, every node's live end.  Empty when identical bit for bit."""
+    bad = []
+    for name in RESULT_NAMES:
+        if sim.results.has(name):
+            got = np.asarray(getattr(sim.results, name))[: t_last + 1]
+            n = int(np.count_nonzero(got != ref.res[name][: t_last + 1]))
+            if n:
+                bad.append(f"results.{name}: {n} cells differ")
+    for name, want in ref.cols.items():
+        if sim.people.has(name):
+            n = int(np.count_nonzero(getattr(sim.people, name) != want))
+            if n:
+                bad.append(f"people.{name}: {n} slots differ")
+    if not np.array_equal(np.asarray(sim.people.live_end), ref.live_end):
+        bad.append("live_end differs")
+    return bad
+
+
+def survey_model_bytes(res, flags, t0, t1):
+    """Bytes per day of the dense day the survey's byte model prices (state + risk of every
+    live agent, of every agent in a node with draws, 17 B per E/I agent, death dates on VD
+    days, RI and SIA columns of every live agent), over ticks [t0, t1): the bandwidth a
+    dense implementation would need to run the same days as fast."""
+    from laser_polio_amd import engine as em
+
+    pop_n = res["pop"].astype(np.int64)
+    n_ei = res["E"].astype(np.int64).sum(axis=1) + res["I"].astype(np.int64).sum(axis=1)
+    tx_new = res["new_exposed"].astype(np.int64) - res["ri_protected"] - res["sia_protected"]
+    total = 0
+    for t in range(t0, t1):
+        n_live, f = int(pop_n[t].sum()), int(flags[t])
+        b = 5 * n_live + 5 * int(pop_n[t][tx_new[t] > 0].sum()) + 17 * int(n_ei[t - 1])
+        if f & em.DAY_VD_STEP:
+            b += 4 * n_live
+        if f & em.DAY_RI:
+            b += 5 * n_live
+        if f & em.DAY_SIA:
+            b += 5 * n_live
+        total += b
+    return total
+
+
+def agent_pass_kernels(flags_t):
+    """Names (as rocprofv3 reports them) of the agent-pass kernels a day with these flags
+    launches once the E / I lists exist."""
+    from laser_polio_amd import engine as em
+
+    out = ["lp::k_ei(lp::Dev, int)"]
+    vd, ri, sia = bool(flags_t & em.DAY_VD_STEP), bool(flags_t & em.DAY_RI), bool(flags_t & em.DAY_SIA)
+    if sia:
+        f = (1 if vd else 0) | (2 if ri else 0) | 4
+        out.append(f"void lp::k_full<{f}, false>(lp::Dev, int)")
+    else:
+        if vd:
+            out.append("void lp::k_full<1, false>(lp::Dev, int)")
+        if ri:
+            out.append("lp::k_ri_list(lp::Dev, int)")
+    return out
+
+
+def window_traffic(flags, t0, t1, per_kernel):
+    """HBM bytes per agent-pass launch over ticks [t0, t1) from the per-kernel bytes per
+    dispatch of the committed rocprofv3 record; None if a kernel of the window is missing."""
+    total = 0.0
+    for t in range(t0, t1):
+        for k in agent_pass_kernels(int(flags[t])):
+            if k not in per_kernel:
+                return None
+            total += per_kernel[k]["fetch"] + per_kernel[k]["write"]
+    return total / (t1 - t0)
+
+
+def national(args):
     import torch
 
-    lp = __graft_entry__.load_package()
+    __graft_entry__.load_package()
     from laser_polio_amd import configs
 
-    nt_needed = 1 + args.warmup + args.steps
+    K, W = args.steps, args.warmup
+    t_begin = max(args.start_day, 1 + W)
+    t_end = t_begin + K
     t_init = time.perf_counter()
-    pars = configs.nigeria_national_pars(scale=args.scale, dur=max(2555, nt_needed - 1))
+    pars = configs.nigeria_national_pars(scale=args.scale, dur=max(2555, t_end - 1))
     if args.tiles_per_block:
         pars.tiles_per_block = args.tiles_per_block
     sim = configs.build(pars)
+    nt = sim.nt
     n0 = int(pars.init_pop.sum())
     log(f"[bench] init: {len(pars.init_pop)} nodes, {n0} agents, capacity {sim.people.capacity}, "
         f"{time.perf_counter() - t_init:.1f}s")
+    parity_day = args.cpu_days + 1 if args.cpu_days > 0 else 0
+    if parity_day and parity_day >= t_begin - W:
+        raise SystemExit(f"--cpu-days {args.cpu_days} must end before the warm-up days (start day {t_begin})")
 
     oracle_sim = None
-    if args.cpu_days > 0:
+    if parity_day:
         from oracle.oracle import OracleSim
 
         oracle_sim = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop)
 
-    sim._timing = not args.graph
     t_up = time.perf_counter()
     eng = sim._ensure_engine()
     torch.cuda.synchronize()
     log(f"[bench] upload {time.perf_counter() - t_up:.1f}s")
     day_flags = eng.arrays["day_flags"]
 
-    eng.run_days(0, 1 + args.warmup, use_graph=bool(args.graph))
-    eng.sync()
-    eng.agent_pass_timing()  # discard warmup events
+    def timed_days(a, b, graph):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run_days(a, b, use_graph=graph)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # tick 0 (initial tally and recount) is initialisation, not a simulated day
+    eng.run_days(0, 1, use_graph=True)
     torch.cuda.synchronize()
-    t_begin, t_end = 1 + args.warmup, 1 + args.warmup + args.steps
-    t0 = time.perf_counter()
-    eng.run_days(t_begin, t_end, use_graph=bool(args.graph))
-    eng.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    run_s = 0.0
+    a = 1
+    parity = None
+    cpu = None
+    if parity_day:
+        run_s += timed_days(1, parity_day + 1, True)
+        eng.sync()
+        sim._mark_device_newer()
+        from oracle import oracle as orc
+
+        threads = int(orc.lib().orc_max_threads())
+        oracle_sim.day(0)
+        tc = time.perf_counter()
+        for t in range(1, args.cpu_days + 1):
+            oracle_sim.day(t)
+        cpu_s = time.perf_counter() - tc
+        cpu_ad = int(oracle_sim.res["pop"][1: args.cpu_days + 1].astype(np.int64).sum())
+        oracle_sim.m.n_threads = 1
+        tc = time.perf_counter()
+        oracle_sim.day(parity_day)
+        one_s = time.perf_counter() - tc
+        one_ad = int(oracle_sim.res["pop"][parity_day].astype(np.int64).sum())
+        cpu = {
+            "value": round(cpu_ad / cpu_s, 1),
+            "unit": "agent-days/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"days 1-{args.cpu_days} of the same {n0}-agent national population ({cpu_s:.1f}s, "
+                      f"OpenMP {threads} threads, host nproc {os.cpu_count()})",
+            "single_thread_value": round(one_ad / one_s, 1),
+            "single_thread_sample": f"day {parity_day} on 1 thread ({one_s:.1f}s)",
+        }
+        log(f"[bench] cpu oracle: {cpu['value']:.4e} agent-days/s over {args.cpu_days} days ({cpu_s:.1f}s); "
+            f"1 thread {cpu['single_thread_value']:.4e}")
+        tc = time.perf_counter()
+        bad = compare_with_oracle(sim, oracle_sim, parity_day)
+        parity = {"oracle_ticks": f"0-{parity_day}", "identical": not bad, "mismatches": bad[:8],
+                  "checked": "every results row, every agent column (whole capacity) and every node's live end"}
+        log(f"[bench] parity vs oracle through day {parity_day}: {'identical' if not bad else bad[:4]} "
+            f"({time.perf_counter() - tc:.1f}s)")
+        del oracle_sim
+        a = parity_day + 1
+    if a < t_begin - W:
+        run_s += timed_days(a, t_begin - W, True)
+    eng.set_timing(not args.graph)
+    run_s += timed_days(t_begin - W, t_begin, bool(args.graph))
+    if not args.graph:
+        eng.agent_pass_timing()  # discard warm-up events
+    elapsed = timed_days(t_begin, t_end, bool(args.graph))
+    run_s += elapsed
     launches, kernel_ms = (0, 0.0) if args.graph else eng.agent_pass_timing()
+    eng.set_timing(False)
+    if t_end < nt:
+        run_s += timed_days(t_end, nt, True)
+    tc = time.perf_counter()
+    eng.sync()     # write-back of the device-held records into the bound columns, and the error check
+    sync_ms = (time.perf_counter() - tc) * 1e3
     sim._mark_device_newer()
-    res = {k: getattr(sim.results, k) for k in ("pop", "E", "I", "I_by_strain", "new_exposed", "ri_protected",
-                                                 "sia_protected", "births", "deaths")}
+    res = {k: np.asarray(getattr(sim.results, k)) for k in ("pop", "E", "I", "I_by_strain", "new_exposed",
+                                                             "ri_protected", "sia_protected", "births", "deaths")}
     agent_days = int(res["pop"][t_begin:t_end].astype(np.int64).sum())
     value = agent_days / elapsed
-    log(f"[bench] {args.steps} days in {elapsed:.3f}s -> {value:.4e} agent-days/s; "
-        f"exposures {int(res['new_exposed'][t_begin:t_end].sum())}")
-
+    full_ad = int(res["pop"][1:nt].astype(np.int64).sum())
+    log(f"[bench] {K} days ({t_begin}-{t_end - 1}) in {elapsed * 1e3:.3f} ms -> {value:.4e} agent-days/s; "
+        f"whole run {nt - 1} days in {run_s:.3f}s -> {full_ad / run_s:.4e}; write-back {sync_ms:.2f} ms")
     if args.dump_days:
-        dump_days(args.dump_days, res, day_flags, t_begin, t_end)
+        dump_days(args.dump_days, res, day_flags, 1, nt)
+
     roofline = None
+    ms_per_step = elapsed * 1e3 / K
     if launches:
         bytes_alg = agent_pass_bytes(res, day_flags, t_begin, t_end)
         achieved = bytes_alg / (kernel_ms / 1e3) / 1e9
         tpw = int(sim.pars.tiles_per_block or 16)
-        tr = measured_traffic(workload_key(args.scale, args.warmup, args.steps, tpw))
+        per_kernel = measured_traffic(args.scale, tpw)
+        traffic = window_traffic(day_flags, t_begin, t_end, per_kernel) if per_kernel else None
+        net_bytes = 4 * len(pars.init_pop) ** 2
+        day_bytes = bytes_alg / K + net_bytes
         roofline = {
             "bound": "hbm",
             "kernel": "agent pass (k_ei + k_full / k_ri_list)",
@@ -219,72 +387,88 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (tr["bytes_per_launch"] if tr else None),
+            "traffic": (round(traffic) if traffic else None),
             "launches": launches,
             "avg_launch_us": round(kernel_ms * 1e3 / launches, 2),
             "bytes_per_launch": int(bytes_alg // launches),
+            "whole_day": {
+                "bytes_per_day": int(day_bytes),
+                "ms_per_day": round(ms_per_step, 4),
+                "achieved": round(day_bytes / (ms_per_step / 1e3) / 1e9, 1),
+                "frac": round(day_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "note": "agent-pass algorithmic bytes plus the network read of the node map, over the whole day",
+            },
+            "survey_byte_model": {
+                "bytes_per_day": int(survey_model_bytes(res, day_flags, t_begin, t_end) / K),
+                "effective_GBs": round(survey_model_bytes(res, day_flags, t_begin, t_end) / elapsed / 1e9, 1),
+                "note": "bandwidth a dense pass over every live agent would need to match this day time",
+            },
         }
-        if tr:
-            roofline["traffic_over_algorithmic"] = round(tr["bytes_per_launch"] / (bytes_alg / launches), 3)
+        if traffic:
+            roofline["traffic_over_algorithmic"] = round(traffic / (bytes_alg / launches), 3)
         log(f"[bench] agent pass: {launches} launches, avg {kernel_ms * 1e3 / launches:.1f} us, {achieved:.0f} GB/s")
 
-    cpu = None
-    if oracle_sim is not None:
-        from oracle import oracle as orc
-
-        days = min(args.cpu_days, sim.nt - 1)
-        oracle_sim.day(0)
-        tc = time.perf_counter()
-        for t in range(1, 1 + days):
-            oracle_sim.day(t)
-        cpu_s = time.perf_counter() - tc
-        cpu_ad = int(oracle_sim.res["pop"][1 : 1 + days].astype(np.int64).sum())
-        threads = int(orc.lib().orc_max_threads())
-        single = None
-        if days + 1 < sim.nt:
-            oracle_sim.m.n_threads = 1
-            tc = time.perf_counter()
-            oracle_sim.day(days + 1)
-            one_s = time.perf_counter() - tc
-            single = {"value": round(int(oracle_sim.res["pop"][days + 1].astype(np.int64).sum()) / one_s, 1), "cores": 1,
-                      "sample": f"day {days + 1} ({one_s:.1f}s)"}
-        cpu = {
-            "value": round(cpu_ad / cpu_s, 1),
-            "unit": "agent-days/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"days 1-{days} of the same {n0}-agent population ({cpu_s:.1f}s, OpenMP {threads} threads, "
-                      f"host nproc {os.cpu_count()})",
-            "single_thread": single,
-        }
-        log(f"[bench] cpu oracle: {cpu['value']:.4e} agent-days/s over {days} days ({cpu_s:.1f}s)")
+    c0 = None
+    if args.cpu_spatial14:
+        c0 = spatial14_cpu()
 
     out = {
         "metric": METRIC,
         "value": round(value, 1),
         "unit": "agent-days/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "i8/u16/i32 state, f32 weights, int64 fixed-point sums",
         "data": "synthetic",
         "config": {
             "workload": f"national: 774 nodes, {n0} agents (population scale {args.scale}), RI + SIA every 180 d, "
-                        f"days {t_begin}-{t_end - 1} of {sim.nt - 1} timed",
+                        f"{nt - 1} days; days {t_begin}-{t_end - 1} timed (days 1-{t_begin - 1} run before them, "
+                        f"the last {W} as warm-up)",
             "nodes": int(len(pars.init_pop)),
             "agents": n0,
             "capacity": int(sim.people.capacity),
             "graph": bool(args.graph),
-            "arithmetic": "int8 state, uint16 timers, f32 weights -> int64 fixed-point sums, f32 node map",
         },
+        "full_run": {"days": nt - 1, "seconds": round(run_s, 4), "agent_days_per_s": round(full_ad / run_s, 1),
+                     "ms_per_day": round(run_s * 1e3 / (nt - 1), 4),
+                     "note": "every simulated day of the run timed (graph replay outside the event-timed window); "
+                             "tick 0 and the day-25 oracle comparison excluded"},
+        "writeback_ms": round(sync_ms, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_spatial14": c0,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
+    if parity is not None and not parity["identical"]:
+        sys.exit(1)
+
+
+def spatial14_cpu():
+    """The CPU oracle on the 14-node spatial workload (4.5e6 agents), all 365 days, on all
+    host threads: the restated reference's agent-days/s at the size it runs on a CPU."""
+    from laser_polio_amd import configs
+    from oracle import oracle as orc
+    from oracle.oracle import OracleSim
+
+    pars = configs.spatial14_pars(scale=1.0)
+    sim = configs.build(pars)
+    o = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop)
+    o.day(0)
+    tc = time.perf_counter()
+    for t in range(1, sim.nt):
+        o.day(t)
+    s = time.perf_counter() - tc
+    ad = int(o.res["pop"][1:].astype(np.int64).sum())
+    threads = int(orc.lib().orc_max_threads())
+    log(f"[bench] cpu oracle, 14-node workload: {ad / s:.4e} agent-days/s ({s:.1f}s, {threads} threads)")
+    return {"value": round(ad / s, 1), "unit": "agent-days/s", "cores": threads, "kind": "port",
+            "sample": f"14-node workload, {int(pars.init_pop.sum())} agents, days 1-{sim.nt - 1} ({s:.1f}s)"}
 
 
 def replicates(args):

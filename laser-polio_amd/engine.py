@@ -224,6 +224,11 @@ class Engine:
         self._check(self.lib.lp_stop_days(self.ctx, out, self.cfg.n_sims))
         return np.array(list(out), dtype=np.int32)
 
+    def set_timing(self, on):
+        """On: lp_run_days launches each day's kernels directly and brackets its agent pass
+        with HIP events (read by agent_pass_timing); off: days replay their hipGraphs."""
+        self._check(self.lib.lp_set_timing(self.ctx, 1 if on else 0))
+
     def agent_pass_timing(self):
         n = ctypes.c_int64()
         ms = ctypes.c_double()
