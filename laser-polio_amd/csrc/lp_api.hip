@@ -188,13 +188,6 @@ int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
          ((f & LP_DAY_SIA) ? lp::FULL_SIA : 0);
 }
 
-// FULL-from-list day kinds whose RI runs from the candidate lists (RI without SIA).
-bool sparse_ri(int kind) {
-  if (kind < lp::KIND_FULL_LIST) return false;
-  const int f = kind - lp::KIND_FULL_LIST + 1;
-  return (f & lp::FULL_RI) && !(f & lp::FULL_SIA);
-}
-
 // Agent pass of one day: k_plain over U = tiles per span on plain days,
 // k_full<kind> on FULL days, k_agents in groups of 2 tiles when recounting.
 template <int U>
@@ -213,23 +206,16 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
     case lp::KIND_FULL_LIST + 0: case lp::KIND_FULL_LIST + 1: case lp::KIND_FULL_LIST + 2:
     case lp::KIND_FULL_LIST + 3: case lp::KIND_FULL_LIST + 4: case lp::KIND_FULL_LIST + 5:
     case lp::KIND_FULL_LIST + 6: {
+      // DiseaseState from the E / I lists, then births and deaths (only the chunks with a
+      // death due or newborn slots), RI from the candidate lists, SIA over the targeted
+      // nodes: each stage a pass of its own, in the reference's order, all tallying
+      // into the per-node accumulators
       const int f = kind - lp::KIND_FULL_LIST + 1;
-      const bool ri_sparse = sparse_ri(kind);
-      const int fd = ri_sparse ? (f & lp::FULL_VD) : f;   // F of the dense pass (0: none)
       hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask);
-      // with RI from the lists the VD step runs alone (RI masked out of its day flags)
-      const int fm = ri_sparse ? mask & ~LP_STAGE_RI : mask;
-      switch (fd) {
-        case 0: break;
-        case 1: hipLaunchKernelGGL((lp::k_full<1, false>), g, b, 0, s, c->d, fm); break;
-        case 2: hipLaunchKernelGGL((lp::k_full<2, false>), g, b, 0, s, c->d, mask); break;
-        case 3: hipLaunchKernelGGL((lp::k_full<3, false>), g, b, 0, s, c->d, mask); break;
-        case 4: hipLaunchKernelGGL((lp::k_full<4, false>), g, b, 0, s, c->d, mask); break;
-        case 5: hipLaunchKernelGGL((lp::k_full<5, false>), g, b, 0, s, c->d, mask); break;
-        case 6: hipLaunchKernelGGL((lp::k_full<6, false>), g, b, 0, s, c->d, mask); break;
-        default: hipLaunchKernelGGL((lp::k_full<7, false>), g, b, 0, s, c->d, mask); break;
-      }
-      if (ri_sparse) hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);
+      if (f & lp::FULL_VD)
+        hipLaunchKernelGGL(lp::k_vd, dim3(c->cfg.n_nodes + grid_of(c->n_spans, lp::WG / 64)), b, 0, s, c->d, mask);
+      if (f & lp::FULL_RI) hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);
+      if (f & lp::FULL_SIA) hipLaunchKernelGGL(lp::k_sia, g, b, 0, s, c->d, mask);
       break;
     }
     default: hipLaunchKernelGGL((lp::k_agents<2, lp::KIND_RECOUNT>), g, b, 0, s, c->d, mask); break;
@@ -308,10 +294,10 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
     const bool ri = host_stage_flags(c->day_flags_h[t], mask) & LP_DAY_RI;
     hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, s, d, ri ? t : -2);
   }
-  // span partials: none written on list-only days (plain, RI-only from the lists)
-  const bool spans_written = kind != lp::KIND_PLAIN_LIST && kind != lp::KIND_FULL_LIST + lp::FULL_RI - 1;
+  // list days tally into node_acc (span partials hold only the carried span weights);
+  // the dense passes of the other kinds write every span's partial
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
-                     kind < lp::KIND_PLAIN_LIST ? 0 : spans_written ? 2 : 1);
+                     kind < lp::KIND_PLAIN_LIST ? 0 : 1);
   if (mask & LP_STAGE_TRANSMISSION) {
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
     // with the probe offsets in LDS k_sample plans round 1 itself

@@ -1609,6 +1609,407 @@ __global__ void __launch_bounds__(WG, 5) k_full(Dev d, int mask) {
 }
 
 // ---------------------------------------------------------------------------
+// VD step days after k_ei, without RI or SIA in the pass (get_deaths and the
+// newborns of VitalDynamics_ABM.step, model.py:1711-1734, 1767-1781).  Only chunks
+// with a death due today (chunk_min_dod <= t) or newborn slots have work, a few
+// thousand of the 1.2 million chunks of the national population, so the pass is
+// chunk-granular instead of one workgroup per span:
+//   workgroups [0, n_nodes): node n's newborn chunks (those overlapping
+//     [nb_begin, live_end)), births and the deaths due in them, the Kaplan-Meier
+//     table of the node's simulation in LDS; the newborn slots join the RI candidates;
+//   the others: one wavefront per span, lane l checks chunk l's earliest death date,
+//     the wave then takes the span's chunks with a death due (and no newborn slot).
+// Each chunk belongs to one wavefront, so its state word, chunk weight and earliest
+// death date are updated without atomics; span weights (partials[P_SW]) by atomic
+// deltas; the per-node tallies (deaths, their paralysis flags, S / S with positive
+// weight / R deltas) by one atomic per field and wavefront into node_acc, which
+// k_node_reduce reads as on list-only days.
+enum : int { VA_DEATHS = 0, VA_DPP, VA_DP, VA_S, VA_SPOS, VA_R, VA_N };
+
+// One chunk of node `node` (one wavefront, lane l slots c*256 + 4l .. +3): deaths
+// due today, and births in [nb_lo, live_hi) when BIRTHS.  Adds to the lane tallies
+// acc[]; returns nothing (the chunk's words, weights and death date are stored).
+template <bool BIRTHS>
+__device__ __forceinline__ void vd_chunk(const Dev& d, int64_t c, int node, int64_t nb_lo, int64_t live_hi, int32_t cm,
+                                         int32_t t, int32_t sh_prev, const lp_sim_params& sp, const int64_t* km,
+                                         int32_t* acc, int lane) {
+  const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
+  const bool live_q = i0 < live_hi;
+  const uint32_t w0 = live_q ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+  int4 ddu = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+  if (cm <= t && live_q) ddu = *reinterpret_cast<const int4*>(d.dod + i0);
+  uint32_t w1 = w0;
+  int32_t dmin = INT_MAX;
+  int64_t cw = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + a;
+    if (i >= live_hi) continue;
+    int8_t s = byte_of(w0, a);
+    if (BIRTHS && i >= nb_lo) {                           // newborn (model.py:1719-1732)
+      const uint64_t slot = (uint64_t)(i - sp.slot_base);
+      const int32_t dod = t + km_age_at_death(km, sp.seed, slot, t, 0);
+      d.dob[i] = t;
+      d.dod[i] = dod;
+      dmin = dod < dmin ? dod : dmin;
+      d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
+      const int16_t base = sp.newborn_ri_timer >= 0 ? (int16_t)sp.newborn_ri_timer : d.ri_timer[i];
+      const int16_t col = (int16_t)(base + sh_prev);     // as of the reference tick
+      if (sp.newborn_ri_timer >= 0 || col != base) d.ri_timer[i] = col;
+      const int64_t w = weight_q24(d.risk[i]);           // a newborn enters S
+      acc[VA_S] += 1; acc[VA_SPOS] += (w > 0); cw += w;
+      s = 0;
+    } else if (s >= 0 && lane_of(ddu, a) <= t) {         // get_deaths (model.py:1775-1777)
+      acc[VA_DEATHS] += 1;
+      const uint4 rd = d.rec[i];
+      if ((int8_t)(rd.y >> 24) == 1) acc[VA_DPP] += 1;
+      if ((int8_t)(rd.w >> 8) == 1) acc[VA_DP] += 1;
+      // its ri_timer stops here: the actual value (chronically missed agents are never decremented)
+      if (sh_prev != 0 && (int8_t)d.missed[i] != 1) d.ri_timer[i] = (int16_t)(d.ri_timer[i] - sh_prev);
+      if (s == 0) {
+        const int64_t w = weight_q24(d.risk[i]);
+        acc[VA_S] -= 1; acc[VA_SPOS] -= (w > 0); cw -= w;
+      } else if (s == 3) {
+        acc[VA_R] -= 1;
+      }
+      s = -1;
+    } else if (s >= 0) {
+      const int32_t dod = lane_of(ddu, a);
+      dmin = dod < dmin ? dod : dmin;
+    }
+    w1 = (w1 & ~(0xffu << (8 * a))) | ((uint32_t)(uint8_t)s << (8 * a));
+  }
+  if (live_q && w1 != w0) reinterpret_cast<uint32_t*>(d.state)[i0 >> 2] = w1;
+  // the chunk's earliest death date: exact when its dates were read, else lowered by newborns
+  const int32_t wm = wave_min_i32(dmin);
+  const int32_t nm = cm <= t ? wm : (wm < cm ? wm : cm);
+  const int64_t cs = wave_sum_i64(cw);
+  if (lane == 0) {
+    if (nm != cm) d.chunk_min_dod[c] = nm;
+    if (cs != 0) {
+      d.chunk_sum[c] += cs;
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)span_of(d, node, c * LP_CHUNK) * NPART + P_SW),
+                (unsigned long long)cs);
+    }
+  }
+}
+
+// The wave's tallies of node `node` -> node_acc (one atomic per nonzero field).
+__device__ __forceinline__ void vd_flush(const Dev& d, int node, int32_t* acc, int lane) {
+  int32_t v[VA_N];
+#pragma unroll
+  for (int f = 0; f < VA_N; ++f) v[f] = wave_sum_i32(acc[f]);
+  if (lane < VA_N) {
+    const int f = lane;
+    const int32_t x = f == 0 ? v[0] : f == 1 ? v[1] : f == 2 ? v[2] : f == 3 ? v[3] : f == 4 ? v[4] : v[5];
+    const int pf = f == VA_DEATHS ? P_DEATHS : f == VA_DPP ? P_DPP : f == VA_DP ? P_DP : f == VA_S ? P_S
+                 : f == VA_SPOS ? P_SPOS : P_R;
+    if (x != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.node_acc + (size_t)node * NPART + pf),
+                (unsigned long long)(int64_t)x);
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_vd(Dev d, int mask) {
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int N = d.n_nodes;
+  const int32_t flags = stage_day_flags(dy.y, mask);
+  const int32_t cnt_ref = d.ri_cnt[d.ri_ref[0] + 1];
+  int32_t acc[VA_N] = {0, 0, 0, 0, 0, 0};
+  if ((int)blockIdx.x < N) {
+    // ---- node n's newborn chunks ----
+    const int n = blockIdx.x;
+    const int sim = d.node_sim[n];
+    const int64_t nb_lo = d.nb_begin[n], live_hi = d.live_end[n];
+    __shared__ int64_t s_km[LP_KM_LEN];
+    if (nb_lo >= live_hi || sim_stopped(d, sim, t) || !(flags & LP_DAY_VD_STEP)) return;
+    const lp_sim_params& sp = d.sims[sim];
+    if (tid < LP_KM_LEN) s_km[tid] = d.km_cd[(size_t)sim * LP_KM_LEN + tid];
+    const int32_t sh_prev = sp.ri_step * (dy.z - cnt_ref);
+    {
+      // the newborn slots join the candidates of the next RI day (today's, if today is one:
+      // births precede RI)
+      __shared__ int32_t s_base;
+      const int b = dy.z & 1;
+      if (tid == 0) s_base = atomicAdd(ri_cnt_of(d, b, n), (int32_t)(live_hi - nb_lo));
+      __syncthreads();
+      uint32_t* L = ri_list_of(d, b, n);
+      for (int64_t x = nb_lo + tid; x < live_hi; x += WG) L[s_base + (x - nb_lo)] = (uint32_t)x;
+    }
+    const int64_t c_lo = nb_lo / LP_CHUNK, c_hi = (live_hi + LP_CHUNK - 1) / LP_CHUNK;
+    for (int64_t c = c_lo + (tid >> 6); c < c_hi; c += WG / 64) {
+      const int32_t cm = d.chunk_min_dod[c];
+      vd_chunk<true>(d, c, n, nb_lo, live_hi, cm, t, sh_prev, sp, s_km, acc, lane);
+    }
+    vd_flush(d, n, acc, lane);
+    return;
+  }
+  // ---- one wavefront per span: chunks with a death due ----
+  const int64_t s = ((int64_t)blockIdx.x - N) * (WG / 64) + (tid >> 6);
+  if (s >= d.n_spans) return;
+  const int4 span = d.spans[s];
+  const int node = span.x, sim = span.w;
+  const int64_t c0 = (int64_t)span.y * (LP_TILE / LP_CHUNK);
+  const int nc = span.z * (LP_TILE / LP_CHUNK);
+  const int32_t cm = lane < nc ? d.chunk_min_dod[c0 + lane] : INT_MAX;
+  const int64_t nb_lo = d.nb_begin[node], live_hi = d.live_end[node];
+  if (sim_stopped(d, sim, t)) return;
+  if (!(flags & LP_DAY_VD_STEP)) { if (lane == 0) atomicOr(d.err, ERR_KIND); return; }
+  // chunks overlapping today's newborn slots belong to the node's workgroup above
+  const int64_t c = c0 + lane;
+  const bool nb_chunk = nb_lo < live_hi && c * LP_CHUNK < live_hi && (c + 1) * LP_CHUNK > nb_lo;
+  uint64_t due = __ballot(lane < nc && cm <= t && !nb_chunk);
+  if (due == 0) return;
+  const lp_sim_params& sp = d.sims[sim];
+  const int32_t sh_prev = sp.ri_step * (dy.z - cnt_ref);
+  while (due) {
+    const int l = __ffsll((unsigned long long)due) - 1;
+    due &= due - 1;
+    vd_chunk<false>(d, c0 + l, node, nb_lo, live_hi, __shfl(cm, l, 64), t, sh_prev, sp, nullptr, acc, lane);
+  }
+  vd_flush(d, node, acc, lane);
+}
+
+// ---------------------------------------------------------------------------
+// SIA campaign days after k_ei (and k_vd / k_ri_list when VitalDynamics and RI
+// step the same day: births, deaths and RI precede SIA, model.py:226-241): fast_sia
+// (model.py:1994-2060) per event, over the agents of the nodes an event targets.
+// One workgroup per span; a span of a node no event targets returns at once.  Each
+// lane takes its quad of each of the span's tiles in turn, with the state,
+// chronically_missed and date_of_birth words of the next two tiles already loading
+// (a rolled loop: one copy of the per-agent code, whose draws are the bulk of the
+// instructions); live, reached agents of an event's age band get one draw per event
+// (the day's events in LDS): vaccinated counts, susceptibles exposed with the
+// event's vaccine strain.  Tallies per workgroup into node_acc (as k_vd), chunk
+// weights per wavefront, the span weight by one atomic; exposed agents join
+// tomorrow's E / I list.
+constexpr int SIA_LDS_EV = 64;   // events of one day held in LDS (more: read from global memory)
+constexpr int SIA_XG = 4;        // tiles whose exposures are queued in LDS and finished together
+
+__global__ void __launch_bounds__(WG) k_sia(Dev d, int mask) {
+  const int4 span = d.spans[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int4 dy = today(d);
+  const int32_t t = dy.x;
+  const int node = span.x, sim = span.w;
+  const int64_t live_hi = d.live_end[node];
+  const int32_t stop_day = d.stop_day[sim];
+  const int32_t flags = stage_day_flags(dy.y, mask);
+  if (stop_day < t) return;
+  if (!(flags & LP_DAY_SIA)) { if (tid == 0) atomicOr(d.err, ERR_KIND); return; }
+  const int32_t sia_b = d.day_sia_begin[t], sia_n = d.day_sia_count[t];
+  __shared__ uint64_t e_thr[SIA_LDS_EV][2];
+  __shared__ int32_t e_lo[SIA_LDS_EV], e_hi[SIA_LDS_EV], e_k[SIA_LDS_EV];
+  __shared__ int s_here;
+  if (tid == 0) s_here = 0;
+  __syncthreads();
+  for (int e = tid; e < sia_n; e += WG) {
+    const int ev = sia_b + e;
+    const uint64_t t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
+    if (t1 != 0) s_here = 1;
+    if (e < SIA_LDS_EV) {
+      e_thr[e][0] = t1;
+      e_thr[e][1] = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1];
+      e_lo[e] = d.sia_min_age[ev];
+      e_hi[e] = d.sia_max_age[ev];
+      e_k[e] = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
+    }
+  }
+  __syncthreads();
+  if (!s_here || (int64_t)span.y * LP_TILE >= live_hi) return;
+  const lp_sim_params& sp = d.sims[sim];
+  const uint64_t seed = sp.seed;
+  const int64_t sbase = sp.slot_base;
+  float r0k[LP_MAX_STRAINS];
+#pragma unroll
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) r0k[k] = sp.strain_r0[k];
+  const bool do_tally = (flags & LP_DAY_TX) != 0;
+  const int nxt = (t + 1) & 1;
+  // lane tallies: vaccinated / protected by the day's last event (the reference's result
+  // rows keep the last event's counts), exposures by strain, S and S-with-weight deltas
+  int32_t c_v = 0, c_p = 0, c_s = 0, c_spos = 0;
+  int32_t c_k[LP_MAX_STRAINS] = {0, 0, 0, 0};
+  int64_t c_sw = 0;
+  // the words of tiles u (cur), u + 1 and u + 2 (in flight)
+  auto load = [&](int u, uint32_t& w, uint32_t& m, int4& b) {
+    const int64_t i0 = ((int64_t)span.y + u) * LP_TILE + (int64_t)tid * 4;
+    const bool v = u < span.z && i0 < live_hi;
+    w = v ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
+    m = v ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0x01010101u;
+    b = v ? *reinterpret_cast<const int4*>(d.dob + i0) : make_int4(0, 0, 0, 0);
+  };
+  // the widest age band of the events targeting the node: agents outside it are skipped
+  int32_t band_lo = INT_MAX, band_hi = INT_MIN;
+  for (int e = 0; e < sia_n; ++e) {
+    const bool on = e < SIA_LDS_EV ? e_thr[e][0] != 0
+                                   : d.sia_thr[((size_t)(sia_b + e) * d.n_nodes + node) * 2] != 0;
+    if (!on) continue;
+    const int32_t lo = e < SIA_LDS_EV ? e_lo[e] : d.sia_min_age[sia_b + e];
+    const int32_t hi = e < SIA_LDS_EV ? e_hi[e] : d.sia_max_age[sia_b + e];
+    band_lo = lo < band_lo ? lo : band_lo;
+    band_hi = hi > band_hi ? hi : band_hi;
+  }
+  // per wave: the tile's state words and birth dates, and the compacted list of its
+  // agents in the band (quad << 2 | agent), so that each round of draws serves 64 of them
+  __shared__ uint32_t c_w[WG / 64][64];
+  __shared__ int4 c_db[WG / 64][64];
+  __shared__ uint16_t c_list[WG / 64][256];
+  // the exposures of up to SIA_XG tiles ((slot - first slot of the span) << 2 | strain),
+  // finished together by the workgroup: their loads in flight at once, one reservation
+  // in tomorrow's list; the span's chunk weight deltas accumulate in LDS
+  __shared__ uint32_t x_buf[SIA_XG * LP_TILE];
+  __shared__ int32_t x_cnt, x_base;
+  __shared__ int64_t x_cw[MAX_TPW * (LP_TILE / LP_CHUNK)];
+  const int64_t span_lo = (int64_t)span.y * LP_TILE;
+  if (tid == 0) x_cnt = 0;
+  if (tid < MAX_TPW * (LP_TILE / LP_CHUNK)) x_cw[tid] = 0;
+  __syncthreads();
+  auto finish = [&]() {
+    __syncthreads();
+    const int n = x_cnt;
+    if (n > 0) {
+      if (tid == 0) x_base = atomicAdd(d.ei_cnt + (size_t)nxt * d.n_nodes + node, n);
+      __syncthreads();
+      uint4* L = ei_list_of(d, nxt, node) + x_base;
+      for (int j = tid; j < n; j += WG) {
+        const uint32_t code = x_buf[j];
+        const int64_t i = span_lo + (int64_t)(code >> 2);
+        const int k = (int)(code & 3u);
+        const float rk = d.risk[i];
+        const uint4 rec = d.rec[i];
+        d.state[i] = 1;                                   // vaccinated out of S, counted as E today
+        *rec_byte(d, i, RB_STRAIN) = (int8_t)k;
+        const int64_t w = weight_q24(rk);
+        c_s -= 1; c_spos -= (w > 0); c_sw -= w;
+        if (w != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&x_cw[(i - span_lo) >> 8]), (unsigned long long)(-w));
+        L[j] = exposed_entry_rec(i, rec, k, t, r0_sel(r0k, k));
+      }
+    }
+    __syncthreads();
+    if (tid == 0) x_cnt = 0;
+    __syncthreads();
+  };
+  uint32_t w_a, m_a, w_b, m_b, w_c, m_c;
+  int4 b_a, b_b, b_c;
+  load(0, w_a, m_a, b_a);
+  load(1, w_b, m_b, b_b);
+#pragma unroll 1
+  for (int u = 0; u < span.z; ++u) {
+    load(u + 2, w_c, m_c, b_c);
+    const int64_t tile = (int64_t)span.y + u;
+    const int64_t i0 = tile * LP_TILE + (int64_t)tid * 4;
+    uint32_t elig = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int32_t age = t - lane_of(b_a, a);
+      if (i0 + a < live_hi && byte_of(w_a, a) >= 0 && byte_of(m_a, a) != 1 && age >= band_lo && age <= band_hi)
+        elig |= 1u << a;
+    }
+    int tot = 0;
+    {
+      const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const uint64_t mk = __ballot((elig >> a) & 1u);
+        if ((elig >> a) & 1u) c_list[wave][tot + __popcll(mk & lt)] = (uint16_t)((lane << 2) | a);
+        tot += __popcll(mk);
+      }
+    }
+    c_w[wave][lane] = w_a;
+    c_db[wave][lane] = b_a;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int base = 0; base < tot; base += 64) {
+      const int j = base + lane;
+      bool ex = false;
+      int k_ex = 0;
+      uint32_t off = 0;
+      if (j < tot) {
+        const int code = c_list[wave][j];
+        const int l = code >> 2, a = code & 3;
+        off = (uint32_t)(u * LP_TILE + (wave * 64 + l) * 4 + a);   // lane l of this wave owns quad wave * 64 + l
+        int8_t s = byte_of(c_w[wave][l], a);
+        const int32_t age = t - lane_of(c_db[wave][l], a);
+        const uint64_t slot = (uint64_t)(span_lo + off - sbase);
+        for (int e = 0; e < sia_n; ++e) {                 // in the day's order
+          uint64_t t1, t2;
+          int32_t lo, hi, k;
+          if (e < SIA_LDS_EV) {
+            lo = e_lo[e]; hi = e_hi[e];
+            if (!(lo <= age && age <= hi)) continue;
+            t1 = e_thr[e][0]; t2 = e_thr[e][1]; k = e_k[e];
+          } else {
+            const int ev = sia_b + e;
+            lo = d.sia_min_age[ev]; hi = d.sia_max_age[ev];
+            if (!(lo <= age && age <= hi)) continue;
+            t1 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2];
+            t2 = d.sia_thr[((size_t)ev * d.n_nodes + node) * 2 + 1];
+            k = d.sia_strain[ev] & (LP_MAX_STRAINS - 1);
+          }
+          if (t1 == 0) continue;
+          const uint4 o = draw(seed, slot, t, ST_SIA | ((uint32_t)e << 8));
+          if ((uint64_t)o.x < t1) {
+            const bool last = e == sia_n - 1;
+            if (last) c_v += 1;
+            if (s == 0 && (uint64_t)o.x < t2) {
+              s = 1;
+              ex = true;
+              k_ex = k;
+              c_k[0] += k == 0; c_k[1] += k == 1; c_k[2] += k == 2; c_k[3] += k == 3;
+              if (last) c_p += 1;
+            }
+          }
+        }
+      }
+      // queue the round's exposures: one LDS reservation per wave
+      const uint64_t mk = __ballot(ex);
+      if (mk) {
+        int b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&x_cnt, __popcll(mk));
+        b0 = __shfl(b0, 0, 64);
+        if (ex) x_buf[b0 + __popcll(mk & ((1ull << lane) - 1ull))] = (off << 2) | (uint32_t)k_ex;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();   // the wave's LDS rows are reused for the next tile
+    if ((u + 1) % SIA_XG == 0 || u + 1 == span.z) finish();   // uniform
+    w_a = w_b; m_a = m_b; b_a = b_b;
+    w_b = w_c; m_b = m_c; b_b = b_c;
+  }
+  // the span's chunk weights
+  if (tid < span.z * (LP_TILE / LP_CHUNK) && x_cw[tid] != 0) d.chunk_sum[(int64_t)span.y * (LP_TILE / LP_CHUNK) + tid] += x_cw[tid];
+  // the workgroup's tallies -> node_acc, the span's weight delta -> its partial
+  __shared__ int64_t red[WG / 64][9];
+  {
+    int64_t v[9] = {c_v, c_p, c_s, c_spos, c_sw, c_k[0], c_k[1], c_k[2], c_k[3]};
+#pragma unroll
+    for (int f = 0; f < 9; ++f) v[f] = wave_sum_i64(v[f]);
+    if (lane == 0) {
+#pragma unroll
+      for (int f = 0; f < 9; ++f) red[wave][f] = v[f];
+    }
+  }
+  __syncthreads();
+  if (tid < 13) {
+    int64_t x = 0;
+    int f = -1;
+    auto sum = [&](int q) { int64_t s = 0; for (int w = 0; w < WG / 64; ++w) s += red[w][q]; return s; };
+    if (tid == 0) { f = P_SIAV; x = sum(0); }
+    else if (tid == 1) { f = P_SIAP; x = sum(1); }
+    else if (tid == 2) { f = P_S; x = sum(2); }
+    else if (tid == 3) { f = P_SPOS; x = sum(3); }
+    else if (tid < 8) { const int k = tid - 4; if (k < d.K) { f = P_SIAPK0 + k; x = sum(5 + k); } }
+    else if (tid < 12) { const int k = tid - 8; if (k < d.K && do_tally) { f = P_E0 + k; x = sum(5 + k); } }
+    else { x = sum(4); if (x != 0) atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)blockIdx.x * NPART + P_SW),
+                                              (unsigned long long)x); }
+    if (f >= 0 && x != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.node_acc + (size_t)node * NPART + f), (unsigned long long)x);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // RECOUNT passes: the span's tiles in groups of G, each group through phases
 // A-D (every agent's risk, paralysis flags and state recounted); tallies
 // accumulate over the groups, one partial per span.
@@ -2008,30 +2409,49 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   const int32_t flags = stage_day_flags(dy.y, mask);
   __shared__ int64_t tally[NPART];
   __shared__ int64_t grp[WG / NPART][NPART];
-  {
-    // thread (f = tid % 32, g = tid / 32) sums field f of spans g, g+8, ...: coalesced
-    const int f = tid % NPART, g = tid / NPART;
-    int64_t acc = 0;
-    constexpr int G = WG / NPART;
-    for (int s = s0 + g; s < s1; s += 4 * G) {
-      const int64_t a0 = d.partials[(size_t)s * NPART + f];
-      const int64_t a1 = s + G < s1 ? d.partials[(size_t)(s + G) * NPART + f] : 0;
-      const int64_t a2 = s + 2 * G < s1 ? d.partials[(size_t)(s + 2 * G) * NPART + f] : 0;
-      const int64_t a3 = s + 3 * G < s1 ? d.partials[(size_t)(s + 3 * G) * NPART + f] : 0;
-      acc += (a0 + a1) + (a2 + a3);
+  __shared__ int64_t s_w;
+  // list days: the day's tallies are the accumulators of the list-day passes (k_ei,
+  // k_vd, k_ri_list, k_sia), cleared here for the next day; the span partials hold only
+  // the carried span weights, whose exclusive prefix (for the sampler) and total are
+  // taken here.  Other days: every span's partial is reduced.
+  if (list) {
+    if (lane_spans) {
+      if (tid < 64) {
+        const int64_t incl = wave_incl_scan_i64(psw, tid);
+        if (s0 + tid < s1) d.span_prefix[s0 + tid] = incl - psw;
+        if (tid == 63) s_w = incl;
+      }
+    } else {
+      const int64_t W = node_span_scan<false>(d.partials, d.span_prefix, s0, s1);
+      if (tid == 0) s_w = W;
     }
-    grp[g][f] = acc;
-  }
-  __syncthreads();
-  if (tid < NPART) {
-    int64_t acc = 0;
+    __syncthreads();
+    if (tid < NPART) {
+      d.node_acc[(size_t)n * NPART + tid] = 0;
+      tally[tid] = tid == P_SW ? s_w : ea;
+    }
+  } else {
+    {
+      // thread (f = tid % 32, g = tid / 32) sums field f of spans g, g+8, ...: coalesced
+      const int f = tid % NPART, g = tid / NPART;
+      int64_t acc = 0;
+      constexpr int G = WG / NPART;
+      for (int s = s0 + g; s < s1; s += 4 * G) {
+        const int64_t a0 = d.partials[(size_t)s * NPART + f];
+        const int64_t a1 = s + G < s1 ? d.partials[(size_t)(s + G) * NPART + f] : 0;
+        const int64_t a2 = s + 2 * G < s1 ? d.partials[(size_t)(s + 2 * G) * NPART + f] : 0;
+        const int64_t a3 = s + 3 * G < s1 ? d.partials[(size_t)(s + 3 * G) * NPART + f] : 0;
+        acc += (a0 + a1) + (a2 + a3);
+      }
+      grp[g][f] = acc;
+    }
+    __syncthreads();
+    if (tid < NPART) {
+      int64_t acc = 0;
 #pragma unroll
-    for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
-    // list 1: k_ei alone (only the carried S weights come from the span partials);
-    // list 2: k_ei and a FULL-day pass (both add up).  The accumulators are cleared
-    // for the next day's k_ei.
-    if (list) d.node_acc[(size_t)n * NPART + tid] = 0;
-    tally[tid] = list == 1 ? (tid == P_SW ? acc : ea) : acc + ea;
+      for (int g = 0; g < WG / NPART; ++g) acc += grp[g][tid];
+      tally[tid] = acc;
+    }
   }
   __syncthreads();
   if (tid < 3) {
@@ -2047,7 +2467,9 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   if (tid < LP_MAX_STRAINS)   // infectious weight per strain as the network mix reads it
     d.beta_f[(size_t)n * LP_MAX_STRAINS + tid] = tid < d.K ? (float)tally[P_IB0 + tid] * 5.9604644775390625e-08f : 0.0f;
   if (flags & LP_DAY_TX) {
-    if (lane_spans) {
+    if (list) {
+      if (tid == 0) d.W_round[n] = s_w;            // the prefix was written above
+    } else if (lane_spans) {
       if (tid < 64) {
         const int64_t incl = wave_incl_scan_i64(psw, tid);
         if (s0 + tid < s1) d.span_prefix[s0 + tid] = incl - psw;
