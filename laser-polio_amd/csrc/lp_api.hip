@@ -188,6 +188,12 @@ int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
          ((f & LP_DAY_SIA) ? lp::FULL_SIA : 0);
 }
 
+// k_ei of one day; above EI_LDS_NODES nodes its piece plan comes from k_ei_plan.
+void launch_ei(lp_ctx* c, hipStream_t s, int mask) {
+  if (c->cfg.n_nodes > lp::EI_LDS_NODES) hipLaunchKernelGGL(lp::k_ei_plan, dim3(1), dim3(lp::WG), 0, s, c->d);
+  hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), dim3(lp::WG), 0, s, c->d, mask);
+}
+
 // Agent pass of one day: k_plain over U = tiles per span on plain days,
 // k_full<kind> on FULL days, k_agents in groups of 2 tiles when recounting.
 template <int U>
@@ -195,7 +201,7 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
   const dim3 g(c->n_spans), b(lp::WG);
   switch (kind) {
     case lp::KIND_PLAIN: hipLaunchKernelGGL((lp::k_plain<U>), g, b, 0, s, c->d, mask); break;
-    case lp::KIND_PLAIN_LIST: hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask); break;
+    case lp::KIND_PLAIN_LIST: launch_ei(c, s, mask); break;
     case 1: hipLaunchKernelGGL((lp::k_full<1, true>), g, b, 0, s, c->d, mask); break;
     case 2: hipLaunchKernelGGL((lp::k_full<2, true>), g, b, 0, s, c->d, mask); break;
     case 3: hipLaunchKernelGGL((lp::k_full<3, true>), g, b, 0, s, c->d, mask); break;
@@ -211,7 +217,7 @@ void launch_agents_u(lp_ctx* c, hipStream_t s, int kind, int mask) {
       // nodes: each stage a pass of its own, in the reference's order, all tallying
       // into the per-node accumulators
       const int f = kind - lp::KIND_FULL_LIST + 1;
-      hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), b, 0, s, c->d, mask);
+      launch_ei(c, s, mask);
       if (f & lp::FULL_VD)
         hipLaunchKernelGGL(lp::k_vd, dim3(c->cfg.n_nodes + grid_of(c->n_spans, lp::WG / 64)), b, 0, s, c->d, mask);
       if (f & lp::FULL_RI) hipLaunchKernelGGL(lp::k_ri_list, dim3(c->cfg.n_nodes, kRiSplit), b, 0, s, c->d, mask);

@@ -213,6 +213,12 @@ __device__ void day_begin_node(const Dev& d, int n, int32_t t, int mask) {
   }
 }
 
+// sim_cases[sim]: 1 when a node of the sim logged E or I on the previous day (set by
+// k_day_end), 0 when none did; CASES_CHECKED when today's early-stop rule has run
+// already (k_day_begin, which sums the rows).  k_node_reduce runs the rule for
+// days begun inside lp_run_days and clears the flag for today's log.
+constexpr int32_t CASES_CHECKED = 2;
+
 // Per sim: the early-stop rule (model.py:788-795) on the previous day's E and I;
 // `cases` < 0: sum the previous day's rows, else whether any was positive.
 __device__ void day_stop_check(const Dev& d, int sim, int32_t t, int mask, int cases) {
@@ -240,7 +246,7 @@ __global__ void k_day_begin(Dev d, int mask) {
   const int sim = d.node_sim[n];
   if (n == d.sims[sim].node_begin) {
     day_stop_check(d, sim, t, mask, -1);
-    d.sim_cases[sim] = 0;
+    d.sim_cases[sim] = CASES_CHECKED;
   }
 }
 
@@ -878,6 +884,44 @@ __global__ void __launch_bounds__(WG) k_plain(Dev d, int mask) {
 // them in place of the span partials.  Cost: proportional to the E and I
 // agents, not to the population.
 
+// Exclusive prefix over n in [0, N) of cnt(n), by every thread of one workgroup:
+// emit(n, prefix) for each n; returns the total.  Each pass takes 4 * WG nodes, 4
+// consecutive ones per thread with their loads in flight together.
+template <typename Cnt, typename Emit>
+__device__ int64_t block_scan_nodes(int N, Cnt cnt, Emit emit) {
+  __shared__ int64_t s_w[WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int64_t carry = 0;
+  for (int base = 0; base < N; base += 4 * WG) {
+    int64_t v[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int n = base + 4 * tid + a;
+      v[a] = n < N ? cnt(n) : 0;
+    }
+    const int64_t sum = v[0] + v[1] + v[2] + v[3];
+    const int64_t incl = wave_incl_scan_i64(sum, lane);
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    int64_t off = carry + incl - sum, tot = 0;
+#pragma unroll
+    for (int j = 0; j < WG / 64; ++j) {
+      const int64_t x = s_w[j];
+      off += j < w ? x : 0;
+      tot += x;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int n = base + 4 * tid + a;
+      if (n < N) emit(n, off);
+      off += v[a];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  return carry;
+}
+
 // Largest n in [0, N) with off[n] <= p (off[0] = 0 <= p): 64-ary search, all lanes.
 __device__ __forceinline__ int find_piece_node(const int32_t* off, int N, int32_t p, int lane) {
   int lo = 0, hi = N;
@@ -908,11 +952,14 @@ __global__ void __launch_bounds__(WG) k_ei(Dev d, int mask) {
   const bool lds = N <= EI_LDS_NODES;
   int32_t total;
   if (lds) {
-    for (int x = threadIdx.x; x <= N; x += WG) s_off[x] = d.ei_piece_off[x];
+    // every workgroup scans the nodes' piece counts of today's list itself
+    const int32_t* cnt = d.ei_cnt + (size_t)cur * N;
+    total = (int32_t)block_scan_nodes(
+        N, [&](int n) { return (int64_t)((cnt[n] + EI_PIECE - 1) / EI_PIECE); },
+        [&](int n, int64_t off) { s_off[n] = (int32_t)off; });
     __syncthreads();
-    total = s_off[N];
   } else {
-    total = d.ei_piece_off[N];
+    total = d.ei_piece_off[N];   // planned by k_ei_plan
   }
   for (int32_t p = gw; p < total; p += nw) {
     int n;
@@ -2407,6 +2454,13 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   if (sim_stopped(d, sim, t)) return;
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(dy.y, mask);
+  if (tid == 0 && n == sp.node_begin) {
+    // today's early-stop rule on yesterday's log (unless k_day_begin ran it); stopping on
+    // day t leaves day t whole, so it can run at any point of the day
+    const int32_t cases = d.sim_cases[sim];
+    if (cases != CASES_CHECKED) day_stop_check(d, sim, t, mask, cases);
+    d.sim_cases[sim] = 0;
+  }
   __shared__ int64_t tally[NPART];
   __shared__ int64_t grp[WG / NPART][NPART];
   __shared__ int64_t s_w;
@@ -2525,60 +2579,6 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
 }
 
 // ---------------------------------------------------------------------------
-// True in exactly one workgroup of the launch: the last one to get here.  Every
-// workgroup's global writes before the call are visible to it afterwards.
-__device__ bool last_block(uint32_t* ticket) {
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t v = atomicAdd(ticket, 1u);
-    am_last = (v == gridDim.x - 1);
-    if (am_last) atomicExch(ticket, 0u);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last;
-}
-
-// Exclusive prefix over n in [0, N) of cnt(n), by every thread of one workgroup:
-// emit(n, prefix) for each n; returns the total.  Each pass takes 4 * WG nodes, 4
-// consecutive ones per thread with their loads in flight together.
-template <typename Cnt, typename Emit>
-__device__ int64_t block_scan_nodes(int N, Cnt cnt, Emit emit) {
-  __shared__ int64_t s_w[WG / 64];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  int64_t carry = 0;
-  for (int base = 0; base < N; base += 4 * WG) {
-    int64_t v[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int n = base + 4 * tid + a;
-      v[a] = n < N ? cnt(n) : 0;
-    }
-    const int64_t sum = v[0] + v[1] + v[2] + v[3];
-    const int64_t incl = wave_incl_scan_i64(sum, lane);
-    if (lane == 63) s_w[w] = incl;
-    __syncthreads();
-    int64_t off = carry + incl - sum, tot = 0;
-#pragma unroll
-    for (int j = 0; j < WG / 64; ++j) {
-      const int64_t x = s_w[j];
-      off += j < w ? x : 0;
-      tot += x;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int n = base + 4 * tid + a;
-      if (n < N) emit(n, off);
-      off += v[a];
-    }
-    carry += tot;
-    __syncthreads();
-  }
-  return carry;
-}
-
 // Exclusive scan of the probes of sampling round `round` over nodes (probe_off),
 // and reservation of their space in the selected list; run by one workgroup.
 __device__ void plan_scan(const Dev& d, int round) {
@@ -3301,16 +3301,28 @@ __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, 
     day_end_node(d, n, t, dy.y, mask, tx_adjust);
     if (begin_next) day_begin_node(d, n, t + 1, mask);
   }
-  if (advance && last_block(d.ticket + 2)) {
-    ei_plan(d, (t + 1) & 1);
-    for (int sim = threadIdx.x; sim < d.n_sims; sim += WG) {
-      if (begin_next) day_stop_check(d, sim, t + 1, mask, d.sim_cases[sim]);
-      d.sim_cases[sim] = 0;
+  if (advance) {
+    // the last workgroup to get here moves the day counter: every workgroup has read it
+    // (its load completed before the ticket, a workgroup-scope release), and nothing
+    // else passes between the workgroups, so no device-scope fence is needed
+    __shared__ bool am_last;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t v = atomicAdd(d.ticket + 2, 1u);
+      am_last = v == gridDim.x - 1;
+      if (am_last) {
+        atomicExch(d.ticket + 2, 0u);
+        if (begin_next) d.sel_count[0] = 0;
+        set_today(d, next);
+      }
     }
-    if (begin_next && threadIdx.x == 0) d.sel_count[0] = 0;
-    if (threadIdx.x == 0) set_today(d, next);
   }
 }
+
+// The piece plan of today's E / I list for k_ei when the node count exceeds what its
+// workgroups scan into LDS themselves (one workgroup).
+__global__ void __launch_bounds__(WG) k_ei_plan(Dev d) { ei_plan(d, today(d).x & 1); }
 
 
 }  // namespace lp
