@@ -159,6 +159,49 @@ int lp_log(lp_ctx* ctx, int32_t t);
  * per-chunk susceptible statistics the library otherwise carries from day to day. */
 int lp_invalidate(lp_ctx* ctx);
 
+/* Population initialisation on the device, in place of the host draws of the
+ * components' constructors: SEIR_ABM.__init__ / DiseaseState_ABM init (ages,
+ * immunity, initial infections, disease timers; model.py:561-713, 568-587),
+ * populate_heterogeneous_values (model.py:816-866), VitalDynamics_ABM death dates
+ * (model.py:1548-1669) and RI_ABM timers (model.py:1887-1894).  Every slot of every
+ * node segment is written: initial agents [seg_begin, live_end), unborn slots after
+ * them (timers and multipliers drawn, the rest at their defaults).  Draws come from
+ * Philox streams keyed by (seed, sim-local slot): the distributions of the host
+ * generator, not its numbers.  Exact-count selections (initial infections) are made
+ * by the caller and passed as slots.  Call after lp_prepare, before the first step. */
+enum { LP_DIST_CONSTANT = 0, LP_DIST_NORMAL = 1, LP_DIST_GAMMA = 2, LP_DIST_LOGNORMAL = 3, LP_DIST_POISSON = 4,
+       LP_DIST_EXPONENTIAL = 5, LP_DIST_UNIFORM_INT = 6 };
+/* constant(a), normal(mean a, std b), gamma(shape a, scale b), lognormal(mean a, std b),
+ * poisson(lam a), exponential(scale a), uniform integers [a, b) */
+typedef struct { int32_t kind; float a, b; } lp_dist;
+#define LP_INIT_BANDS 8
+#define LP_INIT_PYR 32
+typedef struct {
+  uint64_t seed;              /* Philox key of the initial draws */
+  int32_t age_model;          /* 0: exponential (age_mean_days, capped at age_cap_days); 1: pyramid bins */
+  float age_mean_days, age_cap_days;
+  int32_t n_pyr;              /* pyramid: bin b drawn with cumulative weight pyr_cdf[b], age uniform in */
+  float pyr_cdf[LP_INIT_PYR]; /* [pyr_lo[b], pyr_hi[b]) days */
+  int32_t pyr_lo[LP_INIT_PYR], pyr_hi[LP_INIT_PYR];
+  float init_immun;           /* probability that an initial agent starts recovered */
+  int32_t n_bands;            /* recovered by age: age in [band_lo, band_hi) days with probability band_frac */
+  int32_t band_lo[LP_INIT_BANDS], band_hi[LP_INIT_BANDS];
+  float band_frac[LP_INIT_BANDS];
+  lp_dist dur_exp, dur_inf, t_to_paralysis;
+  int32_t het_model;          /* 0: multipliers 1 and inf_mean; 1: both lognormal; 2: lognormal risk, gamma(1) infectivity */
+  float risk_mu, risk_sigma;  /* log-mean and log-sd of the risk (and, model 1, of infectivity / inf_mean) */
+  float inf_mean;             /* mean daily infectivity */
+  float rho;                  /* normal correlation of the copula, 2 sin(pi corr / 6) */
+  int32_t ri;                 /* 1: ri_timer = date_of_birth + U[ri_lo, ri_hi) days */
+  float ri_lo, ri_hi;
+  int32_t vd;                 /* 1: date_of_death from the sim's mortality table (km_cd) */
+  float missed_frac;          /* probability of being chronically missed */
+  int32_t reserved[3];
+} lp_init_sim;
+int64_t lp_sizeof_init_sim(void);
+int lp_init_population(lp_ctx* ctx, const lp_init_sim* sims, int32_t n_sims, const int64_t* infected_slots,
+                       int64_t n_infected);
+
 /* Wait for queued work; returns LP_ERR_CAPACITY / LP_ERR_OVERFLOW if the device flagged one. */
 int lp_sync(lp_ctx* ctx);
 

@@ -1,7 +1,7 @@
 // Host side of the C ABI declared in include/lp_step.h: binding table,
 // validation, span tables, scratch, the per-day kernel sequence, hipGraph
 // capture/replay and error reporting.
-#include "lp_kernels.hip"
+#include "lp_init.hip"
 #include <string>
 #include <vector>
 #include <cstring>
@@ -596,6 +596,41 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   c->list_tick = -1;   // stage calls do not keep the E / I list
   return enqueue_day(c, t, day_kind(c, t, mask, recount), mask, fused_tx_log || !(mask & LP_STAGE_LOG) ? 1 : 0, false,
                      true);
+}
+
+int64_t lp_sizeof_init_sim(void) { return (int64_t)sizeof(lp_init_sim); }
+
+int lp_init_population(lp_ctx* c, const lp_init_sim* sims, int32_t n_sims, const int64_t* infected, int64_t n_inf) {
+  if (!c || !sims || n_inf < 0 || (n_inf > 0 && !infected)) return LP_ERR_ARG;
+  if (!c->prepared) return fail(c, LP_ERR_STATE, "lp_init_population before lp_prepare");
+  if (n_sims != c->cfg.n_sims) return fail(c, LP_ERR_ARG, "lp_init_population: one lp_init_sim per simulation");
+  for (int s = 0; s < n_sims; ++s) {
+    const lp_init_sim& p = sims[s];
+    if (p.n_pyr < 0 || p.n_pyr > LP_INIT_PYR || (p.age_model == 1 && p.n_pyr < 1) || p.n_bands < 0 ||
+        p.n_bands > LP_INIT_BANDS || p.het_model < 0 || p.het_model > 2 || p.age_model < 0 || p.age_model > 1)
+      return fail(c, LP_ERR_ARG, "lp_init_population: bad spec for sim " + std::to_string(s));
+  }
+  lp_init_sim* specs_d = nullptr;
+  int64_t* inf_d = nullptr;
+  HIPCHK(c, hipMalloc(&specs_d, sizeof(lp_init_sim) * n_sims));
+  if (n_inf > 0) HIPCHK(c, hipMalloc(&inf_d, sizeof(int64_t) * n_inf));
+  HIPCHK(c, hipMemcpyAsync(specs_d, sims, sizeof(lp_init_sim) * n_sims, hipMemcpyHostToDevice, c->stream));
+  if (n_inf > 0) HIPCHK(c, hipMemcpyAsync(inf_d, infected, sizeof(int64_t) * n_inf, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(lp::k_init, dim3(c->n_spans), dim3(lp::WG), 0, c->stream, c->d, specs_d);
+  if (n_inf > 0)
+    hipLaunchKernelGGL(lp::k_init_infected, dim3(grid_of(n_inf, 256) < 4096 ? grid_of(n_inf, 256) : 4096), dim3(256), 0,
+                       c->stream, c->d, inf_d, n_inf);
+  const hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(c->stream);
+  (void)hipFree(specs_d);
+  if (inf_d) (void)hipFree(inf_d);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(c, LP_ERR_HIP, std::string("lp_init_population: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  // every carried statistic is recounted from the new columns by the first pass
+  c->recount_pending = true;
+  c->list_tick = -1;
+  c->rec_dirty = false;
+  return LP_OK;
 }
 
 int lp_invalidate(lp_ctx* c) {

@@ -81,6 +81,50 @@ SIM_PARAMS_DTYPE = np.dtype(
     align=True,
 )
 
+LP_DIST_CONSTANT, LP_DIST_NORMAL, LP_DIST_GAMMA, LP_DIST_LOGNORMAL, LP_DIST_POISSON, LP_DIST_EXPONENTIAL, \
+    LP_DIST_UNIFORM_INT = range(7)
+INIT_BANDS = 8
+INIT_PYR = 32
+
+
+class LPDist(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_float), ("b", ctypes.c_float)]
+
+
+class LPInitSim(ctypes.Structure):
+    """lp_init_sim of include/lp_step.h: the device initialisation of one simulation."""
+
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("age_model", ctypes.c_int32),
+        ("age_mean_days", ctypes.c_float),
+        ("age_cap_days", ctypes.c_float),
+        ("n_pyr", ctypes.c_int32),
+        ("pyr_cdf", ctypes.c_float * INIT_PYR),
+        ("pyr_lo", ctypes.c_int32 * INIT_PYR),
+        ("pyr_hi", ctypes.c_int32 * INIT_PYR),
+        ("init_immun", ctypes.c_float),
+        ("n_bands", ctypes.c_int32),
+        ("band_lo", ctypes.c_int32 * INIT_BANDS),
+        ("band_hi", ctypes.c_int32 * INIT_BANDS),
+        ("band_frac", ctypes.c_float * INIT_BANDS),
+        ("dur_exp", LPDist),
+        ("dur_inf", LPDist),
+        ("t_to_paralysis", LPDist),
+        ("het_model", ctypes.c_int32),
+        ("risk_mu", ctypes.c_float),
+        ("risk_sigma", ctypes.c_float),
+        ("inf_mean", ctypes.c_float),
+        ("rho", ctypes.c_float),
+        ("ri", ctypes.c_int32),
+        ("ri_lo", ctypes.c_float),
+        ("ri_hi", ctypes.c_float),
+        ("vd", ctypes.c_int32),
+        ("missed_frac", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
 _lib = None
 
 
@@ -112,6 +156,8 @@ def load_library(path=LIB_PATH):
     lib.lp_debug_buffer.restype = vp
     lib.lp_set_timing.argtypes = [vp, i32]
     lib.lp_agent_pass_timing.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
+    lib.lp_sizeof_init_sim.restype = i64
+    lib.lp_init_population.argtypes = [vp, ctypes.POINTER(LPInitSim), i32, vp, i64]
     lib.lp_last_error.argtypes = [vp]
     lib.lp_last_error.restype = ctypes.c_char_p
     lib.lp_destroy.argtypes = [vp]
@@ -120,6 +166,8 @@ def load_library(path=LIB_PATH):
         raise EngineUnavailable("liblpstep.so ABI version mismatch")
     if lib.lp_sizeof_sim_params() != SIM_PARAMS_DTYPE.itemsize:
         raise EngineUnavailable("lp_sim_params layout mismatch between include/lp_step.h and engine.py")
+    if lib.lp_sizeof_init_sim() != ctypes.sizeof(LPInitSim):
+        raise EngineUnavailable("lp_init_sim layout mismatch between include/lp_step.h and engine.py")
     _lib = lib
     return lib
 
@@ -143,7 +191,9 @@ def thr_from_prob64(p):
 class Engine:
     """One library context plus the device copies of every bound buffer."""
 
-    def __init__(self, config, arrays, timing=False):
+    def __init__(self, config, arrays, timing=False, device_only=()):
+        """device_only: names of buffers allocated on the device without uploading the host
+        array (columns that lp_init_population writes)."""
         self.torch = _require_gpu()
         self.lib = load_library()
         self.device = self.torch.device("cuda", self.torch.cuda.current_device())
@@ -156,7 +206,11 @@ class Engine:
             raise LPError(f"lp_create failed ({ERRORS.get(rc, rc)}): check sizes {config.capacity=} {config.n_nodes=}")
         self.ctx = ctx
         for name, arr in arrays.items():
-            self._upload_new(name, arr)
+            if name in device_only:
+                dt = np.dtype(_TORCH_VIEW.get(arr.dtype, arr.dtype))
+                self.tensors[name] = self.torch.empty(arr.size, dtype=_torch_dtype(self.torch, dt), device=self.device)
+            else:
+                self._upload_new(name, arr)
         self.torch.cuda.synchronize()
         for name, arr in arrays.items():
             t = self.tensors[name]
@@ -176,6 +230,14 @@ class Engine:
         elif a.dtype in _TORCH_VIEW:
             a = a.view(_TORCH_VIEW[a.dtype])
         self.tensors[name] = torch.from_numpy(a.reshape(-1)).to(self.device)
+
+    def init_population(self, specs, infected_slots):
+        """lp_init_population: the agent columns drawn on the device from `specs` (one
+        LPInitSim per simulation); `infected_slots` (int64) start infectious."""
+        arr = (LPInitSim * len(specs))(*specs)
+        inf = np.ascontiguousarray(infected_slots, dtype=np.int64)
+        self._check(self.lib.lp_init_population(self.ctx, arr, len(specs), ctypes.c_void_p(inf.ctypes.data if inf.size else 0),
+                                                int(inf.size)))
 
     def push(self, name, arr):
         """Host -> device copy of one bound buffer (same size as at bind time)."""
@@ -261,6 +323,12 @@ class Engine:
             self.close()
         except Exception:  # noqa: BLE001 -- interpreter shutdown
             pass
+
+
+def _torch_dtype(torch, dt):
+    return {np.dtype(np.int8): torch.int8, np.dtype(np.uint8): torch.uint8, np.dtype(np.int16): torch.int16,
+            np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64, np.dtype(np.float32): torch.float32,
+            np.dtype(np.float64): torch.float64}[np.dtype(dt)]
 
 
 _hiprt = None

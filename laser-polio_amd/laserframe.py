@@ -83,7 +83,9 @@ class LaserFrame:
 
 
 class PeopleFrame(LaserFrame):
-    def __init__(self, init_pop, node_capacity, tile=1024):
+    def __init__(self, init_pop, node_capacity, tile=1024, lazy=False):
+        """lazy: columns are allocated without being filled (their values are written on
+        the GPU and pulled before the host reads them)."""
         init_pop = np.asarray(init_pop, dtype=np.int64)
         cap_n = np.asarray(node_capacity, dtype=np.int64)
         cap_n = ((np.maximum(cap_n, init_pop) + tile - 1) // tile) * tile
@@ -95,6 +97,13 @@ class PeopleFrame(LaserFrame):
         object.__setattr__(self, "seg_end", seg_begin + cap_n)
         object.__setattr__(self, "live_end", seg_begin + init_pop)
         object.__setattr__(self, "_init_cum", np.concatenate([[0], np.cumsum(init_pop)]))
+        object.__setattr__(self, "lazy", bool(lazy))
+
+    def add_scalar_property(self, name, dtype=np.uint32, default=0):
+        if self.lazy:
+            self._cols[name] = np.empty(self._capacity, dtype=dtype)
+        else:
+            super().add_scalar_property(name, dtype=dtype, default=default)
 
     @property
     def count(self):

@@ -99,6 +99,22 @@ def _vaccine_strain(vtype):
     return 1
 
 
+def dist_spec(dist):
+    """lp_dist of a duration Distribution (the device samplers of lp_init_population)."""
+    d, p = dist.dist_type, dist.pars
+    kinds = {"constant": (engmod.LP_DIST_CONSTANT, p.get("value", 1), 0.0),
+             "normal": (engmod.LP_DIST_NORMAL, p.get("mean", 0.0), p.get("std", 1.0)),
+             "gamma": (engmod.LP_DIST_GAMMA, p.get("shape", 2.0), p.get("scale", 1.0)),
+             "lognormal": (engmod.LP_DIST_LOGNORMAL, p.get("mean", 1.0), p.get("sigma", 0.5)),
+             "poisson": (engmod.LP_DIST_POISSON, p.get("lam", 5), 0.0),
+             "exponential": (engmod.LP_DIST_EXPONENTIAL, p.get("scale", 1.0), 0.0),
+             "uniform": (engmod.LP_DIST_UNIFORM_INT, p.get("min", 2), p.get("max", 10))}
+    if d not in kinds:
+        raise ValueError(f"device initialisation has no sampler for the {d} distribution")
+    k, a, b = kinds[d]
+    return engmod.LPDist(k, float(a), float(b))
+
+
 def timer_from_samples(x, hi=127):
     """Float duration samples -> timers: truncate toward zero, clip to [0, hi] (model.py:575-579)."""
     return np.clip(np.trunc(np.asarray(x, dtype=np.float64)), 0, hi).astype(np.uint16)
@@ -174,11 +190,13 @@ class SEIR_ABM:
             pop = pars.init_pop
             if np.any(pop <= 0) or np.any(np.isnan(pop.astype(np.float64))):
                 raise ValueError("pop_by_node must be positive & non-nan")
-            self.people = PeopleFrame(pop, node_capacities(pars))
+            self.device_init = bool(pars.init_on_device)
+            self.people = PeopleFrame(pop, node_capacities(pars), lazy=self.device_init)
             p = self.people
             p.add_scalar_property("disease_state", dtype=np.int8, default=-1)
-            for n in range(len(pop)):
-                p.raw("disease_state")[p.initial_slice(n)] = 0
+            if not self.device_init:
+                for n in range(len(pop)):
+                    p.raw("disease_state")[p.initial_slice(n)] = 0
             p.add_scalar_property("potentially_paralyzed", dtype=np.int8, default=-1)
             p.add_scalar_property("paralyzed", dtype=np.int8, default=0)
             p.add_scalar_property("ipv_protected", dtype=np.int8, default=0)
@@ -186,11 +204,12 @@ class SEIR_ABM:
             p.add_scalar_property("chronically_missed", dtype=np.uint8, default=0)
             n0 = int(pop.sum())
             n_missed = int(pars.missed_frac * n0)
-            if n_missed > 0:
+            if n_missed > 0 and not self.device_init:
                 p.raw("chronically_missed")[p.initial_to_slot(self.rng.choice(n0, size=n_missed, replace=False))] = 1
             p.add_scalar_property("node_id", dtype=np.int16, default=-1)
-            for n in range(len(pop)):
-                p.raw("node_id")[p.initial_slice(n)] = n
+            if not self.device_init:
+                for n in range(len(pop)):
+                    p.raw("node_id")[p.initial_slice(n)] = n
             self.results = LaserFrame(capacity=1)
             self.nodes = np.arange(len(pop))
             self._components = []
@@ -199,6 +218,8 @@ class SEIR_ABM:
             self._touched = set()
             self._dev_newer = set()
             self._timing = False
+            self._device_initialized = False
+            self.init_spec = {"missed_frac": float(pars.missed_frac)}   # device initialisation inputs
 
     @classmethod
     def init_from_file(cls, people, pars=None):
@@ -279,7 +300,14 @@ class SEIR_ABM:
                     self._dev_newer.discard(key)
 
     def export_tables(self):
-        """SimTables of the current host state (used by the GPU engine and by the oracle)."""
+        """SimTables of the current host state (used by the GPU engine and by the oracle).
+        With init_on_device the population is drawn on the GPU first and pulled."""
+        if self.device_init:
+            self._ensure_engine()
+            self._pull_all()
+        return self._export_tables()
+
+    def _export_tables(self):
         pars, p = self.pars, self.people
         N, cap, K = len(self.nodes), p.capacity, len(pars.strain_ids)
         vd, ds, ri, sia, tx = (self._instance(n) for n in default_run_order)
@@ -439,7 +467,7 @@ class SEIR_ABM:
         if self._engine is not None:
             self._push_touched()
             return self._engine
-        tb = self.export_tables()
+        tb = self._export_tables()
         sp = self.sim_params(tb)
         arrays = dict(tb.cols)
         arrays.update(
@@ -473,12 +501,52 @@ class SEIR_ABM:
             tiles_per_block=int(self.pars.tiles_per_block or 0),
             mix_mode=0,
         )
-        self._engine = engmod.Engine(cfg, arrays, timing=self._timing)
+        device_only = set(AGENT_COLUMNS) if (self.device_init and not self._device_initialized) else set()
+        self._engine = engmod.Engine(cfg, arrays, timing=self._timing, device_only=device_only)
         for frame in (self.people, self.results):
             object.__setattr__(frame, "_sync_hook", self._sync_hook)
         self._touched.clear()
         self._dev_newer.clear()
+        if device_only:
+            self._engine.init_population([self.device_init_spec()], self.init_spec.get("infected", np.zeros(0, np.int64)))
+            self._device_initialized = True
+            for name in AGENT_COLUMNS:
+                if self.people.has(name):
+                    self._dev_newer.add((id(self.people), name))
         return self._engine
+
+    def device_init_spec(self):
+        """LPInitSim of this simulation from the components' initialisation inputs."""
+        pars, spec = self.pars, self.init_spec
+        s = engmod.LPInitSim()
+        s.seed = int(np.random.default_rng([int(pars.seed) & 0xFFFFFFFF, 0x1D]).integers(1, 2**63))
+        ages = spec.get("ages", ("exponential", 25 * 365.0, 80 * 365.0))
+        if ages[0] == "exponential":
+            s.age_model, s.age_mean_days, s.age_cap_days = 0, float(ages[1]), float(ages[2])
+        else:
+            cdf, lo, hi = ages[1], ages[2], ages[3]
+            if len(cdf) > engmod.INIT_PYR:
+                raise ValueError(f"device initialisation holds at most {engmod.INIT_PYR} age pyramid bins")
+            s.age_model, s.n_pyr = 1, len(cdf)
+            for b in range(len(cdf)):
+                s.pyr_cdf[b], s.pyr_lo[b], s.pyr_hi[b] = float(cdf[b]), int(lo[b]), int(hi[b])
+        s.init_immun = float(spec.get("init_immun", 0.0))
+        bands = spec.get("bands", [])
+        if len(bands) > engmod.INIT_BANDS:
+            raise ValueError(f"device initialisation holds at most {engmod.INIT_BANDS} recovered-by-age bands")
+        s.n_bands = len(bands)
+        for b, (lo, hi, frac) in enumerate(bands):
+            s.band_lo[b], s.band_hi[b], s.band_frac[b] = int(lo), int(hi), float(frac)
+        for field in ("dur_exp", "dur_inf", "t_to_paralysis"):
+            setattr(s, field, dist_spec(getattr(pars, field)))
+        het = spec.get("het", (0, 0.0, 0.0, 1.0, 0.0))
+        s.het_model, s.risk_mu, s.risk_sigma, s.inf_mean, s.rho = het[0], float(het[1]), float(het[2]), float(het[3]), float(het[4])
+        ri = spec.get("ri")
+        if ri is not None:
+            s.ri, s.ri_lo, s.ri_hi = 1, float(ri[0]), float(ri[1])
+        s.vd = 1 if spec.get("vd") else 0
+        s.missed_frac = float(spec.get("missed_frac", 0.0))
+        return s
 
     def _engine_step(self, mask):
         eng = self._ensure_engine()
@@ -596,10 +664,27 @@ class VitalDynamics_ABM:
     def _initialize_ages_and_births(self):
         p = self.people
         p.add_scalar_property("date_of_birth", dtype=np.int32, default=-1)
+        if self.sim.device_init:
+            self.sim.init_spec["ages"] = self._age_spec()
+            return
         dob = p.raw("date_of_birth")
         for n in range(len(self.nodes)):
             sl = p.initial_slice(n)
             dob[sl] = -self._sample_ages(sl.stop - sl.start)
+
+    def _age_spec(self):
+        """Age model of the initial population for the device initialisation."""
+        pars = self.pars
+        if pars.age_model is not None:
+            kind, mean_y, cap_y = pars.age_model
+            if kind != "exponential":
+                raise ValueError(f"unknown age_model {kind}")
+            return ("exponential", mean_y * 365.0, cap_y * 365.0)
+        if pars.age_pyramid_path is not None:
+            pyr = load_pyramid_csv(pars.age_pyramid_path)
+            w = (pyr[:, 2] + pyr[:, 3]).astype(np.float64)
+            return ("pyramid", np.cumsum(w / w.sum()), np.maximum(pyr[:, 0] * 365, 1), (pyr[:, 1] + 1) * 365)
+        return ("exponential", 25 * 365.0, 80 * 365.0)
 
     def _initialize_deaths(self):
         pars, p = self.pars, self.people
@@ -608,6 +693,9 @@ class VitalDynamics_ABM:
             self.results.add_array_property("births", shape=(self.sim.nt, len(self.nodes)), dtype=np.int32)
             self.results.add_array_property("deaths", shape=(self.sim.nt, len(self.nodes)), dtype=np.int32)
             p.add_scalar_property("date_of_death", dtype=np.int32, default=0)
+            if self.sim.device_init:
+                self.sim.init_spec["vd"] = True
+                return
             dod, dob = p.raw("date_of_death"), p.raw("date_of_birth")
             for n in range(len(self.nodes)):
                 sl = p.initial_slice(n)
@@ -679,12 +767,12 @@ class DiseaseState_ABM:
         p.add_scalar_property("infection_timer", dtype=np.uint16, default=0)
         p.add_scalar_property("paralysis_timer", dtype=np.uint16, default=0)
         et, it, pt = p.raw("exposure_timer"), p.raw("infection_timer"), p.raw("paralysis_timer")
-        B = 1 << 23
-        for b0 in range(0, cap, B):
+        B = 1 << 23 if not sim.device_init else 0
+        for b0 in range(0, cap if B else 0, B or 1):
             b1 = min(cap, b0 + B)
             et[b0:b1] = timer_from_samples(pars.dur_exp(b1 - b0))
             it[b0:b1] = timer_from_samples(pars.dur_inf(b1 - b0))
-        for b0 in range(0, cap, B):
+        for b0 in range(0, cap if B else 0, B or 1):
             b1 = min(cap, b0 + B)
             raw = pars.t_to_paralysis(b1 - b0) - et[b0:b1]
             pt[b0:b1] = np.clip(raw, 0, np.minimum(it[b0:b1], 127)).astype(np.int64).astype(np.uint16)
@@ -702,10 +790,18 @@ class DiseaseState_ABM:
             if not (0.0 <= fr[n] <= 1.0):
                 raise ValueError(f"Invalid immunity fraction: {fr[n]} for node {n}")
             k = int(fr[n] * pars.init_pop[n])
-            if k > 0:
+            if k > 0 and not sim.device_init:
                 sl = p.initial_slice(n)
                 state[sl.start + rng.choice(sl.stop - sl.start, size=k, replace=False)] = 3
-        if pars.init_recovered_by_age is not None:
+        if sim.device_init:
+            if np.any(fr != fr[0]):
+                raise ValueError("device initialisation takes one init_immun fraction for every node")
+            sim.init_spec["init_immun"] = float(fr[0])
+            if pars.init_recovered_by_age is not None:
+                if not p.has("date_of_birth"):
+                    raise ValueError("init_recovered_by_age needs VitalDynamics_ABM (ages)")
+                sim.init_spec["bands"] = [(lo * 365, hi * 365, frac) for lo, hi, frac in pars.init_recovered_by_age]
+        elif pars.init_recovered_by_age is not None:
             dob = p.raw("date_of_birth") if p.has("date_of_birth") else None
             if dob is None:
                 raise ValueError("init_recovered_by_age needs VitalDynamics_ABM (ages)")
@@ -733,14 +829,17 @@ class DiseaseState_ABM:
                     raise ValueError(f"Unsupported value in init_prev list at node {n}: {prev}")
                 k = int(pars.init_pop[n] * prev) if 0 < prev < 1 else min(int(prev), int(pars.init_pop[n]))
                 sl = p.initial_slice(n)
-                alive = sl.start + np.nonzero(state[sl] >= 0)[0]
+                # every initial agent is alive here (immune ones included)
+                alive = np.arange(sl.start, sl.stop) if sim.device_init else sl.start + np.nonzero(state[sl] >= 0)[0]
                 k = min(k, len(alive))
                 if k > 0:
                     parts.append(rng.choice(alive, size=k, replace=False))
             idx = np.concatenate(parts) if parts else []
         else:
             raise ValueError(f"Unsupported init_prev type: {type(pars.init_prev)}")
-        if len(idx):
+        if sim.device_init:
+            sim.init_spec["infected"] = np.asarray(idx, dtype=np.int64)
+        elif len(idx):
             state[np.asarray(idx, dtype=np.int64)] = 2
 
     def _common_init(self, sim):
@@ -855,6 +954,25 @@ def populate_heterogeneous_values(start, end, acq_risk_out, infectivity_out, par
             infectivity_out[b0:b1] = mean_inf * np.exp(mu_r + sg_r * z[:, 1])
 
 
+def heterogeneity_spec(pars):
+    """(model, log-mean, log-sd, infectivity mean, normal correlation) of the copula of
+    populate_heterogeneous_values, for the device initialisation: model 0 constants, 1 both
+    lognormal, 2 lognormal risk with gamma(1) infectivity."""
+    mean_inf = pars.infectivity_mean
+    if mean_inf is None:
+        mean_inf = pars.r0 / np.mean(pars.dur_inf(1000))
+    if not pars.individual_heterogeneity:
+        return (0, 0.0, 0.0, float(mean_inf), 0.0)
+    rho = 2.0 * np.sin(np.pi * pars.corr_risk_inf / 6)
+    if pars.heterogeneity_model == "reference":
+        var_ln = pars.risk_mult_var
+        return (2, np.log(1.0 / np.sqrt(var_ln + 1.0)), np.sqrt(np.log(var_ln + 1.0)), float(max(mean_inf, 1e-10)), rho)
+    if pars.heterogeneity_model == "lognormal":
+        sd = float(pars.heterogeneity_sigma)
+        return (1, np.log(1.0 / np.sqrt(sd * sd + 1.0)), np.sqrt(np.log(sd * sd + 1.0)), float(mean_inf), rho)
+    raise ValueError(f"unknown heterogeneity_model {pars.heterogeneity_model}")
+
+
 class Transmission_ABM:
     """Force of infection over the node network and infection draws (model.py:1152-1490)."""
 
@@ -873,6 +991,9 @@ class Transmission_ABM:
         p = self.people
         p.add_scalar_property("acq_risk_multiplier", dtype=np.float32, default=1.0)
         p.add_scalar_property("daily_infectivity", dtype=np.float32, default=1.0)
+        if self.sim.device_init:
+            self.sim.init_spec["het"] = heterogeneity_spec(self.pars)
+            return
         populate_heterogeneous_values(0, p.capacity, p.raw("acq_risk_multiplier"), p.raw("daily_infectivity"), self.pars, self.sim.rng)
 
     def _initialize_common(self):
@@ -928,7 +1049,11 @@ class RI_ABM:
         p.add_scalar_property("ri_timer", dtype=np.int16, default=-1)
         ri, lo_hi = p.raw("ri_timer"), self.pars.ri_age_days
         dob = p.raw("date_of_birth") if p.has("date_of_birth") else None
-        for n in range(len(self.nodes)):
+        if sim.device_init:
+            if dob is None:
+                raise ValueError("device initialisation of RI timers needs VitalDynamics_ABM (ages)")
+            sim.init_spec["ri"] = (float(lo_hi[0]), float(lo_hi[1]))
+        for n in range(len(self.nodes) if not sim.device_init else 0):
             sl = p.initial_slice(n)
             base = dob[sl] if dob is not None else np.full(sl.stop - sl.start, -1, np.int32)
             days = sim.rng.uniform(lo_hi[0], lo_hi[1], sl.stop - sl.start) if lo_hi[1] > lo_hi[0] else np.full(sl.stop - sl.start, float(lo_hi[0]))

@@ -10,6 +10,9 @@ synthetic workloads (all default to the reference behaviour):
   ri_age_days             (lo, hi) days from birth to RI for the initial population (reference 42..98)
   ri_newborn_timer        ri_timer of newborns (reference 182)
   tiles_per_block         agent-pass tiles per workgroup (0 = autotuned default)
+  init_on_device          draw the per-agent columns on the GPU (lp_init_population) instead of on
+                          the host: same distributions, different numbers; initial immunity and
+                          chronically-missed agents become per-agent Bernoulli draws
 """
 
 import datetime
@@ -94,6 +97,7 @@ default_pars = PropertySet(
         "ri_age_days": (42, 98),
         "ri_newborn_timer": 182,
         "tiles_per_block": 0,
+        "init_on_device": False,
     }
 )
 
