@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0,
                     help="1: the timed window replays hipGraphs too (no per-kernel events, no roofline)")
     ap.add_argument("--tiles-per-block", type=int, default=0)
+    ap.add_argument("--init", choices=["device", "host"], default="device",
+                    help="draw the initial population on the GPU (lp_init_population) or on the host (numpy)")
     ap.add_argument("--workload", choices=["national", "replicates", "adjoint"], default="national",
                     help="national: one 774-node simulation (default); replicates: --sims independent "
                          "simulations of uneven sizes batched in one context, against serial runs of a sample; "
@@ -144,6 +146,8 @@ def main():
     ap.add_argument("--sims", type=int, default=512, help="replicates: simulations in the batch")
     ap.add_argument("--serial-sample", type=int, default=16, help="replicates: simulations also run one at a time")
     ap.add_argument("--workers", type=int, default=16, help="replicates: host processes initialising simulations")
+    ap.add_argument("--check-sims", type=int, default=8,
+                    help="replicates: simulations of the batch replayed alone on the CPU oracle afterwards")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="instead of the benchmark: copy this many GiB device-to-device 5 times (a plain copy kernel "
                          "of known size, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -269,13 +273,18 @@ def national(args):
     t_end = t_begin + K
     t_init = time.perf_counter()
     pars = configs.nigeria_national_pars(scale=args.scale, dur=max(2555, t_end - 1))
+    pars.init_on_device = args.init == "device"
     if args.tiles_per_block:
         pars.tiles_per_block = args.tiles_per_block
     sim = configs.build(pars)
+    if pars.init_on_device:
+        sim._ensure_engine()          # the population is drawn on the GPU here
+        torch.cuda.synchronize()
+    init_s = time.perf_counter() - t_init
     nt = sim.nt
     n0 = int(pars.init_pop.sum())
-    log(f"[bench] init: {len(pars.init_pop)} nodes, {n0} agents, capacity {sim.people.capacity}, "
-        f"{time.perf_counter() - t_init:.1f}s")
+    log(f"[bench] init ({args.init}): {len(pars.init_pop)} nodes, {n0} agents, capacity {sim.people.capacity}, "
+        f"{init_s:.2f}s")
     parity_day = args.cpu_days + 1 if args.cpu_days > 0 else 0
     if parity_day and parity_day >= t_begin - W:
         raise SystemExit(f"--cpu-days {args.cpu_days} must end before the warm-up days (start day {t_begin})")
@@ -433,6 +442,7 @@ def national(args):
             "agents": n0,
             "capacity": int(sim.people.capacity),
             "graph": bool(args.graph),
+            "population_init": f"{args.init} ({init_s:.2f}s, not timed)",
         },
         "full_run": {"days": nt - 1, "seconds": round(run_s, 4), "agent_days_per_s": round(full_ad / run_s, 1),
                      "ms_per_day": round(run_s * 1e3 / (nt - 1), 4),
@@ -481,15 +491,20 @@ def replicates(args):
     agent-days over the sample.  Host initialisation and uploads are not timed."""
     lp = __graft_entry__.load_package()
     from laser_polio_amd import configs
-    from laser_polio_amd.batch import build_replicates
+    from laser_polio_amd.batch import build_replicates, build_replicates_device
 
     S, dur = args.sims, 365
     pars_list = [configs.replicate_pars(i, scale=args.scale, dur=dur) for i in range(S)]
     t_init = time.perf_counter()
-    batch = build_replicates(pars_list, configs.build, workers=max(1, args.workers), log=log)
+    if args.init == "device":
+        import torch
+
+        batch = build_replicates_device(pars_list, configs.build, log=log)
+    else:
+        batch = build_replicates(pars_list, configs.build, workers=max(1, args.workers), log=log)
     n0 = int(sum(int(p.init_pop.sum()) for p in pars_list))
     log(f"[bench] replicates: {S} simulations, {batch.cfg.n_nodes} nodes, {n0} agents, capacity "
-        f"{batch.cfg.capacity}, host init {time.perf_counter() - t_init:.1f}s")
+        f"{batch.cfg.capacity}, host construction {time.perf_counter() - t_init:.1f}s ({args.init} population)")
 
     import torch
 
@@ -507,19 +522,25 @@ def replicates(args):
         t_up = time.perf_counter()
         eng = batch.prepare()
         torch.cuda.synchronize()
-        log(f"[bench] upload {time.perf_counter() - t_up:.1f}s")
+        log(f"[bench] upload / device population {time.perf_counter() - t_up:.1f}s")
+        # the initial state of a sample of simulations, for their oracle replay afterwards
+        check = sorted(set(np.linspace(0, S - 1, min(args.check_sims, S)).round().astype(int).tolist())) if args.check_sims else []
+        init_state = {k: batch.pull_sim(k, names=None)[0] for k in check}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         eng.run_days(0, batch.cfg.nt, use_graph=True)
-        eng.sync()
         torch.cuda.synchronize()
         el_b = time.perf_counter() - t0
+        eng.sync()
         ad_b = batch.agent_days(1, batch.cfg.nt)
         per_sim_ad = []
         pop = np.empty_like(batch.arrays["pop"])
         eng.pull("pop", pop)
         for k in range(S):
             per_sim_ad.append(int(pop[1:, batch.node_off[k]:batch.node_off[k + 1]].astype(np.int64).sum()))
+        checks = batch_properties(batch, eng)
+        log(f"[bench] batch properties: {checks}")
+        parity = replicate_parity(batch, check, init_state, log) if check else None
         batch.close()
         log(f"[bench] batched: {S} sims x {dur} days in {el_b:.3f}s -> {S / el_b * 3600:.4e} sims/hr, "
             f"{ad_b / el_b:.4e} agent-days/s")
@@ -530,9 +551,9 @@ def replicates(args):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             e1.run_days(0, one.cfg.nt, use_graph=True)
-            e1.sync()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
+            e1.sync()
             ads.append(one.agent_days(1, one.cfg.nt))
             one.close()
         A = np.vstack([np.full(len(times), dur, float), np.array(ads, float)]).T
@@ -565,8 +586,61 @@ def replicates(args):
         "serial_sample": {"sims": len(times), "seconds": round(float(np.sum(times)), 4),
                           "fit_us_per_day": round(float(coef[0]) * 1e6, 2),
                           "fit_ps_per_agent_day": round(float(coef[1]) * 1e12, 3)},
+        "batch_properties": checks,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
+    if not checks["ok"] or (parity is not None and not parity["identical"]):
+        sys.exit(1)
+
+
+def batch_properties(batch, eng):
+    """Bookkeeping every simulation of a finished batch must satisfy, from its results:
+    pop[t] = pop[t-1] + births[t] - deaths[t] on every node and day, and S + E + I + R =
+    pop on every node and logged day (through the simulation's early stop, if any)."""
+    res = {}
+    for name in ("pop", "births", "deaths", "S", "E", "I", "R"):
+        out = np.empty_like(batch.arrays[name])
+        eng.pull(name, out)
+        res[name] = out.astype(np.int64)
+    stops = eng.stop_days()
+    bad_pop, bad_seir = 0, 0
+    for k in range(len(batch.packs)):
+        na, nb = batch.node_off[k], batch.node_off[k + 1]
+        last = batch.cfg.nt - 1 if stops[k] < 0 else int(stops[k])
+        r = {n: v[: last + 1, na:nb] for n, v in res.items()}
+        bad_pop += int(np.count_nonzero(r["pop"][1:] != r["pop"][:-1] + r["births"][1:] - r["deaths"][1:]))
+        bad_seir += int(np.count_nonzero(r["S"] + r["E"] + r["I"] + r["R"] != r["pop"]))
+    return {"ok": bad_pop == 0 and bad_seir == 0, "pop_bookkeeping_mismatches": bad_pop,
+            "seir_vs_pop_mismatches": bad_seir, "sims": len(batch.packs)}
+
+
+def replicate_parity(batch, ks, init_state, log):
+    """Simulations ks of the batch, each replayed alone on the CPU oracle from its
+    initial state: every results row, agent column and live end bit for bit."""
+    from oracle.oracle import OracleSim
+    from laser_polio_amd.model import AGENT_COLUMNS
+
+    bad = []
+    tc = time.perf_counter()
+    for k in ks:
+        pk = batch.packs[k]
+        tb = pk.tables
+        tb.cols = init_state[k]
+        o = OracleSim(tb, pop0=pk.results["pop"][0]).run()
+        tb.cols = None
+        cols, res, live = batch.pull_sim(k)
+        for name, want in o.res.items():
+            if name in res and not np.array_equal(res[name], want):
+                bad.append(f"sim {k} results.{name}")
+        for name in AGENT_COLUMNS:
+            if not np.array_equal(cols[name], o.cols[name]):
+                bad.append(f"sim {k} people.{name}")
+        if not np.array_equal(live, o.live_end):
+            bad.append(f"sim {k} live_end")
+    log(f"[bench] {len(ks)} sampled simulations replayed on the oracle: {'identical' if not bad else bad[:6]} "
+        f"({time.perf_counter() - tc:.1f}s)")
+    return {"sims": list(ks), "identical": not bad, "mismatches": bad[:8]}
 
 
 def adjoint_workload(args):
@@ -580,6 +654,13 @@ def adjoint_workload(args):
     (HIP forward and adjoint, 30-day checkpoints), relative error against autograd of the
     plain float32 PyTorch rollout (per parameter group) and against float64 central
     differences (h = 1e-4 relative) on scalars and a sample of r0_scalars."""
+    scale = 0.1 if args.scale == 1.0 else args.scale
+    out = adjoint_result(scale, init=args.init)
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_result(scale, init="device"):
+    """The adjoint workload at population `scale` as a bench record (see adjoint_workload)."""
     import torch
 
     __graft_entry__.load_package()
@@ -587,9 +668,9 @@ def adjoint_workload(args):
     from laser_polio_amd import configs
     import calendar
 
-    scale = 0.1 if args.scale == 1.0 else args.scale
     T = 180
     pars = configs.nigeria_national_pars(scale=scale, dur=T)
+    pars.init_on_device = init == "device"
     sim = configs.build(pars)
     agg = adj.aggregate(sim)
     tx = sim._instance("Transmission_ABM")
@@ -682,7 +763,7 @@ def adjoint_workload(args):
         "rel_err_vs_autograd": {k: float(f"{v:.3e}") for k, v in rel_ag.items()},
         "rel_err_vs_central_fd": {k: float(f"{v:.3e}") for k, v in fd_err.items()},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

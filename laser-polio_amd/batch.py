@@ -19,8 +19,11 @@ which nodes they target may differ.
 
 ``build_replicates(pars_list, workers)`` initialises the simulations of a large
 batch in worker processes that write the agent columns straight into shared
-memory at their batch offsets (host initialisation is the slow part of a
-512-simulation batch), returning a ``BatchedSims`` without per-sim objects.
+memory at their batch offsets, returning a ``BatchedSims`` without per-sim
+objects.  ``build_replicates_device(pars_list)`` draws every simulation's agent
+columns on the GPU instead (lp_init_population, one spec per simulation, draws
+keyed by sim-local slot, so a simulation in a batch equals the same simulation
+initialised alone); the host only builds node tables and schedules.
 """
 
 import multiprocessing as mp
@@ -86,6 +89,7 @@ class BatchedSims:
         self.timing = timing
         self.tiles_per_block = int(tiles_per_block)
         self._owner = None
+        self.device_init = None    # (specs, infected slots per simulation) when drawn on the GPU
         self._pack(cols)
         self.engine = None
 
@@ -134,10 +138,32 @@ class BatchedSims:
 
     # -- running ----------------------------------------------------------------------
     def prepare(self):
-        """Upload the packed buffers (not timed by run())."""
+        """Upload the packed buffers, or draw the agent columns on the GPU (not timed by run())."""
         if self.engine is None:
-            self.engine = engmod.Engine(self.cfg, self.arrays, timing=self.timing)
+            if self.device_init is None:
+                self.engine = engmod.Engine(self.cfg, self.arrays, timing=self.timing)
+            else:
+                specs, infected = self.device_init
+                self.engine = engmod.Engine(self.cfg, self.arrays, timing=self.timing, device_only=set(AGENT_COLUMNS))
+                slots = [np.asarray(x, np.int64) + self.slot_off[k] for k, x in enumerate(infected)]
+                self.engine.init_population(specs, np.concatenate(slots) if slots else np.zeros(0, np.int64))
         return self.engine
+
+    def pull_sim(self, k, names=None):
+        """Agent columns (sim-local slots) and results of simulation k from the device."""
+        eng = self.prepare()
+        eng.sync()      # the device-held records written back into the columns
+        a, b = self.slot_off[k], self.slot_off[k + 1]
+        na, nb = self.node_off[k], self.node_off[k + 1]
+        cols, res = {}, {}
+        for name in (names or AGENT_COLUMNS):
+            cols[name] = eng.tensors[name][a:b].cpu().numpy().view(AGENT_COLUMNS[name][0])
+        for attr, (bind, per) in RESULT_ARRAYS.items():
+            full = eng.tensors[bind].cpu().numpy().reshape(self.arrays[bind].shape)
+            res[attr] = full[:, na:nb]
+        live = np.empty_like(self.arrays["live_end"])
+        eng.pull("live_end", live)
+        return cols, res, live[na:nb] - a
 
     def run(self, use_graph=True, scatter=True):
         """Run every simulation to its end (or early stop) and, with `scatter`, copy each
@@ -190,12 +216,19 @@ class BatchedSims:
         return int(pop[t0:t1].astype(np.int64).sum())
 
     def subset(self, ks, tiles_per_block=None):
-        """A new batch of simulations `ks` (their packed columns copied out of this one)."""
+        """A new batch of simulations `ks` (their packed columns copied out of this one, or,
+        for a batch drawn on the GPU, their initialisation specs)."""
         ks = list(ks)
-        cols = {name: np.concatenate([self.arrays[name][self.slot_off[k]:self.slot_off[k + 1]] for k in ks])
-                for name in AGENT_COLUMNS}
         packs = [self.packs[k] for k in ks]
         tpw = self.tiles_per_block if tiles_per_block is None else tiles_per_block
+        if self.device_init is not None:
+            total = int(sum(self.slot_off[k + 1] - self.slot_off[k] for k in ks))
+            cols = {name: np.empty(total, dtype=dt) for name, (dt, _) in AGENT_COLUMNS.items()}
+            sub = BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
+            sub.device_init = ([self.device_init[0][k] for k in ks], [self.device_init[1][k] for k in ks])
+            return sub
+        cols = {name: np.concatenate([self.arrays[name][self.slot_off[k]:self.slot_off[k + 1]] for k in ks])
+                for name in AGENT_COLUMNS}
         return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
 
     def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True):
@@ -306,3 +339,25 @@ def build_replicates(pars_list, build, workers=8, tiles_per_block=0, timing=Fals
     except BaseException:
         shared.release()
         raise
+
+
+def build_replicates_device(pars_list, build, tiles_per_block=0, timing=False, log=None):
+    """BatchedSims of `build(pars)` for every pars in `pars_list`, their agent columns drawn
+    on the GPU when the batch is prepared (the host builds only node tables, schedules and
+    the exact-count initial infections)."""
+    packs, specs, infected = [], [], []
+    for k, p in enumerate(pars_list):
+        p.init_on_device = True
+        sim = build(p)
+        tb = sim._export_tables()
+        tb.cols = None
+        packs.append(SimPack(tb, sim.sim_params(tb), sim._results_arrays()))
+        specs.append(sim.device_init_spec())
+        infected.append(np.asarray(sim.init_spec.get("infected", np.zeros(0, np.int64)), np.int64))
+        if log is not None and ((k + 1) % 128 == 0 or k + 1 == len(pars_list)):
+            log(f"[batch] built {k + 1}/{len(pars_list)} simulations (population drawn on the GPU)")
+    total = int(sum(pk.tables.capacity for pk in packs))
+    cols = {name: np.empty(total, dtype=dt) for name, (dt, _) in AGENT_COLUMNS.items()}
+    batch = BatchedSims.from_packs(packs, cols, tiles_per_block=tiles_per_block, timing=timing)
+    batch.device_init = (specs, infected)
+    return batch

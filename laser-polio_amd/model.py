@@ -240,6 +240,14 @@ class SEIR_ABM:
             with self.perf_stats.start(cls.__name__ + ".__init__()"):
                 self.instances.append(cls(self))
         self._drop_engine()
+        if self.device_init and not self._device_initialized:
+            # the population is drawn on the GPU the first time an agent column is read
+            object.__setattr__(self.people, "_sync_hook", self._draw_hook)
+
+    def _draw_hook(self, frame, name):
+        if self._binding_of(frame, name) is not None and self._engine is None:
+            self._ensure_engine()
+        self._sync_hook(frame, name)
 
     def _instance(self, name):
         for c in self.instances:

@@ -85,3 +85,36 @@ def test_parallel_build_matches_serial_packing():
                 assert np.array_equal(par.arrays[name], arr), name
     finally:
         par.close()
+
+
+@pytest.mark.gpu
+def test_device_initialised_replicates_full_size():
+    """The replicate workload at full size (512 simulations of 8-40 nodes, 1.6e9 agents,
+    365 days) with every population drawn on the GPU: the batch bookkeeping of every
+    simulation (pop[t] = pop[t-1] + births - deaths; S + E + I + R = pop) and two sampled
+    simulations replayed alone on the CPU oracle, bit for bit."""
+    import time
+
+    import torch
+
+    from laser_polio_amd import configs
+    from laser_polio_amd.batch import build_replicates_device
+    import bench
+
+    t0 = time.perf_counter()
+    pars_list = [configs.replicate_pars(i, scale=1.0, dur=365) for i in range(512)]
+    batch = build_replicates_device(pars_list, configs.build)
+    eng = batch.prepare()
+    torch.cuda.synchronize()
+    print(f"[replicates] 512 simulations, capacity {batch.cfg.capacity}, built {time.perf_counter() - t0:.1f}s",
+          flush=True)
+    ks = [0, 341]
+    init_state = {k: batch.pull_sim(k)[0] for k in ks}
+    eng.run_days(0, batch.cfg.nt, use_graph=True)
+    eng.sync()
+    print(f"[replicates] ran {time.perf_counter() - t0:.1f}s", flush=True)
+    checks = bench.batch_properties(batch, eng)
+    assert checks["ok"], checks
+    par = bench.replicate_parity(batch, ks, init_state, print)
+    assert par["identical"], par
+    batch.close()
