@@ -221,6 +221,11 @@ const void* lp_debug_buffer(lp_ctx* ctx, const char* name);
  * bracketed launches and their summed milliseconds since the last call. */
 int lp_set_timing(lp_ctx* ctx, int32_t on);
 int lp_agent_pass_timing(lp_ctx* ctx, int64_t* launches, double* total_ms);
+/* The milliseconds of each bracketed launch since the last lp_agent_pass_timing call, in
+ * launch order (up to cap of them written to ms); returns their number, or a negative
+ * LP_ERR_* code.  Does not reset the record.  Diagnostics only: no reference
+ * counterpart. */
+int64_t lp_agent_pass_launch_ms(lp_ctx* ctx, float* ms, int64_t cap);
 
 const char* lp_last_error(lp_ctx* ctx);
 void lp_destroy(lp_ctx* ctx);

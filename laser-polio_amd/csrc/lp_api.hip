@@ -703,6 +703,15 @@ int lp_agent_pass_timing(lp_ctx* c, int64_t* launches, double* total_ms) {
   return LP_OK;
 }
 
+int64_t lp_agent_pass_launch_ms(lp_ctx* c, float* ms, int64_t cap) {
+  if (!c || (!ms && cap > 0) || cap < 0) return LP_ERR_ARG;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return LP_ERR_HIP;
+  const int64_t n = (int64_t)(c->ev_used / 2);
+  for (int64_t i = 0; i < n && i < cap; ++i)
+    if (hipEventElapsedTime(&ms[i], c->ev[2 * i], c->ev[2 * i + 1]) != hipSuccess) return LP_ERR_HIP;
+  return n;
+}
+
 int lp_set_timing(lp_ctx* c, int32_t on) {
   if (!c) return LP_ERR_ARG;
   c->timing = on != 0;

@@ -156,6 +156,8 @@ def load_library(path=LIB_PATH):
     lib.lp_debug_buffer.restype = vp
     lib.lp_set_timing.argtypes = [vp, i32]
     lib.lp_agent_pass_timing.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
+    lib.lp_agent_pass_launch_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i64]
+    lib.lp_agent_pass_launch_ms.restype = i64
     lib.lp_sizeof_init_sim.restype = i64
     lib.lp_init_population.argtypes = [vp, ctypes.POINTER(LPInitSim), i32, vp, i64]
     lib.lp_last_error.argtypes = [vp]
@@ -290,6 +292,19 @@ class Engine:
         """On: lp_run_days launches each day's kernels directly and brackets its agent pass
         with HIP events (read by agent_pass_timing); off: days replay their hipGraphs."""
         self._check(self.lib.lp_set_timing(self.ctx, 1 if on else 0))
+
+    def agent_pass_launch_ms(self):
+        """Milliseconds of each event-bracketed agent-pass launch since the last
+        agent_pass_timing() call (does not reset)."""
+        n = int(self.lib.lp_agent_pass_launch_ms(self.ctx, None, 0))
+        if n < 0:
+            raise LPError(f"lp_agent_pass_launch_ms: error {n}")
+        out = np.zeros(n, np.float32)
+        if n:
+            got = int(self.lib.lp_agent_pass_launch_ms(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+            if got < 0:
+                raise LPError(f"lp_agent_pass_launch_ms: error {got}")
+        return out
 
     def agent_pass_timing(self):
         n = ctypes.c_int64()
