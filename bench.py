@@ -117,7 +117,9 @@ def main():
     ap.add_argument("--tiles-per-block", type=int, default=0)
     ap.add_argument("--init", choices=["device", "host"], default="device",
                     help="draw the initial population on the GPU (lp_init_population) or on the host (numpy)")
-    ap.add_argument("--workload", choices=["national", "replicates", "adjoint"], default="national",
+    ap.add_argument("--seeds", type=int, default=4, help="continental: replicate seeds batched in one context")
+    ap.add_argument("--batch-scale", type=float, default=0.25, help="continental: population scale of the batched seeds")
+    ap.add_argument("--workload", choices=["national", "replicates", "adjoint", "continental"], default="national",
                     help="national: one 774-node simulation (default); replicates: --sims independent "
                          "simulations of uneven sizes batched in one context, against serial runs of a sample; "
                          "adjoint: gradients of a mean-field calibration loss on the national geography")
@@ -159,6 +161,9 @@ def main():
 
     if args.workload == "replicates":
         replicates(args)
+        return
+    if args.workload == "continental":
+        continental(args)
         return
     if args.workload == "adjoint":
         adjoint_workload(args)
@@ -691,6 +696,157 @@ def batch_properties(batch, eng):
         bad_seir += int(np.count_nonzero(r["S"] + r["E"] + r["I"] + r["R"] != r["pop"]))
     return {"ok": bad_pop == 0 and bad_seir == 0, "pop_bookkeeping_mismatches": bad_pop,
             "seir_vs_pop_mismatches": bad_seir, "sims": len(batch.packs)}
+
+
+CONTINENTAL_RESULTS = ("pop", "births", "deaths", "S", "E", "I", "R", "E_by_strain", "I_by_strain", "new_exposed",
+                       "new_exposed_by_strain", "sia_new_exposed_by_strain")
+
+
+def continental_single(scale):
+    """One continental simulation at population `scale`, drawn on the GPU, every day timed
+    as replayed hipGraphs; its results checked by result_properties and the state counts
+    of 8 sampled nodes' agent columns against the last day's rows."""
+    import torch
+
+    __graft_entry__.load_package()
+    from laser_polio_amd import configs
+
+    t_init = time.perf_counter()
+    pars = configs.continental_pars(scale=scale)
+    pars.init_on_device = True
+    sim = configs.build(pars)
+    n0 = int(pars.init_pop.sum())
+    eng = sim._ensure_engine()
+    torch.cuda.synchronize()
+    init_s = time.perf_counter() - t_init
+    nt, p = sim.nt, sim.people
+    log(f"[bench] continental: {len(pars.init_pop)} nodes, {n0} agents, capacity {p.capacity}, "
+        f"built and drawn in {init_s:.1f}s")
+    eng.run_days(0, 1, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run_days(1, nt, use_graph=True)
+    torch.cuda.synchronize()
+    run_s = time.perf_counter() - t0
+    eng.sync()
+    sim._mark_device_newer()
+    res = {k: np.asarray(getattr(sim.results, k)) for k in CONTINENTAL_RESULTS}
+    ad = int(res["pop"][1:nt].astype(np.int64).sum())
+    props = result_properties(res)
+    rng = np.random.default_rng(0)
+    bad_nodes = []
+    for n in sorted(rng.choice(len(pars.init_pop), size=8, replace=False).tolist()):
+        a, b = int(p.seg_begin[n]), int(p.live_end[n])
+        st = eng.tensors["disease_state"][a:b].cpu().numpy().view(np.int8)
+        got = [int(np.count_nonzero(st == v)) for v in (0, 1, 2, 3)]
+        want = [int(res[k][nt - 1, n]) for k in ("S", "E", "I", "R")]
+        if got != want:
+            bad_nodes.append(n)
+    props["column_count_mismatch_nodes"] = bad_nodes
+    props["ok"] = props["ok"] and not bad_nodes
+    tx = (res["new_exposed_by_strain"] - res["sia_new_exposed_by_strain"]).astype(np.int64).sum(axis=(0, 1))
+    log(f"[bench] continental: {nt - 1} days in {run_s:.3f}s -> {ad / run_s:.4e} agent-days/s; "
+        f"transmission exposures by strain {tx.tolist()}; properties {props}")
+    out = {"agents": n0, "capacity": int(p.capacity), "days": nt - 1, "seconds": run_s, "agent_days": ad,
+           "init_s": init_s, "transmission_exposures_by_strain": tx.tolist(), "properties": props}
+    eng.close()
+    del sim, eng
+    torch.cuda.empty_cache()
+    return out
+
+
+def continental_batched(seeds, scale):
+    """`seeds` replicate seeds of the continental geography at population `scale`,
+    batched in one library context (every kernel launch of a day covers all of them);
+    each seed's results checked by result_properties."""
+    import torch
+
+    __graft_entry__.load_package()
+    from laser_polio_amd import configs
+    from laser_polio_amd.batch import build_replicates_device
+
+    plist = [configs.continental_pars(scale=scale, run_seed=42 + k) for k in range(seeds)]
+    tb0 = time.perf_counter()
+    batch = build_replicates_device(plist, configs.build)
+    try:
+        beng = batch.prepare()
+        torch.cuda.synchronize()
+        log(f"[bench] {seeds} seeds at scale {scale}: {batch.cfg.n_nodes} nodes, capacity {batch.cfg.capacity}, "
+            f"built and drawn in {time.perf_counter() - tb0:.1f}s")
+        t0 = time.perf_counter()
+        beng.run_days(0, batch.cfg.nt, use_graph=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        beng.sync()
+        bres = {}
+        for name in CONTINENTAL_RESULTS:
+            out = np.empty_like(batch.arrays[name])
+            beng.pull(name, out)
+            bres[name] = out
+        props = [result_properties({n: v[:, batch.node_off[k]:batch.node_off[k + 1]] for n, v in bres.items()})
+                 for k in range(seeds)]
+        ad = batch.agent_days(1, batch.cfg.nt)
+    finally:
+        batch.close()
+        torch.cuda.empty_cache()
+    out = {"seeds": seeds, "scale": scale, "agents": int(sum(int(q.init_pop.sum()) for q in plist)),
+           "capacity": int(batch.cfg.capacity), "seconds": round(el, 4), "agent_days_per_s": round(ad / el, 1),
+           "properties_ok": all(q["ok"] for q in props), "properties": props}
+    log(f"[bench] batched seeds: {out['agents']} agents, 365 days in {el:.3f}s -> {ad / el:.4e} agent-days/s; "
+        f"properties ok: {out['properties_ok']}")
+    return out
+
+
+def continental(args):
+    """The continental stress case (configs.continental_pars): 5000 nodes, about 1.36e9
+    agents at scale 1, three strains (relative infectivity 1 / 0.25 / 0.125), SIA every
+    120 days on 30 % of nodes alternating mOPV2 / nOPV2, births and deaths, 365 days: one
+    simulation at `--scale` (continental_single), then `--seeds` replicate seeds of the
+    same geography at `--batch-scale` batched in one context (continental_batched)."""
+    one = continental_single(args.scale)
+    batched = continental_batched(args.seeds, args.batch_scale) if args.seeds > 1 else None
+    out = {
+        "metric": "agent-days/s, continental stress case",
+        "value": round(one["agent_days"] / one["seconds"], 1),
+        "unit": "agent-days/s",
+        "n_gpus": 1,
+        "steps": one["days"],
+        "warmup": 0,
+        "ms_per_step": round(one["seconds"] * 1e3 / one["days"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i8/u16/i32 state, f32 weights, int64 fixed-point sums",
+        "data": "synthetic",
+        "config": {"workload": f"continental: 5000 nodes, {one['agents']} agents (scale {args.scale}), 3 strains, "
+                               f"SIA every 120 d (mOPV2 / nOPV2), {one['days']} days", "capacity": one["capacity"],
+                   "population_init": f"device ({one['init_s']:.1f}s with host construction, not timed)"},
+        "transmission_exposures_by_strain": one["transmission_exposures_by_strain"],
+        "properties": one["properties"],
+        "batched_seeds": batched,
+    }
+    print(json.dumps(out), flush=True)
+    if not one["properties"]["ok"] or (batched is not None and not batched["properties_ok"]):
+        sys.exit(1)
+
+
+def result_properties(res, last=None):
+    """Bookkeeping one simulation's results must satisfy through tick `last` (default: all
+    rows): pop[t] = pop[t-1] + births[t] - deaths[t] per node and day; S + E + I + R = pop;
+    the per-strain E, I and new-exposure rows sum to their totals.  `res`: results arrays
+    [t, node(, strain)].  Returns the mismatch counts."""
+    r = {k: np.asarray(v, dtype=np.int64) for k, v in res.items()}
+    if last is not None:
+        r = {k: v[: last + 1] for k, v in r.items()}
+    out = {
+        "pop_bookkeeping_mismatches": int(np.count_nonzero(r["pop"][1:] != r["pop"][:-1] + r["births"][1:] - r["deaths"][1:])),
+        "seir_vs_pop_mismatches": int(np.count_nonzero(r["S"] + r["E"] + r["I"] + r["R"] != r["pop"])),
+        "strain_sum_mismatches": int(np.count_nonzero(r["E_by_strain"].sum(-1) != r["E"])
+                                     + np.count_nonzero(r["I_by_strain"].sum(-1) != r["I"])
+                                     + np.count_nonzero(r["new_exposed_by_strain"].sum(-1) != r["new_exposed"])),
+    }
+    out["ok"] = not any(out.values())
+    return out
 
 
 def replicate_parity(batch, ks, init_state, log):

@@ -16,6 +16,16 @@ under-5s (nOPV2), 2555 days, seed 42.
 (about 4.5e6 agents), symmetric distances U(20, 250) km, no RI or SIA, 365 days;
 otherwise as above.
 
+``continental_pars(scale, seed, run_seed)`` -- the continental stress case: 5000
+nodes, populations LogNormal(ln 2e5, 0.8) (about 1.4e9 agents at scale 1),
+coordinates uniform in a 4000 x 4000 km box, three strains at relative
+infectivity 1 / 0.25 / 0.125 (VDPV2, Sabin2, nOPV2), strain-specific SIA
+campaigns every 120 days on a random 30% of nodes at 80% coverage of under-5s
+(alternately mOPV2 -> Sabin2 and nOPV2 -> nOPV2), births and deaths at CBR 35,
+no RI, 365 days; otherwise as the national case.  `seed` draws the geography,
+`run_seed` (default `seed`) is the simulation's own seed, so replicate seeds
+share one geography.
+
 ``replicate_pars(i, master_seed, scale)`` -- replicate i of the batched-replicates
 workload: a spatial14-type simulation with node count uniform 8..40, total
 population uniform 0.5..6 million (times `scale`), R0 uniform 8..18, seasonal
@@ -96,6 +106,33 @@ def nigeria_national_pars(scale=1.0, seed=42, dur=2555, n=774):
         ri_newborn_timer=42,
         vx_prob_sia=np.full(n, 0.8),
         sia_schedule=sia,
+    )
+    return PropertySet(p)
+
+
+def continental_pars(scale=1.0, seed=42, dur=365, n=5000, run_seed=None):
+    rng = np.random.default_rng(seed)
+    pop = np.maximum(1, np.round(rng.lognormal(np.log(2e5 * scale), 0.8, n))).astype(np.int64)
+    xy = rng.uniform(0, 4000, (n, 2))
+    d = np.sqrt(((xy[:, None, :] - xy[None, :, :]) ** 2).sum(-1))
+    p = _common(seed, 14, n, rng)
+    sia = []
+    vaccines = ("mOPV2", "nOPV2")
+    for j, t in enumerate(range(120, dur + 1, 120)):
+        nodes = np.sort(rng.choice(n, size=int(round(0.3 * n)), replace=False))
+        day = np.datetime64("2019-01-01") + np.timedelta64(t, "D")
+        sia.append({"date": str(day), "nodes": nodes.tolist(), "age_range": (0, 5 * 365 - 1),
+                    "vaccinetype": vaccines[j % 2]})
+    p.update(
+        seed=seed if run_seed is None else int(run_seed),
+        dur=dur,
+        init_pop=pop,
+        distances=d,
+        vx_prob_ri=None,
+        vx_prob_sia=np.full(n, 0.8),
+        sia_schedule=sia,
+        strain_ids={"VDPV2": 0, "Sabin2": 1, "nOPV2": 2},
+        strain_r0_scalars={0: 1.0, 1: 0.25, 2: 0.125},
     )
     return PropertySet(p)
 
