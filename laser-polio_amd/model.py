@@ -120,6 +120,30 @@ def timer_from_samples(x, hi=127):
     return np.clip(np.trunc(np.asarray(x, dtype=np.float64)), 0, hi).astype(np.uint16)
 
 
+def load_sim(path, pars=None, verbose=1):
+    """A model continued from a snapshot, as run_sim.py:388-405 builds it: the agents of
+    `path` (PeopleFrame.load_snapshot, capacities for pars' init_pop and cbr over dur + 10
+    days), every stock component made by its init_from_file, results.R from the snapshot,
+    pars.old_r0 from the snapshot's r0 (daily_infectivity rescaled when pars.r0 differs).
+    The components run in the stock order (default_run_order)."""
+    rp = resolve_pars(pars)
+    people, results_r, loaded = PeopleFrame.load_snapshot(path, n_ppl=rp.init_pop, cbr=rp.cbr, nt=rp.dur + 10)
+    sim = SEIR_ABM.init_from_file(people, pars, verbose)
+    if loaded and loaded.get("r0") is not None:
+        sim.pars.old_r0 = loaded["r0"]
+    made = {}
+    for cls in (DiseaseState_ABM, VitalDynamics_ABM, SIA_ABM, RI_ABM, Transmission_ABM):
+        made[cls.__name__] = cls.init_from_file(sim)
+    if results_r is not None:
+        R = sim.results.raw("R")
+        k = min(len(R), len(results_r))
+        R[:k] = results_r[:k]
+    order = [made[name] for name in default_run_order if name in made]
+    sim._components = [type(c) for c in order]
+    sim.instances = order
+    return sim
+
+
 class SimTables:
     """Everything one simulation hands to the GPU step (and to the oracle):
     agent columns, node tables, sim scalars, network, mortality table, schedule."""
@@ -129,18 +153,20 @@ class SimTables:
 
 
 # =============================================================================
-def node_capacities(pars):
-    """Agent slots per node for a run: the initial population plus the births that
-    calc_capacity expects over dur + 100 days, with a 4-sigma margin (newborns of a node
-    stay in its segment)."""
+def node_capacities(pars, frame_pop=None):
+    """Agent slots per node for a run: the agents of the frame (`frame_pop`, default the
+    initial population) plus the births that calc_capacity expects of the node's whole
+    population over dur + 100 days, with a 4-sigma margin (newborns of a node stay in its
+    segment)."""
     pop = np.atleast_1d(pars.init_pop).astype(np.int64)
+    fp = pop if frame_pop is None else np.atleast_1d(frame_pop).astype(np.int64)
     cbr = None if pars.cbr is None else np.broadcast_to(np.atleast_1d(np.asarray(pars.cbr, dtype=np.float64)), pop.shape)
-    cap = np.array(pop, dtype=np.int64)
+    cap = np.array(fp, dtype=np.int64)
     if cbr is not None:
         for n in range(len(pop)):
             eb = calc_capacity(pop[n], pars.dur + 100, cbr[n]) - pop[n]
             if eb > 0:
-                cap[n] = int((1 + 4 / np.sqrt(eb)) * (pop[n] + eb)) + 16
+                cap[n] = int((1 + 4 / np.sqrt(eb)) * (fp[n] + eb)) + 16
     return cap
 
 
@@ -186,12 +212,21 @@ class SEIR_ABM:
             pars = self.pars
             pars.init_pop = np.atleast_1d(pars.init_pop).astype(np.int64)
             if pars.init_sus_by_age is not None:
-                raise NotImplementedError("init_sus_by_age (susceptible-only initialisation) is not supported")
-            pop = pars.init_pop
+                # only the susceptibles are agents; the immune rest of each node is counted
+                # in results.R (model.py:115-121, 614-665)
+                if pars.init_on_device:
+                    raise ValueError("init_sus_by_age is initialised on the host (init_on_device must be False)")
+                by_node = pars.init_sus_by_age.groupby("node_id")["n_susceptible"].sum()
+                sus = np.zeros(len(pars.init_pop), dtype=np.int64)
+                sus[by_node.index.to_numpy(dtype=np.int64)] = by_node.to_numpy().astype(np.int64)
+                pars.init_sus = sus
+                pop = sus
+            else:
+                pop = pars.init_pop
             if np.any(pop <= 0) or np.any(np.isnan(pop.astype(np.float64))):
                 raise ValueError("pop_by_node must be positive & non-nan")
             self.device_init = bool(pars.init_on_device)
-            self.people = PeopleFrame(pop, node_capacities(pars), lazy=self.device_init)
+            self.people = PeopleFrame(pop, node_capacities(pars, frame_pop=pop), lazy=self.device_init)
             p = self.people
             p.add_scalar_property("disease_state", dtype=np.int8, default=-1)
             if not self.device_init:
@@ -212,18 +247,37 @@ class SEIR_ABM:
                     p.raw("node_id")[p.initial_slice(n)] = n
             self.results = LaserFrame(capacity=1)
             self.nodes = np.arange(len(pop))
-            self._components = []
-            self.instances = []
-            self._engine = None
-            self._touched = set()
-            self._dev_newer = set()
-            self._timing = False
-            self._device_initialized = False
-            self.init_spec = {"missed_frac": float(pars.missed_frac)}   # device initialisation inputs
+            self._init_bookkeeping()
+
+    def _init_bookkeeping(self):
+        self._components = []
+        self.instances = []
+        self._engine = None
+        self._touched = set()
+        self._dev_newer = set()
+        self._timing = False
+        self._device_initialized = False
+        self.init_spec = {"missed_frac": float(self.pars.missed_frac)}   # device initialisation inputs
 
     @classmethod
-    def init_from_file(cls, people, pars=None):
-        raise NotImplementedError("snapshot loading is not part of the GPU step")
+    def init_from_file(cls, people, pars=None, verbose=1):
+        """A model around agents loaded from a snapshot (PeopleFrame.load_snapshot;
+        model.py:177-195): nothing is drawn for the loaded agents.  Components are then
+        made with their own init_from_file (see load_sim)."""
+        model = cls.__new__(cls)
+        model.common_init(pars, verbose)
+        pars = model.pars
+        if pars.init_pop is None:
+            pars.init_pop = np.asarray(people.init_pop, dtype=np.int64)
+        pars.init_pop = np.atleast_1d(pars.init_pop).astype(np.int64)
+        if len(pars.init_pop) != people.n_nodes:
+            raise ValueError(f"init_pop has {len(pars.init_pop)} nodes, the snapshot {people.n_nodes}")
+        model.device_init = False
+        model.people = people
+        model.results = LaserFrame(capacity=1)
+        model.nodes = np.arange(people.n_nodes)
+        model._init_bookkeeping()
+        return model
 
     # -- components -------------------------------------------------------------
     @property
@@ -633,6 +687,24 @@ class SEIR_ABM:
 
 # =============================================================================
 class VitalDynamics_ABM:
+    @classmethod
+    def init_from_file(cls, sim):
+        """For a model loaded from a snapshot (model.py:1529-1537): results rows, birth rates
+        and the death table; the loaded agents keep their dates."""
+        self = cls.__new__(cls)
+        self._common_init(sim)
+        p, pars = sim.people, sim.pars
+        for name in ("date_of_birth", "date_of_death"):
+            if not p.has(name):
+                p.add_scalar_property(name, dtype=np.int32, default=AGENT_COLUMNS[name][1])
+        self.km_cd = create_cumulative_deaths(int(np.sum(pars.init_pop)), max_age_years=100)
+        if pars.cbr is not None:
+            for name in ("births", "deaths"):
+                if not self.results.has(name):
+                    self.results.add_array_property(name, shape=(sim.nt, len(self.nodes)), dtype=np.int32)
+        self._initialize_birth_rates()
+        return self
+
     """Births (node tails) and deaths by precomputed date of death (model.py:1521-1764)."""
 
     def __init__(self, sim):
@@ -676,9 +748,28 @@ class VitalDynamics_ABM:
             self.sim.init_spec["ages"] = self._age_spec()
             return
         dob = p.raw("date_of_birth")
+        table = self.pars.init_sus_by_age
         for n in range(len(self.nodes)):
             sl = p.initial_slice(n)
-            dob[sl] = -self._sample_ages(sl.stop - sl.start)
+            if table is not None:
+                dob[sl] = -self._sample_ages_by_bin(table[table["node_id"] == n], sl.stop - sl.start)
+            else:
+                dob[sl] = -self._sample_ages(sl.stop - sl.start)
+
+    def _sample_ages_by_bin(self, rows, n):
+        """Ages (days) of a node's susceptibles from its init_sus_by_age rows: a bin drawn in
+        proportion to n_susceptible, the day uniform in [age_min_yr, age_max_yr) x 365
+        (model.py:1553-1573)."""
+        rng = self.sim.rng
+        w = rows["n_susceptible"].to_numpy(dtype=np.float64)
+        lo = (rows["age_min_yr"].to_numpy(dtype=np.float64) * 365).astype(np.int64)
+        hi = (rows["age_max_yr"].to_numpy(dtype=np.float64) * 365).astype(np.int64)
+        if n == 0 or w.sum() <= 0:
+            return np.ones(n, dtype=np.int32)
+        b = rng.choice(len(w), size=n, p=w / w.sum())
+        ages = lo[b] + np.floor(rng.random(n) * np.maximum(hi[b] - lo[b], 1)).astype(np.int64)
+        ages[ages <= 0] = 1
+        return ages.astype(np.int32)
 
     def _age_spec(self):
         """Age model of the initial population for the device initialisation."""
@@ -705,10 +796,32 @@ class VitalDynamics_ABM:
                 self.sim.init_spec["vd"] = True
                 return
             dod, dob = p.raw("date_of_death"), p.raw("date_of_birth")
+            life = np.zeros(len(self.nodes))
             for n in range(len(self.nodes)):
                 sl = p.initial_slice(n)
                 ages = -dob[sl].astype(np.int64)
-                dod[sl] = km_age_at_death(self.km_cd, ages, self.sim.rng) - ages
+                at_death = km_age_at_death(self.km_cd, ages, self.sim.rng)
+                dod[sl] = at_death - ages
+                life[n] = at_death.mean() / 365 if len(at_death) else 0.0
+            pars.life_expectancies = life
+            if pars.init_sus_by_age is not None:
+                self._immune_deaths()
+
+    def _immune_deaths(self):
+        """Deaths of the immune population that init_sus_by_age leaves out of the frame,
+        drawn up front: per node and day Poisson(sum over age bins of n_immune / (remaining
+        life in days)), remaining life = max(node life expectancy - mid-bin age, 1) years;
+        none on day 0 (model.py:1651-1669)."""
+        pars = self.pars
+        df = pars.init_sus_by_age
+        mid = (df["age_min_yr"].to_numpy(dtype=np.float64) + df["age_max_yr"].to_numpy(dtype=np.float64)) / 2
+        nid = df["node_id"].to_numpy(dtype=np.int64)
+        remaining = np.maximum(pars.life_expectancies[nid] - mid, 1.0)
+        rate = df["n_immune"].to_numpy(dtype=np.float64) / (remaining * 365)
+        per_node = np.bincount(nid, weights=rate, minlength=len(self.nodes))
+        deaths = self.results.raw("deaths")
+        deaths += self.sim.rng.poisson(np.outer(np.ones(deaths.shape[0]), per_node)).astype(np.int32)
+        deaths[0, :] = 0
 
     def _initialize_birth_rates(self):
         pars = self.pars
@@ -766,6 +879,34 @@ def km_age_at_death(cd, ages_days, rng):
 class DiseaseState_ABM:
     """E->I->R progression, paralysis, seeding, early stop (model.py:505-811)."""
 
+    @classmethod
+    def init_from_file(cls, sim):
+        """For a model loaded from a snapshot (model.py:506-524): results rows; the slots not
+        yet handed out get heterogeneity values and timers drawn, potentially_paralyzed,
+        paralyzed and ipv_protected -1."""
+        self = cls.__new__(cls)
+        self._common_init(sim)
+        self._initialize_results_arrays()
+        p, pars, rng = sim.people, sim.pars, sim.rng
+        for name in AGENT_COLUMNS:
+            if name != "ri_timer" and not p.has(name):
+                dt, default = AGENT_COLUMNS[name]
+                p.add_scalar_property(name, dtype=dt, default=default)
+        risk, inf = p.raw("acq_risk_multiplier"), p.raw("daily_infectivity")
+        et, it, pt = p.raw("exposure_timer"), p.raw("infection_timer"), p.raw("paralysis_timer")
+        for n in range(p.n_nodes):
+            a, b = int(p.live_end[n]), int(p.seg_end[n])
+            if b <= a:
+                continue
+            populate_heterogeneous_values(a, b, risk, inf, pars, rng)
+            et[a:b] = timer_from_samples(pars.dur_exp(b - a))
+            it[a:b] = timer_from_samples(pars.dur_inf(b - a))
+            raw = pars.t_to_paralysis(b - a) - et[a:b]
+            pt[a:b] = np.clip(raw, 0, np.minimum(it[a:b], 127)).astype(np.int64).astype(np.uint16)
+            for name in ("potentially_paralyzed", "paralyzed", "ipv_protected"):
+                p.raw(name)[a:b] = -1
+        return self
+
     def __init__(self, sim):
         self._common_init(sim)
         self._initialize_results_arrays()
@@ -786,7 +927,9 @@ class DiseaseState_ABM:
             pt[b0:b1] = np.clip(raw, 0, np.minimum(it[b0:b1], 127)).astype(np.int64).astype(np.uint16)
         state = p.raw("disease_state")
         N = len(pars.init_pop)
-        if isinstance(pars.init_immun, (float, int)) and not isinstance(pars.init_immun, bool):
+        if pars.init_sus_by_age is not None:
+            self._init_susceptibles_only()
+        elif isinstance(pars.init_immun, (float, int)) and not isinstance(pars.init_immun, bool):
             fr = np.full(N, float(pars.init_immun))
         elif isinstance(pars.init_immun, (list, tuple, np.ndarray)):
             fr = np.asarray(pars.init_immun, dtype=np.float64)
@@ -794,7 +937,7 @@ class DiseaseState_ABM:
                 raise ValueError("init_immun must match init_pop shape")
         else:
             raise ValueError(f"Unsupported init_immun type: {type(pars.init_immun)}")
-        for n in range(N):
+        for n in range(N if pars.init_sus_by_age is None else 0):
             if not (0.0 <= fr[n] <= 1.0):
                 raise ValueError(f"Invalid immunity fraction: {fr[n]} for node {n}")
             k = int(fr[n] * pars.init_pop[n])
@@ -809,7 +952,7 @@ class DiseaseState_ABM:
                 if not p.has("date_of_birth"):
                     raise ValueError("init_recovered_by_age needs VitalDynamics_ABM (ages)")
                 sim.init_spec["bands"] = [(lo * 365, hi * 365, frac) for lo, hi, frac in pars.init_recovered_by_age]
-        elif pars.init_recovered_by_age is not None:
+        elif pars.init_recovered_by_age is not None and pars.init_sus_by_age is None:
             dob = p.raw("date_of_birth") if p.has("date_of_birth") else None
             if dob is None:
                 raise ValueError("init_recovered_by_age needs VitalDynamics_ABM (ages)")
@@ -822,7 +965,7 @@ class DiseaseState_ABM:
                     band = (ages >= lo_y * 365) & (ages < hi_y * 365)
                     rec |= band & (u < frac)
                 state[sl][rec] = 3
-        total = int(pars.init_pop.sum())
+        total = int(p.init_pop.sum())     # initial agents of the frame
         idx = []
         if isinstance(pars.init_prev, float):
             idx = p.initial_to_slot(rng.choice(total, size=int(total * pars.init_prev), replace=False))
@@ -849,6 +992,30 @@ class DiseaseState_ABM:
             sim.init_spec["infected"] = np.asarray(idx, dtype=np.int64)
         elif len(idx):
             state[np.asarray(idx, dtype=np.int64)] = 2
+
+    def _init_susceptibles_only(self):
+        """init_sus_by_age (model.py:614-665): every agent of the frame is susceptible; the
+        immune remainder of each node (init_pop - init_sus) is added to every row of
+        results.R, less the cumulative immune deaths VitalDynamics drew up front; per age
+        bin, n_ipv_protected of the node's agents in the bin are IPV-protected."""
+        pars, p, rng = self.pars, self.people, self.sim.rng
+        R = self.results.raw("R")
+        R += (pars.init_pop - pars.init_sus).astype(np.int32)[None, :]
+        df = pars.init_sus_by_age
+        ipv = p.raw("ipv_protected")
+        if "n_ipv_protected" in df.columns and p.has("date_of_birth"):
+            dob = p.raw("date_of_birth")
+            rows = df[df["n_ipv_protected"] > 0]
+            for node in rows["node_id"].unique():
+                sl = p.initial_slice(int(node))
+                ages_y = -dob[sl].astype(np.float64) / 365.0
+                for _, row in rows[rows["node_id"] == node].iterrows():
+                    elig = sl.start + np.nonzero((ages_y >= row["age_min_yr"]) & (ages_y < row["age_max_yr"]))[0]
+                    k = min(int(row["n_ipv_protected"]), len(elig))
+                    if k > 0:
+                        ipv[rng.choice(elig, size=k, replace=False)] = 1
+        if self.results.has("deaths"):
+            R -= np.cumsum(self.results.raw("deaths"), axis=0).astype(np.int32)
 
     def _common_init(self, sim):
         self.sim = sim
@@ -984,6 +1151,26 @@ def heterogeneity_spec(pars):
 class Transmission_ABM:
     """Force of infection over the node network and infection draws (model.py:1152-1490)."""
 
+    @classmethod
+    def init_from_file(cls, sim):
+        """For a model loaded from a snapshot (model.py:1169-1188): network and results; when
+        pars.r0 differs from the snapshot's (pars.old_r0), every daily_infectivity is scaled
+        by r0 / old_r0."""
+        self = cls.__new__(cls)
+        self.sim = sim
+        self.people = sim.people
+        self.nodes = np.arange(len(sim.pars.init_pop))
+        self.pars = sim.pars
+        self.results = sim.results
+        self.verbose = sim.pars["verbose"] if "verbose" in sim.pars else 1
+        self.r0_scalars = np.array(self.pars.r0_scalars)
+        old = getattr(sim.pars, "old_r0", None)
+        if old is not None and sim.pars.r0 != old:
+            inf = sim.people.raw("daily_infectivity")
+            inf *= np.float32(sim.pars.r0 / old)
+        self._initialize_common()
+        return self
+
     def __init__(self, sim):
         self.sim = sim
         self.people = sim.people
@@ -1045,6 +1232,23 @@ class Transmission_ABM:
 class RI_ABM:
     """Routine OPV + IPV every step_size days (model.py:1858-1991)."""
 
+    @classmethod
+    def init_from_file(cls, sim):
+        """For a model loaded from a snapshot (model.py:1873-1885): results rows; the loaded
+        agents keep their ri_timer."""
+        self = cls.__new__(cls)
+        self.sim = sim
+        self.step_size = sim.pars.step_size_RI_ABM
+        self.people = sim.people
+        self.nodes = sim.nodes
+        self.pars = sim.pars
+        self.results = sim.results
+        self.verbose = self.pars["verbose"] if "verbose" in self.pars else 1
+        if not self.people.has("ri_timer"):
+            self.people.add_scalar_property("ri_timer", dtype=np.int16, default=-1)
+        self._initialize_results()
+        return self
+
     def __init__(self, sim):
         self.sim = sim
         self.step_size = sim.pars.step_size_RI_ABM
@@ -1066,7 +1270,10 @@ class RI_ABM:
             base = dob[sl] if dob is not None else np.full(sl.stop - sl.start, -1, np.int32)
             days = sim.rng.uniform(lo_hi[0], lo_hi[1], sl.stop - sl.start) if lo_hi[1] > lo_hi[0] else np.full(sl.stop - sl.start, float(lo_hi[0]))
             ri[sl] = (base + days).astype(np.int32).astype(np.int16)
-        nt, N, K = sim.nt, len(sim.nodes), len(self.pars.strain_ids)
+        self._initialize_results()
+
+    def _initialize_results(self):
+        nt, N, K = self.sim.nt, len(self.sim.nodes), len(self.pars.strain_ids)
         self.results.add_array_property("ri_vaccinated", shape=(nt, N), dtype=np.int32)
         self.results.add_array_property("ri_protected", shape=(nt, N), dtype=np.int32)
         self.results.add_array_property("ri_new_exposed_by_strain", shape=(nt, N, K), dtype=np.int32)
@@ -1084,6 +1291,11 @@ class RI_ABM:
 
 class SIA_ABM:
     """Campaign vaccination on scheduled dates (model.py:2063-2162)."""
+
+    @classmethod
+    def init_from_file(cls, sim):
+        """For a model loaded from a snapshot (model.py:2070-2075): results and schedule."""
+        return cls(sim)
 
     def __init__(self, sim):
         self.sim = sim

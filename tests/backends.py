@@ -56,7 +56,7 @@ def seeding_of(sim):
 
 def oracle_of(sim, **kw):
     """OracleSim of a constructed (not yet run) sim."""
-    results0 = {k: sim.results.raw(k) for k in ("R",) if sim.results.has(k)}
+    results0 = {k: sim.results.raw(k) for k in ("R", "deaths") if sim.results.has(k)}
     return OracleSim(sim.export_tables(), pop0=sim.pars.init_pop, results0=results0, seeding=seeding_of(sim), **kw)
 
 
