@@ -202,6 +202,26 @@ int64_t lp_sizeof_init_sim(void);
 int lp_init_population(lp_ctx* ctx, const lp_init_sim* sims, int32_t n_sims, const int64_t* infected_slots,
                        int64_t n_infected);
 
+/* Gravity network of one simulation (Transmission_ABM._initialize_common, model.py:1213-1258:
+ * gravity(init_pops, dist, k * 10^k_exponent, a, b, c) / sum(init_pops)^c, then
+ * row_normalizer(net, max_migr_frac)). */
+typedef struct {
+  double k;            /* gravity_k * 10^gravity_k_exponent */
+  double a, b, c;
+  double max_rowsum;   /* max_migr_frac */
+  int32_t reserved[2];
+} lp_gravity_spec;
+
+/* Build every simulation's network on the device into the bound "network_t" (each
+ * simulation's block transposed, float32) and "keep" (1 - row sum), from one spec per
+ * simulation (host array), the node populations (device, float64, one per node of the
+ * context) and the distances (device, float64, each simulation's n x n block row-major at
+ * the same offset as its "network_t" block).  Replaces the host construction of
+ * model.py:1213-1258 for calibration loops that rebuild networks per trial.  Takes effect
+ * from the next simulated day; queued on the context stream. */
+int lp_build_network(lp_ctx* ctx, const lp_gravity_spec* specs, int32_t n_sims, const double* pop,
+                     const double* dist);
+
 /* Wait for queued work; returns LP_ERR_CAPACITY / LP_ERR_OVERFLOW if the device flagged one. */
 int lp_sync(lp_ctx* ctx);
 

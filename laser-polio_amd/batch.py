@@ -90,6 +90,7 @@ class BatchedSims:
         self.tiles_per_block = int(tiles_per_block)
         self._owner = None
         self.device_init = None    # (specs, infected slots per simulation) when drawn on the GPU
+        self._net_inputs = None    # device populations and distances for set_gravity
         self._pack(cols)
         self.engine = None
 
@@ -102,6 +103,7 @@ class BatchedSims:
         self.slot_off = np.concatenate([[0], np.cumsum(caps)])
         self.node_off = np.concatenate([[0], np.cumsum(nns)])
         net_off = np.concatenate([[0], np.cumsum(nns * nns)])
+        self.net_off = net_off
         N = int(self.node_off[-1])
         if cols is None:
             cols = {name: np.concatenate([tb.cols[name] for tb in tbs]) for name in AGENT_COLUMNS}
@@ -148,6 +150,38 @@ class BatchedSims:
                 slots = [np.asarray(x, np.int64) + self.slot_off[k] for k, x in enumerate(infected)]
                 self.engine.init_population(specs, np.concatenate(slots) if slots else np.zeros(0, np.int64))
         return self.engine
+
+    def set_gravity(self, k=None, a=None, b=None, c=None, max_rowsum=None, pull=True):
+        """Rebuild every simulation's gravity network on the device (lp_build_network).
+        Each of k (gravity_k * 10^gravity_k_exponent), a, b, c and max_rowsum (max_migr_frac)
+        is None (the simulation's current value), a scalar, or one value per simulation.
+        With `pull` the host copies of network_t and keep (and each simulation's tables)
+        are refreshed, so a simulation still replays alone with its device-built network."""
+        import torch
+
+        S = len(self.tables)
+        if any(getattr(tb, "gravity_spec", None) is None for tb in self.tables):
+            raise ValueError("set_gravity needs gravity networks (migration_method 'gravity') in every simulation")
+        cols = []
+        for i, v in enumerate((k, a, b, c, max_rowsum)):
+            cur = [tb.gravity_spec[i] for tb in self.tables]
+            cols.append(cur if v is None else np.broadcast_to(np.asarray(v, dtype=np.float64), (S,)).tolist())
+        specs = [tuple(float(x) for x in row) for row in zip(*cols)]
+        eng = self.prepare()
+        if self._net_inputs is None:
+            pop = np.concatenate([np.asarray(tb.init_pop, np.float64) for tb in self.tables])
+            dist = np.concatenate([np.asarray(tb.distances, np.float64).reshape(-1) for tb in self.tables])
+            self._net_inputs = (torch.as_tensor(pop, device="cuda"), torch.as_tensor(dist, device="cuda"))
+        eng.build_network(specs, *self._net_inputs)
+        for tb, spec in zip(self.tables, specs):
+            tb.gravity_spec = spec
+        if pull:
+            eng.pull("network_t", self.arrays["network_t"])
+            eng.pull("keep", self.arrays["keep"])
+            for i, tb in enumerate(self.tables):
+                tb.network_t = self.arrays["network_t"][self.net_off[i]:self.net_off[i + 1]].copy()
+                tb.keep = self.arrays["keep"][self.node_off[i]:self.node_off[i + 1]].copy()
+        return specs
 
     def pull_sim(self, k, names=None):
         """Agent columns (sim-local slots) and results of simulation k from the device."""

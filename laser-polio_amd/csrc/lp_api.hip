@@ -2,6 +2,7 @@
 // validation, span tables, scratch, the per-day kernel sequence, hipGraph
 // capture/replay and error reporting.
 #include "lp_init.hip"
+#include "lp_network.hip"
 #include <string>
 #include <vector>
 #include <cstring>
@@ -115,6 +116,8 @@ struct lp_ctx {
   size_t ev_used = 0;
   int64_t time_launches = 0;
   double time_ms = 0.0;
+  lp_gravity_spec* net_specs = nullptr;            // lp_build_network inputs (n_sims), allocated on first use
+  double* net_scratch = nullptr;                   // row scales and 1 / S^c per node
 };
 
 namespace {
@@ -630,6 +633,36 @@ int lp_init_population(lp_ctx* c, const lp_init_sim* sims, int32_t n_sims, const
   c->recount_pending = true;
   c->list_tick = -1;
   c->rec_dirty = false;
+  return LP_OK;
+}
+
+int lp_build_network(lp_ctx* c, const lp_gravity_spec* specs, int32_t n_sims, const double* pop, const double* dist) {
+  if (!c || !specs || !pop || !dist) return LP_ERR_ARG;
+  if (!c->prepared) return fail(c, LP_ERR_STATE, "lp_build_network before lp_prepare");
+  if (n_sims != c->cfg.n_sims) return fail(c, LP_ERR_ARG, "lp_build_network: one lp_gravity_spec per simulation");
+  int max_n = 0;
+  for (int s = 0; s < n_sims; ++s) {
+    const lp_gravity_spec& g = specs[s];
+    if (!(g.max_rowsum >= 0.0) || !(g.k >= 0.0)) return fail(c, LP_ERR_ARG, "lp_build_network: bad spec for sim " + std::to_string(s));
+  }
+  std::vector<lp_sim_params> sp((size_t)n_sims);
+  HIPCHK(c, hipMemcpyAsync(sp.data(), c->d.sims, sizeof(lp_sim_params) * n_sims, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (const auto& p : sp) max_n = std::max(max_n, (int)p.n_nodes);
+  int rc;
+  if (!c->net_specs && (rc = alloc(c, &c->net_specs, (size_t)n_sims))) return rc;
+  if (!c->net_scratch && (rc = alloc(c, &c->net_scratch, 2 * (size_t)c->cfg.n_nodes))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->net_specs, specs, sizeof(lp_gravity_spec) * n_sims, hipMemcpyHostToDevice, c->stream));
+  double* scale = c->net_scratch;
+  double* inv_sc = c->net_scratch + c->cfg.n_nodes;
+  hipLaunchKernelGGL(lp::k_net_rows, dim3(c->cfg.n_nodes), dim3(lp::WG), 0, c->stream, c->d, c->net_specs, pop, dist, scale,
+                     inv_sc);
+  const int tiles = (max_n + lp::NET_TILE - 1) / lp::NET_TILE;
+  hipLaunchKernelGGL(lp::k_net_tiles, dim3(tiles * tiles, n_sims), dim3(lp::WG), 0, c->stream, c->d, c->net_specs, pop,
+                     dist, scale, inv_sc);
+  HIPCHK(c, hipGetLastError());
+  // the specs are copied before the call returns (pageable host memory)
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return LP_OK;
 }
 

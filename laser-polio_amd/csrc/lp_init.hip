@@ -1,3 +1,4 @@
+#pragma once
 // Population initialisation on the device (lp_init_population, include/lp_step.h):
 // the per-agent columns that SEIR_ABM and its components draw at construction
 // (model.py:561-713 ages, immunity, initial infections; 816-866 heterogeneity;

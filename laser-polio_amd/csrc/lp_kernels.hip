@@ -1,3 +1,4 @@
+#pragma once
 // Kernels of the laser-polio daily agent step on MI355X (gfx950, wave64).
 //
 // One simulated day is this kernel sequence (all stages fused, reference order

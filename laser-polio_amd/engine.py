@@ -91,6 +91,12 @@ class LPDist(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_float), ("b", ctypes.c_float)]
 
 
+class LPGravitySpec(ctypes.Structure):
+    """lp_gravity_spec of include/lp_step.h."""
+    _fields_ = [("k", ctypes.c_double), ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
+                ("max_rowsum", ctypes.c_double), ("reserved", ctypes.c_int32 * 2)]
+
+
 class LPInitSim(ctypes.Structure):
     """lp_init_sim of include/lp_step.h: the device initialisation of one simulation."""
 
@@ -160,6 +166,7 @@ def load_library(path=LIB_PATH):
     lib.lp_agent_pass_launch_ms.restype = i64
     lib.lp_sizeof_init_sim.restype = i64
     lib.lp_init_population.argtypes = [vp, ctypes.POINTER(LPInitSim), i32, vp, i64]
+    lib.lp_build_network.argtypes = [vp, ctypes.POINTER(LPGravitySpec), i32, vp, vp]
     lib.lp_last_error.argtypes = [vp]
     lib.lp_last_error.restype = ctypes.c_char_p
     lib.lp_destroy.argtypes = [vp]
@@ -292,6 +299,22 @@ class Engine:
         """On: lp_run_days launches each day's kernels directly and brackets its agent pass
         with HIP events (read by agent_pass_timing); off: days replay their hipGraphs."""
         self._check(self.lib.lp_set_timing(self.ctx, 1 if on else 0))
+
+    def build_network(self, specs, pop, dist):
+        """Every simulation's gravity network on the device (lp_build_network): `specs` =
+        one (k, a, b, c, max_rowsum) per simulation, `pop` / `dist` = float64 device tensors
+        (one population per node; each simulation's n x n distances at its network offset)."""
+        import torch
+
+        if pop.dtype != torch.float64 or dist.dtype != torch.float64 or not pop.is_cuda or not dist.is_cuda:
+            raise ValueError("build_network: pop and dist must be float64 device tensors")
+        if pop.numel() != self.cfg.n_nodes or dist.numel() != self.tensors["network_t"].numel():
+            raise ValueError("build_network: pop needs one value per node, dist the size of network_t")
+        arr = (LPGravitySpec * len(specs))()
+        for i, (k, a, b, c, mx) in enumerate(specs):
+            arr[i].k, arr[i].a, arr[i].b, arr[i].c, arr[i].max_rowsum = k, a, b, c, mx
+        self._check(self.lib.lp_build_network(self.ctx, arr, len(specs), ctypes.c_void_p(pop.data_ptr()),
+                                              ctypes.c_void_p(dist.data_ptr())))
 
     def agent_pass_launch_ms(self):
         """Milliseconds of each event-bracketed agent-pass launch since the last

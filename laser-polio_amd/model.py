@@ -381,9 +381,11 @@ class SEIR_ABM:
             net = np.asarray(tx.network, dtype=np.float64)
             keep = (1.0 - net.sum(axis=1)).astype(np.float32)
             network_t = np.ascontiguousarray(net.T.astype(np.float32)).reshape(-1)
+            dist, gspec = getattr(tx, "distances", None), getattr(tx, "gravity_spec", None)
         else:
             keep = np.ones(N, np.float32)
             network_t = np.zeros(N * N, np.float32)
+            dist, gspec = None, None
         if vd is not None:
             # the reference's step_size * birth_rate (model.py:1712), kept in float64
             brs = vd.step_size * np.asarray(vd.birth_rate, dtype=np.float64)
@@ -413,6 +415,9 @@ class SEIR_ABM:
             r0s=r0s,
             keep=keep,
             network_t=network_t,
+            distances=dist,
+            gravity_spec=gspec,
+            init_pop=np.asarray(pars.init_pop, dtype=np.float64),
             birth_rate_step=brs,
             thr_ri=thr_ri,
             thr_ipv=thr_ipv,
@@ -1205,8 +1210,12 @@ class Transmission_ABM:
             off = ~np.eye(len(ids), dtype=bool)
             dist[off & (dist == 0)] = 1.0
         method = pars.migration_method.lower()
+        self.distances = dist
+        self.gravity_spec = None
         if method == "gravity":
             k = pars.gravity_k * 10 ** (pars.gravity_k_exponent)
+            self.gravity_spec = (float(k), float(pars.gravity_a), float(pars.gravity_b), float(pars.gravity_c),
+                                 float(pars.max_migr_frac))
             net = gravity(init_pops, dist, k, pars.gravity_a, pars.gravity_b, pars.gravity_c)
             net /= np.power(init_pops.sum(), pars.gravity_c)
         elif method == "radiation":
