@@ -900,47 +900,26 @@ def adjoint_result(scale, init="device"):
     __graft_entry__.load_package()
     from laser_polio_amd import adjoint as adj
     from laser_polio_amd import configs
-    import calendar
 
     T = 180
     pars = configs.nigeria_national_pars(scale=scale, dur=T)
     pars.init_on_device = init == "device"
     sim = configs.build(pars)
-    agg = adj.aggregate(sim)
-    tx = sim._instance("Transmission_ABM")
     rp = sim.pars
-    doy = np.array([sim.datevec[t].timetuple().tm_yday for t in range(1, T + 1)], dtype=np.float64)
-    diy = np.array([366.0 if calendar.isleap(sim.datevec[t].year) else 365.0 for t in range(1, T + 1)])
-    dist = np.asarray(rp.distances, dtype=np.float64)
     log(f"[bench] adjoint: {len(rp.init_pop)} nodes, {int(rp.init_pop.sum())} agents (scale {scale}), {T} days")
+    dev = torch.device("cuda", 0)
+    prob = adj.MeanFieldProblem(sim, T, torch.float32, dev)     # compartments of day 0
     t0 = time.perf_counter()
     sim.run()
     y_daily = (sim.results.new_exposed.astype(np.int64) - sim.results.ri_protected - sim.results.sia_protected)[1:T + 1]
     log(f"[bench] ABM target: {int(y_daily.sum())} transmission exposures in {time.perf_counter() - t0:.1f}s")
-    dev = torch.device("cuda", 0)
-
-    def tensors(dtype, device):
-        t = lambda x: torch.as_tensor(np.asarray(x, dtype=np.float64), dtype=dtype, device=device)  # noqa: E731
-        const = {k: t(agg[k]) for k in ("S", "E", "I", "rho", "inv_pop")}
-        params = {"r0s": t(np.asarray(tx.r0_scalars, dtype=np.float64) * 0.9), "k": t(rp.gravity_k * 0.8),
-                  "a": t(rp.gravity_a), "b": t(rp.gravity_b), "c": t(rp.gravity_c),
-                  "amp": t(rp.seasonal_amplitude * 1.2), "peak": t(float(rp.seasonal_peak_doy))}
-        return const, params, t(doy), t(diy), t(rp.init_pop), t(dist)
-
-    def loss(params, const, doy_t, diy_t, pop_t, dist_t, y, fn):
-        net = adj.gravity_network(pop_t, dist_t, params["k"], params["a"], params["b"], params["c"],
-                                  float(rp.max_migr_frac), float(rp.gravity_k_exponent))
-        season = adj.seasonality(params["amp"], params["peak"], doy_t, diy_t)
-        inc = fn(params["r0s"], net, season, const["S"], const["E"], const["I"], const["rho"], const["inv_pop"],
-                 agg["iota"], agg["inv_tau_e"], agg["inv_tau_i"])
-        return adj.poisson_nll(adj.weekly(inc), y)
-
-    c32, p32, doy32, diy32, pop32, dist32 = tensors(torch.float32, dev)
-    y32 = adj.weekly(torch.as_tensor(y_daily, dtype=torch.float32, device=dev))
+    prob.set_target(y_daily)
+    perturb = {"r0s": 0.9, "k": 0.8, "amp": 1.2}
+    p32 = prob.params(**perturb)
 
     def grad_hip():
         p = {k: v.clone().requires_grad_(True) for k, v in p32.items()}
-        L = loss(p, c32, doy32, diy32, pop32, dist32, y32, adj.rollout)
+        L = prob.loss(p, adj.rollout)
         L.backward()
         return float(L), {k: v.grad.detach() for k, v in p.items()}
 
@@ -954,15 +933,15 @@ def adjoint_result(scale, init="device"):
         times.append(time.perf_counter() - t0)
     p = {k: v.clone().requires_grad_(True) for k, v in p32.items()}
     t0 = time.perf_counter()
-    L_ag = loss(p, c32, doy32, diy32, pop32, dist32, y32, adj.rollout_reference)
+    L_ag = prob.loss(p, adj.rollout_reference)
     L_ag.backward()
     torch.cuda.synchronize()
     t_ag = time.perf_counter() - t0
     rel_ag = {k: float((g_hip[k].double() - p[k].grad.double()).norm() / p[k].grad.double().norm().clamp_min(1e-30))
               for k in p32}
-    c64, p64, doy64, diy64, pop64, dist64 = tensors(torch.float64, "cpu")
-    y64 = y32.double().cpu()
-    f64 = lambda q: float(loss(q, c64, doy64, diy64, pop64, dist64, y64, adj.rollout_reference))  # noqa: E731
+    prob.to(torch.float64, "cpu")
+    p64 = prob.params(**perturb)
+    f64 = lambda q: float(prob.loss(q, adj.rollout_reference))  # noqa: E731
     fd_err = {}
     for name, idx in [("k", 0), ("a", 0), ("b", 0), ("c", 0), ("amp", 0), ("peak", 0)] + [("r0s", i) for i in (0, 100, 400)]:
         base = {k: v.clone() for k, v in p64.items()}

@@ -204,3 +204,67 @@ def aggregate(sim):
         "iota": iota, "inv_tau_e": 1.0 / max(1e-6, float(pars.dur_exp.mean())),
         "inv_tau_i": 1.0 / max(1e-6, float(pars.dur_inf.mean())),
     }
+
+
+class MeanFieldProblem:
+    """The mean-field calibration loss of a constructed SEIR_ABM: node compartments
+    aggregated from its agents when the problem is made (aggregate: make it before the
+    simulation runs), its geography, seasonality calendar and gravity settings, T days;
+    with a target set (set_target: daily per-node incidence [T, nodes]), loss(params) is
+    the Poisson negative log-likelihood of weekly incidence of the T-day rollout.
+    params: r0s (per node), k, a, b, c (gravity), amp, peak (seasonality)."""
+
+    NAMES = ("r0s", "k", "a", "b", "c", "amp", "peak")
+
+    def __init__(self, sim, T, dtype=torch.float32, device="cuda"):
+        import calendar
+
+        agg = aggregate(sim)
+        rp = sim.pars
+        tx = sim._instance("Transmission_ABM")
+        self.agg, self.rp, self.T = agg, rp, int(T)
+        dist = getattr(tx, "distances", None)
+        dist = np.asarray(rp.distances if dist is None else dist, dtype=np.float64)
+        doy = np.array([sim.datevec[t].timetuple().tm_yday for t in range(1, T + 1)], dtype=np.float64)
+        diy = np.array([366.0 if calendar.isleap(sim.datevec[t].year) else 365.0 for t in range(1, T + 1)])
+        self._np = {"doy": doy, "diy": diy, "pop": np.asarray(rp.init_pop, dtype=np.float64), "dist": dist, "y": None}
+        self._base = {"r0s": np.asarray(tx.r0_scalars, dtype=np.float64) * np.ones(len(rp.init_pop)),
+                      "k": float(rp.gravity_k), "a": float(rp.gravity_a), "b": float(rp.gravity_b),
+                      "c": float(rp.gravity_c), "amp": float(rp.seasonal_amplitude), "peak": float(rp.seasonal_peak_doy)}
+        self.y = None
+        self.to(dtype, device)
+
+    def _t(self, x):
+        return torch.as_tensor(np.asarray(x, dtype=np.float64), dtype=self.dtype, device=self.device)
+
+    def to(self, dtype, device):
+        self.dtype, self.device = dtype, device
+        self.const = {k: self._t(self.agg[k]) for k in ("S", "E", "I", "rho", "inv_pop")}
+        self.doy, self.diy, self.pop, self.dist = (self._t(self._np[k]) for k in ("doy", "diy", "pop", "dist"))
+        self.y = None if self._np["y"] is None else weekly(self._t(self._np["y"]))
+        return self
+
+    def set_target(self, y_daily):
+        y = np.asarray(y_daily, dtype=np.float64)
+        if y.shape != (self.T, len(self._np["pop"])):
+            raise ValueError(f"target must be [T={self.T}, nodes], got {y.shape}")
+        self._np["y"] = y
+        self.y = weekly(self._t(y))
+        return self
+
+    def params(self, **scale):
+        """The simulation's own parameter values as tensors, each multiplied by scale[name]."""
+        return {k: self._t(np.asarray(v, dtype=np.float64) * scale.get(k, 1.0)) for k, v in self._base.items()}
+
+    def incidence(self, params, fn=rollout):
+        rp, agg = self.rp, self.agg
+        net = gravity_network(self.pop, self.dist, params["k"], params["a"], params["b"], params["c"],
+                              float(rp.max_migr_frac), float(rp.gravity_k_exponent))
+        season = seasonality(params["amp"], params["peak"], self.doy, self.diy)
+        return fn(params["r0s"], net, season, self.const["S"], self.const["E"], self.const["I"], self.const["rho"],
+                  self.const["inv_pop"], agg["iota"], agg["inv_tau_e"], agg["inv_tau_i"])
+
+    def loss(self, params, fn=rollout):
+        if self.y is None:
+            raise ValueError("set_target before loss")
+        return poisson_nll(weekly(self.incidence(params, fn)), self.y)
