@@ -12,6 +12,7 @@ mean of 200 seeds must be within 4 standard errors (or 0.005) of z.
 
 import numpy as np
 import pytest
+from scipy import stats
 from scipy.optimize import brentq
 
 from laser_polio_amd import DiseaseState_ABM, SEIR_ABM, Transmission_ABM, VitalDynamics_ABM, configs
@@ -79,3 +80,87 @@ def test_200_seed_final_size_matches_kermack_mckendrick():
         print(f"[ensemble] R0 {r0} (effective {r0_eff:.4f}): mean final size {m:.5f} +- {se:.5f}, "
               f"Kermack-McKendrick {z:.5f}", flush=True)
         assert abs(m - z) < max(4 * se, 0.005), (r0, m, se, z)
+
+
+# -- the reference's own random streams and number formats ---------------------------------------
+def _six_nodes(seed, dur=120):
+    rng = np.random.default_rng(2024)          # one geography for every seed
+    n = 6
+    pop = np.round(rng.lognormal(np.log(1e4), 0.3, n)).astype(np.int64)
+    d = rng.uniform(20, 150, (n, n))
+    d = np.triu(d, 1)
+    d = d + d.T
+    p = configs.spatial14_pars(scale=0.01, seed=seed, dur=dur)
+    p.init_pop, p.distances = pop, d
+    p.r0_scalars = rng.lognormal(0.0, 0.3, n)
+    p.init_prev = [0.002] * n
+    p.cbr = None
+    p.seed = seed
+    return p
+
+
+def _six_build(p):
+    sim = SEIR_ABM(p)
+    sim.components = [VitalDynamics_ABM, DiseaseState_ABM, Transmission_ABM]
+    return sim
+
+
+def _refsim_totals(args):
+    from oracle.refsim import ReferenceSim
+
+    tables, pop0, seed = args
+    r = ReferenceSim(tables, pop0, seed).run()
+    return r.res["new_exposed"][1:].sum(axis=0)
+
+
+def _compare_ensembles(a, b, label):
+    """Per-node totals [seeds, nodes] of two ensembles: means within 3 standard errors per
+    node and KS p > 0.01 per node (6 nodes), the all-node total within 2 standard errors."""
+    out = []
+    for n in range(a.shape[1]):
+        se = np.sqrt(a[:, n].var(ddof=1) / len(a) + b[:, n].var(ddof=1) / len(b))
+        p = stats.ks_2samp(a[:, n], b[:, n]).pvalue
+        out.append((n, a[:, n].mean(), b[:, n].mean(), se, p))
+        assert abs(a[:, n].mean() - b[:, n].mean()) < 3 * se, (label, out[-1])
+        assert p > 0.01, (label, out[-1])
+    ta, tb_ = a.sum(axis=1), b.sum(axis=1)
+    se = np.sqrt(ta.var(ddof=1) / len(ta) + tb_.var(ddof=1) / len(tb_))
+    assert abs(ta.mean() - tb_.mean()) < 2 * se, (label, ta.mean(), tb_.mean(), se)
+    return out
+
+
+def test_reference_semantics_match_the_oracle_in_distribution():
+    """The restatement with the reference's own streams against the oracle (the GPU's
+    draws) over 60 seeds from the same initial populations: the substitutions leave the
+    per-node incidence distribution unchanged (CPU check of the checker)."""
+    from oracle.oracle import OracleSim
+    from oracle.refsim import ReferenceSim
+
+    a, b = [], []
+    for s in range(60):
+        sim = _six_build(_six_nodes(100 + s))
+        tb = sim.export_tables()
+        a.append(OracleSim(tb, pop0=sim.pars.init_pop).run().res["new_exposed"][1:].sum(axis=0))
+        b.append(ReferenceSim(tb, sim.pars.init_pop, 5000 + s).run().res["new_exposed"][1:].sum(axis=0))
+    _compare_ensembles(np.array(a), np.array(b), "oracle vs reference streams")
+
+
+@pytest.mark.gpu
+def test_200_seed_ensemble_matches_reference_streams():
+    """200 seeds of a 6-node epidemic on the GPU (batched) against 200 runs of the
+    reference's algorithm with its own random streams from the same initial populations:
+    per-node incidence means within 3 SE, KS p > 0.01, total within 2 SE."""
+    import laser_polio_amd as lp
+
+    sims = [_six_build(_six_nodes(100 + s)) for s in range(200)]
+    # the reference-stream runs first: the exported tables share the agent columns, which
+    # the batch writes back after its run
+    print("[ensemble] 200 runs with the reference's streams on the host", flush=True)
+    ref = np.array([_refsim_totals((s.export_tables(), s.pars.init_pop, 5000 + i)) for i, s in enumerate(sims)])
+    print("[ensemble] 200 x 6-node simulations on the GPU", flush=True)
+    b = lp.BatchedSims(sims).run(use_graph=True)
+    b.close()
+    gpu = np.array([np.asarray(s.results.new_exposed)[1:].sum(axis=0) for s in sims])
+    rows = _compare_ensembles(gpu, ref, "GPU vs reference streams")
+    for n, ma, mb, se, p in rows:
+        print(f"[ensemble] node {n}: GPU {ma:.1f} reference streams {mb:.1f} (SE {se:.1f}) KS p {p:.3f}", flush=True)
