@@ -597,11 +597,16 @@ def replicates(args):
             batch.tiles_per_block = tpw
             batch.cfg.tiles_per_block = tpw
         else:
-            tpw, tune = batch.autotune()
+            tpw, tune = batch.autotune(grids=True)
             log(f"[bench] autotune tiles_per_block: " + ", ".join(f"{k}: {v * 1e3:.1f} ms" for k, v in tune.items())
-                + f" -> {tpw}")
+                + f" -> {tpw}; grids " + ", ".join(
+                    f"{g} {getattr(batch.cfg, g)} ({', '.join(f'{c}: {v * 1e3:.1f}' for c, v in batch.tune_log[g].items())} ms)"
+                    for g in batch.GRID_CANDIDATES))
         sample = sorted(set(np.linspace(0, S - 1, min(args.serial_sample, S)).round().astype(int).tolist()))
         singles = [batch.subset([k], tiles_per_block=tpw) for k in sample]
+        for one in singles:                 # the serial runs use the same launch shapes
+            for g in batch.GRID_CANDIDATES:
+                setattr(one.cfg, g, getattr(batch.cfg, g))
         t_up = time.perf_counter()
         eng = batch.prepare()
         torch.cuda.synchronize()
@@ -662,7 +667,8 @@ def replicates(args):
         "data": "synthetic",
         "config": {"workload": f"replicates: {S} simulations, {batch.cfg.n_nodes} nodes, {n0} agents "
                                f"(scale {args.scale}), 365 days, batched in one context",
-                   "tiles_per_block": int(tpw)},
+                   "tiles_per_block": int(tpw),
+                   "grids": {g: int(getattr(batch.cfg, g)) for g in batch.GRID_CANDIDATES}},
         "serial_sims_per_hr": round(S / serial_total * 3600, 1),
         "speedup_vs_serial": round(serial_total / el_b, 2),
         "batched_agent_days_per_s": round(ad_b / el_b, 1),

@@ -88,7 +88,10 @@ typedef struct {
   int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
   int32_t tiles_per_block; /* tiles of 1024 slots per span: 1, 2, 4, 8 or 16 (0 = default 16) */
   int32_t mix_mode;        /* must be 0: per-node fixed-order tree reduction of the network mix */
-  int32_t reserved[7];
+  int32_t ei_grid;         /* workgroups of the E / I list pass, 64..65535 (0 = 2048) */
+  int32_t sample_grid;     /* workgroups of the round-1 sampler, 64..65535 (0 = 2048) */
+  int32_t commit_grid;     /* workgroups of the round-1 commit, 64..65535 (0 = 512) */
+  int32_t reserved[4];
 } lp_config;
 
 /* Per-simulation parameters; array bound as "sim_params" (LP_U8, n_sims * sizeof). */

@@ -265,28 +265,47 @@ class BatchedSims:
                 for name in AGENT_COLUMNS}
         return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
 
-    def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True):
-        """Pick tiles_per_block by timing `days` simulated days of the first `max_sims`
-        simulations with each candidate (fresh uploads, so nothing of this batch runs)."""
+    GRID_CANDIDATES = {"ei_grid": (1024, 2048, 4096), "sample_grid": (1024, 2048, 4096), "commit_grid": (256, 512, 1024)}
+
+    def _time_subset(self, ks, days, use_graph, tpw, grids):
         import torch
 
+        sub = self.subset(ks, tiles_per_block=tpw)
+        for name, g in grids.items():
+            setattr(sub.cfg, name, int(g))
+        eng = sub.prepare()
+        eng.run_days(0, 2, use_graph=use_graph)
+        eng.sync()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.run_days(2, min(self.cfg.nt, 2 + days), use_graph=use_graph)
+        eng.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        sub.close()
+        return el
+
+    def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True, grids=False):
+        """Pick tiles_per_block by timing `days` simulated days of the first `max_sims`
+        simulations with each candidate (fresh contexts, so nothing of this batch runs);
+        with `grids`, then the workgroup counts of the list pass, the round-1 sampler and
+        its commit, one after the other from GRID_CANDIDATES (the launch shapes of
+        lp_config).  Returns (tiles_per_block, times by tiles_per_block), and records the
+        chosen grids in self.cfg and the trial times in self.tune_log."""
         ks = range(min(max_sims, len(self.packs)))
-        times = {}
-        for tpw in candidates:
-            sub = self.subset(ks, tiles_per_block=tpw)
-            eng = sub.prepare()
-            eng.run_days(0, 2, use_graph=use_graph)
-            eng.sync()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            eng.run_days(2, min(self.cfg.nt, 2 + days), use_graph=use_graph)
-            eng.sync()
-            torch.cuda.synchronize()
-            times[tpw] = time.perf_counter() - t
-            sub.close()
+        times = {tpw: self._time_subset(ks, days, use_graph, tpw, {}) for tpw in candidates}
         best = min(times, key=times.get)
         self.tiles_per_block = best
         self.cfg.tiles_per_block = best
+        self.tune_log = {"tiles_per_block": dict(times)}
+        if grids:
+            chosen = {}
+            for name, cands in self.GRID_CANDIDATES.items():
+                tried = {g: self._time_subset(ks, days, use_graph, best, {**chosen, name: g}) for g in cands}
+                chosen[name] = min(tried, key=tried.get)
+                self.tune_log[name] = tried
+            for name, g in chosen.items():
+                setattr(self.cfg, name, int(g))
         return best, times
 
     def close(self):

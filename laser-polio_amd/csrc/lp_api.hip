@@ -101,6 +101,7 @@ struct lp_ctx {
   int32_t n_spans = 0;
   int32_t tpw = 4;
   int32_t agent_grid = 1;
+  int32_t ei_grid = 2048, sample_grid = 2048, commit_grid = 512;   // launch shapes of the list and sampling kernels
   std::vector<void*> allocs;
   hipGraphExec_t graph[lp::N_KINDS] = {};          // one per day kind: plain, FULL_* combinations, plain from the list
   bool graph_ready[lp::N_KINDS] = {};
@@ -166,8 +167,11 @@ int alloc(lp_ctx* c, T** p, size_t n) {
 
 int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
-// Workgroups of k_ei (4 wavefronts each); each wavefront loops over list pieces.
+// Default workgroups of k_ei (4 wavefronts each; each wavefront loops over list pieces),
+// k_sample and k_commit; lp_config may override them (the batch autotuner does).
 constexpr int kEiGrid = 2048;
+constexpr int kSampleGrid = 2048;
+constexpr int kCommitGrid = 512;
 // Workgroups per node of k_ri_list (each takes every kRiSplit-th batch of 1024 candidates).
 constexpr int kRiSplit = 8;
 
@@ -194,7 +198,7 @@ int day_kind(const lp_ctx* c, int32_t t, int mask, int recount) {
 // k_ei of one day; above EI_LDS_NODES nodes its piece plan comes from k_ei_plan.
 void launch_ei(lp_ctx* c, hipStream_t s, int mask) {
   if (c->cfg.n_nodes > lp::EI_LDS_NODES) hipLaunchKernelGGL(lp::k_ei_plan, dim3(1), dim3(lp::WG), 0, s, c->d);
-  hipLaunchKernelGGL(lp::k_ei, dim3(kEiGrid), dim3(lp::WG), 0, s, c->d, mask);
+  hipLaunchKernelGGL(lp::k_ei, dim3(c->ei_grid), dim3(lp::WG), 0, s, c->d, mask);
 }
 
 // Agent pass of one day: k_plain over U = tiles per span on plain days,
@@ -312,8 +316,8 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
     // with the probe offsets in LDS k_sample plans round 1 itself
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
     if (!off_lds) hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
-    hipLaunchKernelGGL(lp::k_sample, dim3(2048), dim3(lp::WG), off_lds, s, d);
-    hipLaunchKernelGGL(lp::k_commit, dim3(512), dim3(lp::WG), 0, s, d);
+    hipLaunchKernelGGL(lp::k_sample, dim3(c->sample_grid), dim3(lp::WG), off_lds, s, d);
+    hipLaunchKernelGGL(lp::k_commit, dim3(c->commit_grid), dim3(lp::WG), 0, s, d);
     hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);
   }
   hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust, advance ? 1 : 0);
@@ -343,8 +347,13 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
       cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->nt > (1 << 23) || cfg->n_sia_events < 0 ||
       cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode != 0)
     return LP_ERR_ARG;
+  for (int32_t g : {cfg->ei_grid, cfg->sample_grid, cfg->commit_grid})
+    if (g != 0 && (g < 64 || g > 65535)) return LP_ERR_ARG;
   lp_ctx* c = new lp_ctx();
   c->cfg = *cfg;
+  c->ei_grid = cfg->ei_grid ? cfg->ei_grid : kEiGrid;
+  c->sample_grid = cfg->sample_grid ? cfg->sample_grid : kSampleGrid;
+  c->commit_grid = cfg->commit_grid ? cfg->commit_grid : kCommitGrid;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return LP_ERR_HIP; }
   c->own_stream = true;
   *out = c;

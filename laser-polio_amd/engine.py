@@ -57,7 +57,10 @@ class LPConfig(ctypes.Structure):
         ("n_sia_events", ctypes.c_int32),
         ("tiles_per_block", ctypes.c_int32),
         ("mix_mode", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("ei_grid", ctypes.c_int32),
+        ("sample_grid", ctypes.c_int32),
+        ("commit_grid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

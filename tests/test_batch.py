@@ -118,3 +118,24 @@ def test_device_initialised_replicates_full_size():
     par = bench.replicate_parity(batch, ks, init_state, print)
     assert par["identical"], par
     batch.close()
+
+
+@pytest.mark.gpu
+def test_launch_shapes_do_not_change_results():
+    """The workgroup counts of the list pass, the sampler and its commit (lp_config) leave
+    every draw where it was: a batch with other grids still equals the oracle, and the
+    autotuner picks each from its candidates."""
+    print("[batch-grids] autotune and non-default grids", flush=True)
+    from test_gpu_parity import _compare
+
+    sims = _replicates(4)
+    refs = [OracleSim(s.export_tables(), pop0=s.pars.init_pop).run() for s in sims]
+    b = lp.BatchedSims(sims)
+    best, _ = b.autotune(days=5, max_sims=4, grids=True)
+    for g, cands in b.GRID_CANDIDATES.items():
+        assert getattr(b.cfg, g) in cands
+    b.cfg.ei_grid, b.cfg.sample_grid, b.cfg.commit_grid = 1024, 4096, 256
+    b.run(use_graph=True)
+    for s, ref in zip(sims, refs):
+        _compare(s, ref)
+    b.close()
