@@ -109,6 +109,7 @@ struct lp_ctx {
   int32_t list_tick = -1;                          // the tick whose E / I list is complete (-1: none)
   bool rec_dirty = false;                          // agent records newer than the bound columns
   bool rec_actual = true;                          // records hold the actual timers of E / I agents
+  bool ran = false;                                // a day or stage has been queued
   std::vector<int32_t> day_flags_h;
   std::vector<int32_t> ri_cnt_h;
   std::string err;
@@ -262,6 +263,15 @@ void enqueue_sync_columns(lp_ctx* c) {
   hipLaunchKernelGGL(lp::k_unpack, dim3(rec_grid(c)), dim3(lp::WG), 0, c->stream, c->d);
   hipLaunchKernelGGL(lp::k_ri_sync, dim3(c->n_spans), dim3(lp::WG), 0, c->stream, c->d);
   hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, c->stream, c->d, -2);
+}
+
+// Before the first day a context runs: bound ri_timer columns hold actual values as of
+// the end of the tick before it (a run that starts at tick t > 0, e.g. from a checkpoint,
+// has had every RI step through t - 1 applied to them).
+void first_run(lp_ctx* c, int32_t t) {
+  if (c->ran) return;
+  c->ran = true;
+  if (t > 0) hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, c->stream, c->d, t - 1);
 }
 
 // The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
@@ -544,6 +554,7 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
   if (!c->prepared) return fail(c, LP_ERR_STATE, "lp_run_days before lp_prepare");
   if (t_begin < 0 || t_end > c->cfg.nt || t_begin > t_end) return fail(c, LP_ERR_ARG, "bad tick range");
   if (t_begin == t_end) return LP_OK;
+  first_run(c, t_begin);
   // inside the call each day's k_day_end begins the next day; only the first is begun here
   hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t_begin, t_end);
   bool first = true;
@@ -599,6 +610,7 @@ int lp_step_stage(lp_ctx* c, int32_t t, int32_t mask) {
   if (!c->prepared) return fail(c, LP_ERR_STATE, "step before lp_prepare");
   if (t < 0 || t >= c->cfg.nt) return fail(c, LP_ERR_ARG, "bad tick");
   if (mask & ~LP_STAGE_ALL) return fail(c, LP_ERR_ARG, "bad stage mask");
+  first_run(c, t);
   hipLaunchKernelGGL(lp::k_set_day, dim3(1), dim3(64), 0, c->stream, c->d, t, 0);
   const bool fused_tx_log = (mask & LP_STAGE_TRANSMISSION) && (mask & LP_STAGE_LOG);
   const bool log_only = (mask & LP_STAGE_LOG) && !(mask & LP_STAGE_TRANSMISSION);

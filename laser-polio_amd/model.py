@@ -361,6 +361,65 @@ class SEIR_ABM:
                     self._engine.pull(self._binding_of(frame, name), frame.raw(name))
                     self._dev_newer.discard(key)
 
+    # -- exact checkpoints --------------------------------------------------------------
+    def save_checkpoint(self, path):
+        """Everything a run continues from, to `path` (.npz): every agent column over the
+        whole capacity (the values drawn for slots not yet handed out included), the node
+        segments, every results array, the next tick, the host generator's state (seed
+        schedule draws) and the parameters.  Draws are keyed by (seed, slot or node, day),
+        so a model built with the same parameters and components that loads it
+        (load_checkpoint) and runs on ends with the same state as a run never stopped."""
+        import json
+
+        from .laserframe import _jsonable
+
+        if self._engine is not None:
+            self._engine.sync()
+            self._pull_all()
+        p, r = self.people, self.results
+        arrays = {"seg_begin": p.seg_begin, "seg_end": p.seg_end, "live_end": p.live_end}
+        for name in p.column_names():
+            arrays["col_" + name] = p.raw(name)
+        for name in r.column_names():
+            arrays["res_" + name] = r.raw(name)
+        meta = {"format": "laser_polio_amd.checkpoint/1", "t": int(self.t), "nt": int(self.nt),
+                "capacity": int(p.capacity), "n_nodes": int(p.n_nodes), "rng": self.rng.bit_generator.state,
+                "pars": _jsonable(self.pars)}
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        with open(path, "wb") as fh:
+            np.savez(fh, **arrays)
+
+    def load_checkpoint(self, path):
+        """Continue from a checkpoint saved by a model with the same parameters and
+        components (same capacity, nodes and duration): its agent columns, node segments,
+        results, tick and host generator state replace this model's; the next run()
+        recounts every carried statistic from the loaded columns and goes on from the saved
+        tick."""
+        import json
+
+        with np.load(path, allow_pickle=False) as z:
+            meta = json.loads(bytes(z["meta"]).decode())
+            p, r = self.people, self.results
+            if (meta["capacity"], meta["n_nodes"], meta["nt"]) != (p.capacity, p.n_nodes, self.nt):
+                raise ValueError("checkpoint of a model with another capacity, node count or duration")
+            if not (np.array_equal(z["seg_begin"], p.seg_begin) and np.array_equal(z["seg_end"], p.seg_end)):
+                raise ValueError("checkpoint of a model with other node segments")
+            eng = self._engine
+            if eng is not None:              # nothing of the device state survives the load
+                eng.close()
+                self._engine = None
+                for frame in (self.people, self.results):
+                    object.__setattr__(frame, "_sync_hook", None)
+            for name in p.column_names():
+                p.raw(name)[...] = z["col_" + name]
+            for name in r.column_names():
+                r.raw(name)[...] = z["res_" + name]
+            p.live_end[...] = z["live_end"]
+        self.t = int(meta["t"])
+        self.rng.bit_generator.state = meta["rng"]
+        self._device_initialized = True     # the loaded columns are the population
+        self.should_stop = False
+
     def export_tables(self):
         """SimTables of the current host state (used by the GPU engine and by the oracle).
         With init_on_device the population is drawn on the GPU first and pulled."""
@@ -625,23 +684,27 @@ class SEIR_ABM:
         return all(type(c) in stock for c in self.instances)
 
     # -- running ------------------------------------------------------------------
-    def run(self, use_graph=True):
+    def run(self, use_graph=True, until=None):
+        """Run ticks self.t .. nt - 1 (with `until`, stop before tick `until`: a later run()
+        goes on from there, e.g. after save_checkpoint)."""
         if self.verbose >= 1:
             print("Initialization complete. Running simulation...")
+        end = self.nt if until is None else max(self.t, min(int(until), self.nt))
         with self.perf_stats.start("SEIR_ABM.run()"):
             if self._fused_ok():
-                self._run_fused(use_graph)
+                self._run_fused(use_graph, end)
             else:
-                self._run_per_tick()
+                self._run_per_tick(end)
         if self.verbose >= 1:
             print("Simulation complete.")
 
-    def _run_fused(self, use_graph):
+    def _run_fused(self, use_graph, end=None):
+        end = self.nt if end is None else end
         eng = self._ensure_engine()
         ds = self._instance("DiseaseState_ABM")
         # Seeds are applied inside DiseaseState_ABM.step (t >= 1), between the DS
         # progression and RI: such a day runs as VD+DS, host seeding, RI+SIA+TX+log.
-        seed_days = sorted(t for t in (ds.seed_schedule if ds else {}) if max(self.t, 1) <= t < self.nt)
+        seed_days = sorted(t for t in (ds.seed_schedule if ds else {}) if max(self.t, 1) <= t < end)
         t = self.t
         for sd in seed_days:
             if t < sd:
@@ -654,8 +717,8 @@ class SEIR_ABM:
             self._push_touched()
             eng.step(sd, engmod.STAGE_RI | engmod.STAGE_SIA | engmod.STAGE_TRANSMISSION | engmod.STAGE_LOG)
             t = sd + 1
-        if t < self.nt:
-            eng.run_days(t, self.nt, use_graph)
+        if t < end:
+            eng.run_days(t, end, use_graph)
         eng.sync()
         self._mark_device_newer()
         stop = int(eng.stop_days()[0])
@@ -665,10 +728,10 @@ class SEIR_ABM:
             if self.verbose >= 1:
                 print(f"[SEIR_ABM] Early stopping at t={self.t}: no E/I and no future seed_schedule events.")
         else:
-            self.t = self.nt
+            self.t = end
 
-    def _run_per_tick(self):
-        for tick in range(self.t, self.nt):
+    def _run_per_tick(self, end=None):
+        for tick in range(self.t, self.nt if end is None else end):
             if tick == 0:
                 self.log_results(0)
                 self.t += 1
