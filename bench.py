@@ -18,7 +18,7 @@ Also reported on the same JSON line:
                 below) over its HIP-event-timed duration against MI355X's 8 TB/s HBM
                 peak; its HBM traffic per launch from the committed rocprofv3 record;
                 whole_day: the counter bytes of every kernel of a window day over
-                the day time; survey_byte_model: the bandwidth a dense day would need
+                the day time; dense_byte_model: the bandwidth a dense day would need
   cpu_baseline  the CPU oracle (restated reference kernels, OpenMP on all host
                 threads) on the same initial population for the first --cpu-days
                 days, plus the next day on a single thread
@@ -201,8 +201,8 @@ def compare_with_oracle(sim, ref, t_last):
     return bad
 
 
-def survey_model_bytes(res, flags, t0, t1):
-    """Bytes per day of the dense day the survey's byte model prices (state + risk of every
+def dense_model_bytes(res, flags, t0, t1):
+    """Bytes per day of the dense day of the per-agent byte model (state + risk of every
     live agent, of every agent in a node with draws, 17 B per E/I agent, death dates on VD
     days, RI and SIA columns of every live agent), over ticks [t0, t1): the bandwidth a
     dense implementation would need to run the same days as fast."""
@@ -489,9 +489,9 @@ def national(args):
                 "network_bytes": net_bytes,
                 "note": "counter bytes of every kernel of the window's days (rocprofv3 record) over the day time",
             },
-            "survey_byte_model": {
-                "bytes_per_day": int(survey_model_bytes(res, day_flags, t_begin, t_end) / K),
-                "effective_GBs": round(survey_model_bytes(res, day_flags, t_begin, t_end) / elapsed / 1e9, 1),
+            "dense_byte_model": {
+                "bytes_per_day": int(dense_model_bytes(res, day_flags, t_begin, t_end) / K),
+                "effective_GBs": round(dense_model_bytes(res, day_flags, t_begin, t_end) / elapsed / 1e9, 1),
                 "note": "bandwidth a dense pass over every live agent would need to match this day time",
             },
         }
