@@ -87,7 +87,9 @@ typedef struct {
   int32_t nt;              /* ticks = dur + 1, at most 2^23 */
   int32_t n_sia_events;    /* rows of the SIA event tables (may be 0) */
   int32_t tiles_per_block; /* tiles of 1024 slots per span: 1, 2, 4, 8 or 16 (0 = default 16) */
-  int32_t mix_mode;        /* must be 0: per-node fixed-order tree reduction of the network mix */
+  int32_t mix_mode;        /* network mix: 0 = per-node fixed-order tree reduction (the oracle's order,
+                              bit-identical replays); 1 = every simulation's prevalence x network
+                              contraction on MFMA (within 1e-6 of float64, not replay-identical) */
   int32_t ei_grid;         /* workgroups of the E / I list pass, 64..65535 (0 = 2048) */
   int32_t sample_grid;     /* workgroups of the round-1 sampler, 64..65535 (0 = 2048) */
   int32_t commit_grid;     /* workgroups of the round-1 commit, 64..65535 (0 = 512) */
@@ -235,7 +237,9 @@ int lp_stop_days(lp_ctx* ctx, int32_t* out, int32_t n_sims);
  * "p" float[n_nodes*n_strains], "expected" float[n_nodes], "total" int64[n_nodes],
  * "beta_mixed" float[n_nodes*n_strains]; of the last VitalDynamics step:
  * "birth_expected" int64[n_nodes][2] = floor and Bernoulli threshold ceil(frac * 2^32)
- * of the expected births.  Returns NULL for unknown names. */
+ * of the expected births; "beta" float[n_nodes][4], the infectious weight per node and
+ * strain slot the network mix reads; with mix_mode 1, "mix" float[n_nodes][4], the
+ * contraction sum_j N_ji beta_j done on MFMA.  Returns NULL for unknown names. */
 const void* lp_debug_buffer(lp_ctx* ctx, const char* name);
 
 /* Timing of the agent-pass kernel: with timing on, lp_run_days launches the day

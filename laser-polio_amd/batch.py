@@ -136,7 +136,8 @@ class BatchedSims:
             arrays[bind] = np.ascontiguousarray(np.concatenate([p.results[bind] for p in self.packs], axis=1))
         self.arrays = arrays
         self.cfg = engmod.LPConfig(capacity=int(self.slot_off[-1]), n_nodes=N, n_sims=S, n_strains=K, nt=nt,
-                                   n_sia_events=E, tiles_per_block=self.tiles_per_block, mix_mode=0)
+                                   n_sia_events=E, tiles_per_block=self.tiles_per_block,
+                                   mix_mode=int(getattr(tbs[0], "mix_mode", 0) or 0))
 
     # -- running ----------------------------------------------------------------------
     def prepare(self):

@@ -102,6 +102,7 @@ struct lp_ctx {
   int32_t tpw = 4;
   int32_t agent_grid = 1;
   int32_t ei_grid = 2048, sample_grid = 2048, commit_grid = 512;   // launch shapes of the list and sampling kernels
+  int32_t mix_tiles = 0;                           // mix_mode 1: 16-row tiles of the largest simulation
   std::vector<void*> allocs;
   hipGraphExec_t graph[lp::N_KINDS] = {};          // one per day kind: plain, FULL_* combinations, plain from the list
   bool graph_ready[lp::N_KINDS] = {};
@@ -322,6 +323,8 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   hipLaunchKernelGGL(lp::k_node_reduce, dim3(N), dim3(lp::WG), 0, s, d, mask, recount,
                      kind < lp::KIND_PLAIN_LIST ? 0 : 1);
   if (mask & LP_STAGE_TRANSMISSION) {
+    if (c->cfg.mix_mode == 1)
+      hipLaunchKernelGGL(lp::k_mix_mfma, dim3(c->cfg.n_sims, c->mix_tiles), dim3(64), 0, s, d);
     hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
     // with the probe offsets in LDS k_sample plans round 1 itself
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
@@ -355,7 +358,7 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
   *out = nullptr;
   if (cfg->capacity <= 0 || cfg->capacity % LP_TILE != 0 || cfg->n_nodes <= 0 || cfg->n_sims <= 0 ||
       cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->nt > (1 << 23) || cfg->n_sia_events < 0 ||
-      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode != 0)
+      cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode < 0 || cfg->mix_mode > 1)
     return LP_ERR_ARG;
   for (int32_t g : {cfg->ei_grid, cfg->sample_grid, cfg->commit_grid})
     if (g != 0 && (g < 64 || g > 65535)) return LP_ERR_ARG;
@@ -504,6 +507,13 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.ei_piece_off, (size_t)N + 1))) return rc;
   if ((rc = alloc(c, &d.node_acc, (size_t)N * lp::NPART))) return rc;
   if ((rc = alloc(c, &d.beta_f, (size_t)N * LP_MAX_STRAINS))) return rc;
+  d.mix_buf = nullptr;
+  if (g.mix_mode == 1) {
+    if ((rc = alloc(c, &d.mix_buf, (size_t)N * LP_MAX_STRAINS))) return rc;
+    int max_n = 0;
+    for (const auto& p : sims) max_n = std::max(max_n, (int)p.n_nodes);
+    c->mix_tiles = (max_n + 15) / 16;
+  }
   if ((rc = alloc(c, &d.sim_cases, (size_t)S))) return rc;
   if ((rc = alloc(c, &d.ninfo, (size_t)N))) return rc;
   if ((rc = alloc(c, &d.ri_list, 2 * (size_t)g.capacity))) return rc;
@@ -737,6 +747,8 @@ const void* lp_debug_buffer(lp_ctx* c, const char* name) {
   if (!std::strcmp(name, "expected")) return c->d.expected_dbg;
   if (!std::strcmp(name, "total")) return c->d.total_dbg;
   if (!std::strcmp(name, "beta_mixed")) return c->d.beta_mixed;
+  if (!std::strcmp(name, "beta")) return c->d.beta_f;
+  if (!std::strcmp(name, "mix")) return c->d.mix_buf;
   if (!std::strcmp(name, "size")) return c->d.size;
   if (!std::strcmp(name, "birth_expected")) return c->d.birth_dbg;
   return nullptr;

@@ -105,6 +105,7 @@ struct Dev {
   int32_t* ei_piece_off;   // [nodes + 1] prefix over nodes of the day's list pieces
   int64_t* node_acc;       // [nodes][32] tallies accumulated by k_ei (zeroed each day)
   float* beta_f;           // [nodes][4] infectious weight per strain (node tally / 2^24)
+  float* mix_buf;          // [nodes][4] network mix sum_j N_ji beta_j on MFMA (mix_mode 1), else null
   int32_t* sim_cases;      // [sims] 1 when a node of the sim logged E or I today
   NodeInfo* ninfo;         // [nodes]
   const int32_t* ri_cnt;   // [nt + 1] ri_cnt[t + 1] = RI days among ticks 0..t
@@ -2662,6 +2663,10 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
     for (int w = 32; w >= 1; w >>= 1) x = x + __shfl_down(x, w, 64);
     part[k] = __shfl(x, 0, 64);
   }
+  if (d.mix_buf) {                 // mix_mode 1: the contraction came from k_mix_mfma
+#pragma unroll
+    for (int k = 0; k < LP_MAX_STRAINS; ++k) part[k] = d.mix_buf[(size_t)i * LP_MAX_STRAINS + k];
+  }
   // node-level map, then the count draw with chunks spread over lanes
   const float popc = (float)(pop_t > 1 ? pop_t : 1);
   float pk[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -2755,6 +2760,48 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
     if (k >= K) break;
     cum = cum + (ptot > 0.0f ? pk[k] / ptot : 0.0f);
     d.thr_strain[(size_t)i * LP_MAX_STRAINS + k] = thr_from_prob(cum);
+  }
+}
+
+// mix_mode 1: every simulation's network mix as a matrix product on MFMA,
+// mix[i][k] = sum_j NT[i][j] beta[j][k] with NT the simulation's network_t block (row i:
+// the weights into node i) and beta the nodes' infectious weight per strain, in float32.
+// One wavefront per (simulation, 16 rows): v_mfma_f32_16x16x4_f32 over k-steps of 4
+// sources, A = NT[i0 .. i0+15][j0 .. j0+3] (lane i + 16 k holds A[i][k]), B = beta[j0 ..
+// j0+3][0 .. 15] with strains in columns 0-3 and zeros beyond (lane j + 16 k holds
+// B[k][j]), D[i][j] in lane j + 16 (i / 4), element i % 4.  The accumulation order is the
+// matrix unit's, not the oracle's fixed order.
+__global__ void __launch_bounds__(64) k_mix_mfma(Dev d) {
+  const int sim = blockIdx.x, tile = blockIdx.y, lane = threadIdx.x;
+  const lp_sim_params& sp = d.sims[sim];
+  const int n = sp.n_nodes, i0 = tile * 16;
+  if (i0 >= n) return;
+  const float* NT = d.network_t + sp.net_off;
+  const float* B = d.beta_f + (size_t)sp.node_begin * LP_MAX_STRAINS;
+  const int ai = i0 + (lane & 15), kk = lane >> 4, bj = lane & 15;
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  // eight accumulators over interleaved k-steps, summed as a tree at the end: the
+  // sequential depth of each sum is n / 32, not n / 4 (float32 error ~ sqrt of it)
+  constexpr int NACC = 8;
+  v4f acc[NACC];
+#pragma unroll
+  for (int q = 0; q < NACC; ++q) acc[q] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int j0 = 0; j0 < n; j0 += 4 * NACC) {
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) {
+      const int j = j0 + 4 * q + kk;
+      const float a = (ai < n && j < n) ? NT[(size_t)ai * n + j] : 0.0f;
+      const float b = (bj < LP_MAX_STRAINS && j < n) ? B[(size_t)j * LP_MAX_STRAINS + bj] : 0.0f;
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+    }
+  }
+  const v4f s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  if (bj < LP_MAX_STRAINS) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i0 + 4 * kk + r;
+      if (row < n) d.mix_buf[(size_t)(sp.node_begin + row) * LP_MAX_STRAINS + bj] = s[r];
+    }
   }
 }
 

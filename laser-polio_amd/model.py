@@ -491,6 +491,7 @@ class SEIR_ABM:
             ri_step=int(pars.step_size_RI_ABM),
             newborn_ri_timer=int(pars.ri_newborn_timer) if ri is not None else -1,
             last_seed_day=int(max(seed_days)) if seed_days else -1,
+            mix_mode=int(pars.mix_mode or 0),
             **sched,
         )
 
@@ -625,7 +626,7 @@ class SEIR_ABM:
             nt=tb.nt,
             n_sia_events=tb.n_sia_events,
             tiles_per_block=int(self.pars.tiles_per_block or 0),
-            mix_mode=0,
+            mix_mode=int(self.pars.mix_mode or 0),
         )
         device_only = set(AGENT_COLUMNS) if (self.device_init and not self._device_initialized) else set()
         self._engine = engmod.Engine(cfg, arrays, timing=self._timing, device_only=device_only)

@@ -13,6 +13,9 @@ synthetic workloads (all default to the reference behaviour):
   init_on_device          draw the per-agent columns on the GPU (lp_init_population) instead of on
                           the host: same distributions, different numbers; initial immunity and
                           chronically-missed agents become per-agent Bernoulli draws
+  mix_mode                0: network mix in the oracle's fixed order (bit-identical replays);
+                          1: the prevalence x network contraction on MFMA (float32 within 1e-6
+                          of float64, not replay-identical)
 """
 
 import datetime
@@ -98,6 +101,7 @@ default_pars = PropertySet(
         "ri_newborn_timer": 182,
         "tiles_per_block": 0,
         "init_on_device": False,
+        "mix_mode": 0,
     }
 )
 

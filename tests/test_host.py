@@ -76,7 +76,7 @@ def test_create_rejects_out_of_range_configs():
     lib = _lib()
     good = dict(capacity=1 << 20, n_nodes=4, n_sims=1, n_strains=1, nt=366, n_sia_events=0, mix_mode=0)
     bad = [dict(nt=(1 << 23) + 1), dict(nt=0), dict(capacity=1 << 32), dict(capacity=1000), dict(n_nodes=32768),
-           dict(n_strains=5), dict(n_strains=0), dict(n_sims=0), dict(mix_mode=1), dict(ei_grid=32),
+           dict(n_strains=5), dict(n_strains=0), dict(n_sims=0), dict(mix_mode=2), dict(mix_mode=-1), dict(ei_grid=32),
            dict(sample_grid=70000), dict(commit_grid=-1)]
     for over in bad:
         cfg = em.LPConfig(**{**good, **over})
