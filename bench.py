@@ -124,6 +124,8 @@ def main():
                          "simulations of uneven sizes batched in one context, against serial runs of a sample; "
                          "adjoint: gradients of a mean-field calibration loss on the national geography")
     ap.add_argument("--sims", type=int, default=512, help="replicates: simulations in the batch")
+    ap.add_argument("--mix-mode", type=int, default=0, choices=[0, 1],
+                    help="replicates: 1 = the network mix of every simulation on MFMA (k_mix_mfma)")
     ap.add_argument("--serial-sample", type=int, default=16, help="replicates: simulations also run one at a time")
     ap.add_argument("--workers", type=int, default=16, help="replicates: host processes initialising simulations")
     ap.add_argument("--check-sims", type=int, default=8,
@@ -578,6 +580,8 @@ def replicates(args):
 
     S, dur = args.sims, 365
     pars_list = [configs.replicate_pars(i, scale=args.scale, dur=dur) for i in range(S)]
+    for p in pars_list:
+        p.mix_mode = args.mix_mode
     t_init = time.perf_counter()
     if args.init == "device":
         import torch
@@ -613,6 +617,8 @@ def replicates(args):
         log(f"[bench] upload / device population {time.perf_counter() - t_up:.1f}s")
         # the initial state of a sample of simulations, for their oracle replay afterwards
         check = sorted(set(np.linspace(0, S - 1, min(args.check_sims, S)).round().astype(int).tolist())) if args.check_sims else []
+        if args.mix_mode == 1:
+            check = []          # the MFMA mix is not a replay of the oracle's order
         init_state = {k: batch.pull_sim(k, names=None)[0] for k in check}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -667,7 +673,7 @@ def replicates(args):
         "data": "synthetic",
         "config": {"workload": f"replicates: {S} simulations, {batch.cfg.n_nodes} nodes, {n0} agents "
                                f"(scale {args.scale}), 365 days, batched in one context",
-                   "tiles_per_block": int(tpw),
+                   "tiles_per_block": int(tpw), "mix_mode": int(args.mix_mode),
                    "grids": {g: int(getattr(batch.cfg, g)) for g in batch.GRID_CANDIDATES}},
         "serial_sims_per_hr": round(S / serial_total * 3600, 1),
         "speedup_vs_serial": round(serial_total / el_b, 2),
