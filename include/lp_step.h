@@ -145,7 +145,8 @@ int lp_owns_stream(lp_ctx* ctx);
 /* Fused days: runs ticks [t_begin, t_end) with every stage the schedule enables,
  * in reference order (VitalDynamics, DiseaseState, RI, SIA, Transmission, log).
  * Tick 0 only logs the initial state.  Days after a sim's early stop are no-ops.
- * use_graph != 0 replays one captured hipGraph per day. */
+ * use_graph != 0 replays captured hipGraphs: one per day kind, and one per run of 2, 4 or
+ * 8 consecutive days of one kind. */
 int lp_run_days(lp_ctx* ctx, int32_t t_begin, int32_t t_end, int32_t use_graph);
 
 /* Single stages on tick t (any OR of LP_STAGE_*), for callers that step

@@ -85,6 +85,8 @@ const BindSpec kBinds[] = {
 };
 #undef F
 constexpr int kNumBinds = sizeof(kBinds) / sizeof(kBinds[0]);
+// Captured runs of 1, 2, 4 and 8 consecutive days of one kind.
+constexpr int kRunLevels = 4;
 
 std::string g_names;
 
@@ -104,8 +106,10 @@ struct lp_ctx {
   int32_t ei_grid = 2048, sample_grid = 2048, commit_grid = 512;   // launch shapes of the list and sampling kernels
   int32_t mix_tiles = 0;                           // mix_mode 1: 16-row tiles of the largest simulation
   std::vector<void*> allocs;
-  hipGraphExec_t graph[lp::N_KINDS] = {};          // one per day kind: plain, FULL_* combinations, plain from the list
-  bool graph_ready[lp::N_KINDS] = {};
+  // per day kind (plain, FULL_* combinations, their list forms): graph[kind][j] runs 2^j
+  // consecutive days of that kind, so a run of such days pays one graph launch per 8
+  hipGraphExec_t graph[lp::N_KINDS][kRunLevels] = {};
+  bool graph_ready[lp::N_KINDS][kRunLevels] = {};
   bool recount_pending = true;                     // next agent pass recounts carried statistics
   int32_t list_tick = -1;                          // the tick whose E / I list is complete (-1: none)
   bool rec_dirty = false;                          // agent records newer than the bound columns
@@ -275,6 +279,19 @@ void first_run(lp_ctx* c, int32_t t) {
   if (t > 0) hipLaunchKernelGGL(lp::k_set_ri_ref, dim3(1), dim3(64), 0, c->stream, c->d, t - 1);
 }
 
+// Whether every day in [a, b) runs as day kind `kind`, a list kind, inside lp_run_days
+// (no recount pending; each day's list is completed by the day before it).
+bool same_list_kind(const lp_ctx* c, int32_t a, int32_t b, int kind) {
+  for (int32_t u = a; u < b; ++u) {
+    if (!(host_stage_flags(c->day_flags_h[u], LP_STAGE_ALL) & LP_DAY_DS)) return false;
+    int k = day_kind(c, u, LP_STAGE_ALL, 0);
+    if (k == lp::KIND_PLAIN) k = lp::KIND_PLAIN_LIST;
+    else if (k >= lp::KIND_FULL && k < lp::KIND_RECOUNT) k = lp::KIND_FULL_LIST + k - lp::KIND_FULL;
+    if (k != kind) return false;
+  }
+  return true;
+}
+
 // The kernel sequence of one tick.  kind: agent-pass variant (KIND_RECOUNT
 // recounts every carried statistic and the paralysis counts); mask: stages;
 // tx_adjust: apply today's transmission exposures to the log; advance: move
@@ -339,6 +356,21 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   return LP_OK;
 }
 
+// Captures `run` consecutive days of kind `kind` from day t into *g.  Nothing runs:
+// the graph is launched later.
+int capture_days(lp_ctx* c, hipGraphExec_t* g, int32_t t, int kind, int run) {
+  hipGraph_t graph;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = LP_OK;
+  for (int r = 0; r < run && rc == LP_OK; ++r) rc = enqueue_day(c, t + r, kind, LP_STAGE_ALL, 1, true, false);
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  HIPCHK(c, hipGraphInstantiate(g, graph, nullptr, nullptr, 0));
+  HIPCHK(c, hipGraphDestroy(graph));
+  return LP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -395,7 +427,8 @@ int lp_bind(lp_ctx* c, const char* name, void* ptr, int32_t dtype, int64_t n) {
 int lp_set_stream(lp_ctx* c, void* s) {
   if (!c) return LP_ERR_ARG;
   for (int k = 0; k < lp::N_KINDS; ++k)
-    if (c->graph_ready[k]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
+    for (int j = 0; j < kRunLevels; ++j)
+      if (c->graph_ready[k][j]) return fail(c, LP_ERR_STATE, "stream change after graph capture");
   const hipStream_t ns = reinterpret_cast<hipStream_t>(s);
   if (ns == c->stream) return LP_OK;
   // work queued on the current stream completes before anything is queued on the new one
@@ -494,6 +527,8 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.chunk_sum, nch))) return rc;
   if ((rc = alloc(c, &d.fin_prefix, c->n_spans))) return rc;
   if ((rc = alloc(c, &d.span_prefix, c->n_spans))) return rc;
+  if ((rc = alloc(c, &d.span_lane, (size_t)N * 64))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.span_lane, 0, (size_t)N * 64 * sizeof(lp::SpanLane), c->stream));
   if ((rc = alloc(c, &d.nb_begin, N))) return rc;
   if ((rc = alloc(c, &d.births, N))) return rc;
   if ((rc = alloc(c, &d.node_tally, (size_t)N * lp::NPART))) return rc;
@@ -531,6 +566,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.take_all, N))) return rc;
   if ((rc = alloc(c, &d.n_uniq, N))) return rc;
   if ((rc = alloc(c, &d.m_round, N))) return rc;
+  if ((rc = alloc(c, &d.resv, N))) return rc;
   if ((rc = alloc(c, &d.W_round, N))) return rc;
   if ((rc = alloc(c, &d.thr_strain, (size_t)N * LP_MAX_STRAINS))) return rc;
   if ((rc = alloc(c, &d.newexp, (size_t)N * LP_MAX_STRAINS))) return rc;
@@ -540,6 +576,7 @@ int lp_prepare(lp_ctx* c) {
   if ((rc = alloc(c, &d.ticket, 4))) return rc;
   d.sel_cap = g.capacity / 4 + 65536;
   if ((rc = alloc(c, &d.sel_list, d.sel_cap))) return rc;
+  if ((rc = alloc(c, &d.sel_word, d.sel_cap))) return rc;
   c->day_flags_h.resize(g.nt);
   HIPCHK(c, hipMemcpy(c->day_flags_h.data(), c->d.day_flags, g.nt * 4, hipMemcpyDeviceToHost));
   {
@@ -595,18 +632,20 @@ int lp_run_days(lp_ctx* c, int32_t t_begin, int32_t t_end, int32_t use_graph) {
     else enqueue_actualize(c);
     c->list_tick = t + 1;   // every day kind run here completes the next tick's list
     if (use_graph && !c->timing && !first && c->stream != nullptr) {
-      if (!c->graph_ready[kind]) {
-        hipGraph_t graph;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true, false);
-        hipError_t e = hipStreamEndCapture(c->stream, &graph);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(c, LP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        HIPCHK(c, hipGraphInstantiate(&c->graph[kind], graph, nullptr, nullptr, 0));
-        HIPCHK(c, hipGraphDestroy(graph));
-        c->graph_ready[kind] = true;
+      // the following days of the same list kind (each completes the next one's list,
+      // so they need nothing from the host in between) join this day's graph: 2^j days
+      int j = 0;
+      if (list_kind(kind))
+        while (j + 1 < kRunLevels && t + (2 << j) <= t_end && same_list_kind(c, t + (1 << j), t + (2 << j), kind)) ++j;
+      const int run = 1 << j;
+      if (!c->graph_ready[kind][j]) {
+        if ((rc = capture_days(c, &c->graph[kind][j], t, kind, run))) return rc;
+        c->graph_ready[kind][j] = true;
       }
-      HIPCHK(c, hipGraphLaunch(c->graph[kind], c->stream));
+      HIPCHK(c, hipGraphLaunch(c->graph[kind][j], c->stream));
+      c->rec_dirty = true;   // as enqueue_day marks it: the columns are behind the records
+      t += run - 1;
+      c->list_tick = t + 1;
     } else {
       if ((rc = enqueue_day(c, t, kind, LP_STAGE_ALL, 1, true, first))) return rc;
     }
@@ -790,7 +829,9 @@ const char* lp_last_error(lp_ctx* c) { return c ? c->err.c_str() : "null context
 void lp_destroy(lp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < lp::N_KINDS; ++k) if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+  for (int k = 0; k < lp::N_KINDS; ++k)
+    for (int j = 0; j < kRunLevels; ++j)
+      if (c->graph[k][j]) (void)hipGraphExecDestroy(c->graph[k][j]);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -48,6 +48,7 @@ constexpr int MAX_TPW = 16;         // tiles per span at most
 constexpr int NPART = 32;           // fields of a tally partial
 constexpr int8_t SEL_MARK = 0x7F;   // state byte of an agent selected for exposure today
 constexpr uint64_t SEL_INVALID = ~0ull;
+constexpr uint64_t SEL_SLOT = (1ull << 40) - 1;   // slot field of a selected-list entry; all ones: no agent
 constexpr int MAX_ROUNDS = 4096;    // bound on the finisher's rounds (each round claims >= 1 agent)
 
 // Tally fields.
@@ -67,6 +68,19 @@ struct NodeInfo {
   float r0k[LP_MAX_STRAINS];
   int32_t sim, lnode;          // sim index; node index inside the sim
   int32_t span_begin, n_spans;
+  // the simulation's fields the network mix and count draw read (one load level, not two)
+  int64_t net_off;
+  uint64_t thr_zi;
+  float zi;
+  int32_t disp_r, sim_nodes, sim_node_begin;
+};
+
+// Per node, lane l < 64: span l of the node's (at most 64) spans, its exclusive prefix of
+// the carried selection weights and its tiles, written by k_node_reduce so that the
+// sampler loads them with the node's other data (prefix INT64_MAX past the last span).
+struct SpanLane {
+  int64_t prefix;
+  int32_t tile0, ntiles;
 };
 
 struct Dev {
@@ -117,7 +131,10 @@ struct Dev {
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
   int64_t* probe_off; int64_t* round_base; uint64_t* sel_list; int64_t* sel_count; int64_t sel_cap;
+  uint32_t* sel_word;      // [sel_cap] round 1: the hit agent's state word as the sampler read it
   int64_t* fin_prefix; int64_t* span_prefix;
+  SpanLane* span_lane;     // [nodes][64]
+  int32_t* resv;           // [nodes] first list position of today's round-1 exposures (-1: appended)
   uint32_t* ticket;   // [4] last-block-done counters
   int64_t capacity; int32_t n_nodes, n_sims, K, nt, n_spans, tpw;
 };
@@ -185,34 +202,54 @@ __device__ int32_t km_age_at_death(const int64_t* cd, uint64_t seed, uint64_t sl
 }
 
 // ---------------------------------------------------------------------------
-// Per node: births, per-day resets (model.py:1711-1719).
-__device__ void day_begin_node(const Dev& d, int n, int32_t t, int mask) {
+// Per node: births, per-day resets (model.py:1711-1719), in two parts so that a
+// caller can issue the loads (day_begin_load) ahead of unrelated stores of its own:
+// the compiler cannot move them past stores through other pointers of Dev.
+struct DayBeginIn {
+  int32_t flags, stopped, pop_prev, lnode;
+  uint64_t rate, seed;
+  int64_t live_end, seg_end;
+};
+__device__ __forceinline__ DayBeginIn day_begin_load(const Dev& d, int n, int32_t t, int mask) {
+  DayBeginIn in;
   const int sim = d.node_sim[n];
   const lp_sim_params& sp = d.sims[sim];
-  const int32_t flags = stage_day_flags(d.day_flags[t], mask);
+  in.flags = stage_day_flags(d.day_flags[t], mask);
+  in.stopped = sim_stopped(d, sim, t) ? 1 : 0;
+  in.pop_prev = t >= 1 ? d.r_pop[(size_t)(t - 1) * d.n_nodes + n] : 0;
+  in.lnode = n - sp.node_begin;
+  in.rate = d.birth_rate_step[n];
+  in.seed = sp.seed;
+  in.live_end = d.live_end[n];
+  in.seg_end = d.seg_end[n];
+  return in;
+}
+__device__ __forceinline__ void day_begin_apply(const Dev& d, int n, int32_t t, const DayBeginIn& in) {
   for (int k = 0; k < LP_MAX_STRAINS; ++k) d.newexp[n * LP_MAX_STRAINS + k] = 0;
   d.n_uniq[n] = 0; d.size[n] = 0; d.take_all[n] = 0; d.m_round[n] = 0;
   d.births[n] = 0;
-  d.nb_begin[n] = d.live_end[n];
+  d.nb_begin[n] = in.live_end;
   d.ei_cnt[(size_t)((t + 1) & 1) * d.n_nodes + n] = 0;
-  if (sim_stopped(d, sim, t)) return;
-  if ((flags & LP_DAY_VD_STEP) && t >= 1) {
+  if (in.stopped) return;
+  if ((in.flags & LP_DAY_VD_STEP) && t >= 1) {
     // expected = step * rate * pop[t-1] in float64 as the reference computes it
     // (model.py:1712-1716), reproduced exactly in integer arithmetic
-    const int32_t pop_prev = d.r_pop[(size_t)(t - 1) * d.n_nodes + n];
     uint64_t thr = 0;
-    int64_t b = pop_prev > 0 ? f64_mul_floor(d.birth_rate_step[n], (uint32_t)pop_prev, &thr) : 0;
+    int64_t b = in.pop_prev > 0 ? f64_mul_floor(in.rate, (uint32_t)in.pop_prev, &thr) : 0;
     d.birth_dbg[2 * n] = b;
     d.birth_dbg[2 * n + 1] = (int64_t)thr;
     if (thr) {
-      const uint4 o = philox(0u, (uint32_t)(n - sp.node_begin), (uint32_t)t, ST_BIRTH, sp.seed);
+      const uint4 o = philox(0u, (uint32_t)in.lnode, (uint32_t)t, ST_BIRTH, in.seed);
       b += (uint64_t)o.x < thr ? 1 : 0;
     }
-    const int64_t room = d.seg_end[n] - d.live_end[n];
+    const int64_t room = in.seg_end - in.live_end;
     if (b > room) { b = room; atomicOr(d.err, ERR_CAPACITY); }
     d.births[n] = (int32_t)b;
-    d.live_end[n] += b;
+    d.live_end[n] = in.live_end + b;
   }
+}
+__device__ void day_begin_node(const Dev& d, int n, int32_t t, int mask) {
+  day_begin_apply(d, n, t, day_begin_load(d, n, t, mask));
 }
 
 // sim_cases[sim]: 1 when a node of the sim logged E or I on the previous day (set by
@@ -719,6 +756,12 @@ __global__ void k_node_info(Dev d) {
   ni.lnode = n - sp.node_begin;
   ni.span_begin = d.node_span_begin[n];
   ni.n_spans = d.node_span_begin[n + 1] - ni.span_begin;
+  ni.net_off = sp.net_off;
+  ni.thr_zi = sp.thr_zi;
+  ni.zi = sp.zi;
+  ni.disp_r = sp.disp_r;
+  ni.sim_nodes = sp.n_nodes;
+  ni.sim_node_begin = sp.node_begin;
   d.ninfo[n] = ni;
 }
 
@@ -1120,7 +1163,8 @@ __global__ void __launch_bounds__(WG) k_ei_actual(Dev d, int b, int32_t t_next) 
   const uint4* L = ei_list_of(d, b, n);
   for (int32_t j = threadIdx.x; j < cnt; j += WG) {
     const uint4 e = L[j];
-    if ((int64_t)e.x < d.capacity) ei_actual_timers(d, e, t_next);
+    const int s = ent_state(e);
+    if ((int64_t)e.x < d.capacity && (s == 1 || s == 2)) ei_actual_timers(d, e, t_next);   // not a hole
   }
 }
 
@@ -2453,6 +2497,11 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
   // up to 64 spans: wave 0 holds the carried span weights for the span prefix (lane = span)
   const bool lane_spans = s1 - s0 <= 64;
   const int64_t psw = (lane_spans && tid < 64 && s0 + tid < s1) ? d.partials[(size_t)(s0 + tid) * NPART + P_SW] : 0;
+  const int4 spn = (lane_spans && tid < 64 && s0 + tid < s1) ? d.spans[s0 + tid] : make_int4(0, 0, 0, 0);
+  // lane tid's SpanLane record from its span's exclusive prefix
+  auto span_lane = [&](int64_t prefix) {
+    d.span_lane[(size_t)n * 64 + tid] = s0 + tid < s1 ? SpanLane{prefix, spn.y, spn.z} : SpanLane{INT64_MAX, 0, 0};
+  };
   if (sim_stopped(d, sim, t)) return;
   const lp_sim_params& sp = d.sims[sim];
   const int32_t flags = stage_day_flags(dy.y, mask);
@@ -2475,6 +2524,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
       if (tid < 64) {
         const int64_t incl = wave_incl_scan_i64(psw, tid);
         if (s0 + tid < s1) d.span_prefix[s0 + tid] = incl - psw;
+        span_lane(incl - psw);
         if (tid == 63) s_w = incl;
       }
     } else {
@@ -2529,6 +2579,7 @@ __global__ void __launch_bounds__(WG) k_node_reduce(Dev d, int mask, int recount
       if (tid < 64) {
         const int64_t incl = wave_incl_scan_i64(psw, tid);
         if (s0 + tid < s1) d.span_prefix[s0 + tid] = incl - psw;
+        span_lane(incl - psw);
         const int64_t W = __shfl(incl, 63, 64);
         if (tid == 0) d.W_round[n] = W;
       }
@@ -2603,13 +2654,19 @@ __device__ void plan_scan(const Dev& d, int round) {
 // order of a 256-lane strided sum followed by a halving tree over the 256
 // partials: lane l holds the partials of lanes l, l + 64, l + 128, l + 192, folds
 // them in-lane for the first two tree levels and across lanes for the rest.
-__device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& sp, int32_t t, int lane) {
+// `off`: no transmission today (the node's sampling sizes are zeroed instead); a
+// stopped simulation is tested after the mix's loads are issued.
+__device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t, int lane, bool off) {
   const int K = d.K;
-  const int ns = sp.n_nodes, li = i - sp.node_begin;
-  const float* NT = d.network_t + sp.net_off + (size_t)li * ns;
-  const float4* B = reinterpret_cast<const float4*>(d.beta_f) + sp.node_begin;
+  const int sim = ni.sim;
+  const int ns = ni.sim_nodes, li = ni.lnode;
+  const float* NT = d.network_t + ni.net_off + (size_t)li * ns;
+  const float4* B = reinterpret_cast<const float4*>(d.beta_f) + ni.sim_node_begin;
   // the node's own inputs, loaded ahead of the mix
   const int64_t* tl = d.node_tally + (size_t)i * NPART;
+  int32_t* lcnt = d.ei_cnt + (size_t)((t + 1) & 1) * d.n_nodes + i;   // tomorrow's list (its passes ran)
+  const int32_t lbase = *lcnt;
+  const int64_t room = d.seg_end[i] - ni.seg_begin;
   const int32_t pop_t = d.r_pop[(size_t)t * d.n_nodes + i];
   const float season = d.season[(size_t)t * d.n_sims + sim];
   const float keep = d.keep[i], r0s = d.r0s[i];
@@ -2667,6 +2724,10 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
 #pragma unroll
     for (int k = 0; k < LP_MAX_STRAINS; ++k) part[k] = d.mix_buf[(size_t)i * LP_MAX_STRAINS + k];
   }
+  if (off || sim_stopped(d, sim, t)) {
+    if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
+    return;
+  }
   // node-level map, then the count draw with chunks spread over lanes
   const float popc = (float)(pop_t > 1 ? pop_t : 1);
   float pk[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -2692,9 +2753,9 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   int mode = 0;
   if (ex > 0.0f) {
     if (!local_zero) mode = 1;
-    else if (sp.zi < 1.0f) {
-      const uint4 o = philox(0xFFFFFFFFu, (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
-      if ((uint64_t)o.x >= sp.thr_zi) mode = 2;
+    else if (ni.zi < 1.0f) {
+      const uint4 o = philox(0xFFFFFFFFu, (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
+      if ((uint64_t)o.x >= ni.thr_zi) mode = 2;
     }
   }
   int64_t mine = 0;
@@ -2705,7 +2766,7 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
     const int64_t nch = n_full + (last > 0.0f ? 1 : 0);
     for (int64_t j = lane; j < nch; j += 64) {
       const float lc = j < n_full ? 16.0f : last;
-      const uint4 o = philox((uint32_t)j, (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
+      const uint4 o = philox((uint32_t)j, (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
       const float u = u24(o.x);
       float p = expf_lp(-lc), F = p;
       int k = 0;
@@ -2719,18 +2780,18 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
       mine += k;
     }
   } else if (mode == 2) {
-    const float dm = ex / (1.0f - sp.zi);
-    const float q = dm / (float)sp.disp_r;
+    const float dm = ex / (1.0f - ni.zi);
+    const float q = dm / (float)ni.disp_r;
     const float lq = log1pf_lp(q);
     if (lq > 0.0f) {
       const float omp = q / (1.0f + q);
       int32_t rc_max = (int32_t)(16.0f / lq);
       if (rc_max < 1) rc_max = 1;
-      const int32_t n_full = sp.disp_r / rc_max, rest = sp.disp_r % rc_max;
+      const int32_t n_full = ni.disp_r / rc_max, rest = ni.disp_r % rc_max;
       const int32_t nch = n_full + (rest > 0 ? 1 : 0);
       for (int32_t j = lane; j < nch; j += 64) {
         const int32_t rc = j < n_full ? rc_max : rest;
-        const uint4 o = philox((uint32_t)(j + 1), (uint32_t)li, (uint32_t)t, ST_COUNT, sp.seed);
+        const uint4 o = philox((uint32_t)(j + 1), (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
         const float u = u24(o.x);
         float P = expf_lp(-((float)rc * lq)), F = P;
         int64_t k = 0;
@@ -2753,7 +2814,16 @@ __device__ void mix_foi_node(const Dev& d, int i, int sim, const lp_sim_params& 
   const int32_t ta = (sz > 0 && (int64_t)sz >= spos) ? 1 : 0;
   d.size[i] = sz;
   d.take_all[i] = ta;
-  d.m_round[i] = ta ? 0 : sz;
+  const int32_t m = ta ? 0 : sz;
+  d.m_round[i] = m;
+  // positions for the round's probes in tomorrow's list (k_commit), when the segment
+  // also holds every later exposure of the node (at most sz more, appended)
+  if ((int64_t)lbase + m + sz <= room) {
+    *lcnt = lbase + m;
+    d.resv[i] = lbase;
+  } else {
+    d.resv[i] = -1;
+  }
   float cum = 0.0f;
 #pragma unroll
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
@@ -2812,14 +2882,8 @@ __global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
   const int i = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i < d.n_nodes) {
-    const int sim = d.node_sim[i];
-    const lp_sim_params& sp = d.sims[sim];
     const int32_t flags = stage_day_flags(dy.y, mask);
-    if (sim_stopped(d, sim, t) || !(flags & LP_DAY_TX)) {
-      if (lane == 0) { d.m_round[i] = 0; d.size[i] = 0; d.take_all[i] = 0; }
-    } else {
-      mix_foi_node(d, i, sim, sp, t, lane);
-    }
+    mix_foi_node(d, i, d.ninfo[i], t, lane, !(flags & LP_DAY_TX));
   }
 }
 
@@ -2848,10 +2912,14 @@ __device__ __forceinline__ int64_t ld64(const int64_t* p) {
 
 // The agent hit by target T inside the span of `ntiles` tiles from tile `tile0`,
 // whose prefix (weight before it) is sp_base: the chunk by a wave scan of the span's
-// chunk weights, the agent by a wave scan of the chunk's agent weights (MARKED_AS_S
-// as for resolve_probe).  Returns the slot, or -1 if the weights were inconsistent.
-template <bool ATOMIC_IO, bool MARKED_AS_S>
-__device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int64_t sp_base, int64_t T, int lane) {
+// chunk weights, the agent by a wave scan of the chunk's agent weights (only S
+// agents, state 0, weigh).  ATOMIC_IO: the weights and states are read with atomic
+// loads (other workgroups may change them during the kernel).  Returns the slot, or -1
+// if the weights were inconsistent; *word (when given) receives the hit agent's state
+// word as read here.
+template <bool ATOMIC_IO>
+__device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int64_t sp_base, int64_t T, int lane,
+                                    uint32_t* word = nullptr) {
   const int64_t c0 = tile0 * (LP_TILE / LP_CHUNK);
   const int nc = ntiles * (LP_TILE / LP_CHUNK);
   const int64_t cw = lane < nc ? ld64<ATOMIC_IO>(d.chunk_sum + c0 + lane) : 0;
@@ -2863,7 +2931,7 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
   const int64_t c = c0 + Lc;
   const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
   uint32_t st4;
-  if (ATOMIC_IO || MARKED_AS_S)
+  if (ATOMIC_IO)
     st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   else
     st4 = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
@@ -2872,8 +2940,7 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const uint32_t b = (st4 >> (8 * a)) & 0xffu;
-    const bool s_like = b == 0 || (MARKED_AS_S && b == (uint32_t)(uint8_t)SEL_MARK);
-    w[a] = s_like ? weight_q24(lane_of(rk, a)) : 0;
+    w[a] = b == 0 ? weight_q24(lane_of(rk, a)) : 0;
   }
   const int64_t lsum = w[0] + w[1] + w[2] + w[3];
   const int64_t incl = wave_incl_scan_i64(lsum, lane);
@@ -2884,25 +2951,26 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
   if (hb == 0) return -1;
   const int L = __ffsll((unsigned long long)hb) - 1;
   const int ha = __shfl(hit, L, 64);
+  if (word) *word = __shfl(st4, L, 64);
   return c * LP_CHUNK + (int64_t)L * 4 + ha;
 }
 
 // resolve_in_tiles over span s.
-template <bool ATOMIC_IO, bool MARKED_AS_S>
-__device__ __forceinline__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane) {
+template <bool ATOMIC_IO>
+__device__ __forceinline__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane,
+                                                   uint32_t* word = nullptr) {
   const int4 span = d.spans[s];
-  return resolve_in_tiles<ATOMIC_IO, MARKED_AS_S>(d, span.y, span.z, sp_base, T, lane);
+  return resolve_in_tiles<ATOMIC_IO>(d, span.y, span.z, sp_base, T, lane, word);
 }
 
 
 // One wavefront per probe: target T in [0, W); the span with prefix <= T by a
 // 64-ary search over the node's span prefix, the chunk inside the span by a wave
 // scan of its chunk weights, the agent inside the chunk by a wave scan of the
-// agents' weights.  MARKED_AS_S: an agent already claimed in this round
-// (SEL_MARK) keeps its weight, so that all probes of a round see the weights of
-// the round's start.  Returns the slot, or -1 if the weights were inconsistent.
-template <bool ATOMIC_IO, bool MARKED_AS_S>
-__device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane) {
+// agents' weights.  Returns the slot, or -1 if the weights were inconsistent.
+template <bool ATOMIC_IO>
+__device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane,
+                                 uint32_t* word = nullptr) {
   int64_t lo = d.node_span_begin[node];
   int64_t hi = d.node_span_begin[node + 1];   // search spans [lo, hi)
   while (hi - lo > 1) {
@@ -2917,7 +2985,7 @@ __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_
     hi = nhi < hi ? nhi : hi;
   }
   const int64_t s = lo;
-  return resolve_in_span<ATOMIC_IO, MARKED_AS_S>(d, s, ld64<ATOMIC_IO>(prefix + s), T, lane);
+  return resolve_in_span<ATOMIC_IO>(d, s, ld64<ATOMIC_IO>(prefix + s), T, lane, word);
 }
 
 // Claim a hit: state byte 0 -> SEL_MARK with a word CAS; false if the agent was
@@ -2931,6 +2999,22 @@ __device__ __forceinline__ bool claim(int8_t* state, int64_t slot) {
     const uint32_t nw = (old & ~(0xffu << sh)) | ((uint32_t)(uint8_t)SEL_MARK << sh);
     const uint32_t prev = atomicCAS(w, old, nw);
     if (prev == old) return true;
+    old = prev;
+  }
+}
+
+// claim() starting from the state word `seen` (the word read when the agent was
+// resolved, its byte then S): the first compare-and-swap needs no load before it,
+// and a word changed since costs one more attempt with the value it returns.
+__device__ __forceinline__ bool claim_from(int8_t* state, int64_t slot, uint32_t seen) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(state) + (slot >> 2);
+  const int sh = 8 * (int)(slot & 3);
+  uint32_t old = seen & ~(0xffu << sh);
+  while (true) {
+    const uint32_t nw = old | ((uint32_t)(uint8_t)SEL_MARK << sh);
+    const uint32_t prev = atomicCAS(w, old, nw);
+    if (prev == old) return true;
+    if (((prev >> sh) & 0xffu) != 0) return false;
     old = prev;
   }
 }
@@ -3011,44 +3095,50 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
       node_lo = off[node];
       node_hi = off[node + 1];
       const NodeInfo& ni = d.ninfo[node];
+      const SpanLane sl = d.span_lane[(size_t)node * 64 + lane];   // beside the NodeInfo loads
       seed = ni.seed;
       lnode = (uint32_t)ni.lnode;
       W = d.W_round[node];
       s0 = ni.span_begin;
       ns = ni.n_spans;
-      const bool mine = ns <= 64 && lane < ns;
-      spx = mine ? d.span_prefix[s0 + lane] : INT64_MAX;
-      const int4 sp4 = mine ? d.spans[s0 + lane] : make_int4(0, 0, 0, 0);
-      stile = sp4.y;
-      sntile = sp4.z;
+      spx = ns <= 64 ? sl.prefix : INT64_MAX;
+      stile = sl.tile0;
+      sntile = sl.ntiles;
     }
     const int64_t j = p - node_lo;
     const uint4 o = philox((uint32_t)j, lnode, (uint32_t)t, ST_PROBE | (1u << 8), seed);
     const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
     const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
+    // no agent is claimed (SEL_MARK) before k_commit, so the state words are read plainly
     int64_t slot = -1;
+    uint32_t word = 0;
     if (W > 0) {
       if (ns <= 64) {
         const uint64_t ok = __ballot(spx <= T);      // monotone over the node's spans
         if (ok) {
           const int L = 63 - __clzll((unsigned long long)ok);
-          slot = resolve_in_tiles<false, true>(d, __shfl(stile, L, 64), __shfl(sntile, L, 64), __shfl(spx, L, 64), T,
-                                               lane);
+          slot = resolve_in_tiles<false>(d, __shfl(stile, L, 64), __shfl(sntile, L, 64), __shfl(spx, L, 64), T,
+                                         lane, &word);
         }
       } else {
-        slot = resolve_probe<false, true>(d, node, T, d.span_prefix, lane);
+        slot = resolve_probe<false>(d, node, T, d.span_prefix, lane, &word);
       }
     }
     if (lane == 0) {
       if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-      d.sel_list[base + p] = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)node << 40));
+      // a probe that found no agent keeps its node (k_commit writes its list position)
+      d.sel_list[base + p] = (slot < 0 ? SEL_SLOT : (uint64_t)slot) | ((uint64_t)node << 40);
+      d.sel_word[base + p] = word;
     }
   }
 }
 
 // Grid-wide, after round 1: the first probe of each agent hit in round 1 claims
 // it (the round's distinct agents, counted per node in n_uniq); a claimed agent
-// loses its weight and is exposed.
+// loses its weight and is exposed.  Probe j of node n owns position resv[n] + j of
+// the node's list for tomorrow (reserved by k_mix_foi): it writes its agent's entry
+// there, or a hole (state 0, skipped by every reader of the lists) when it lost the
+// claim; a node without a reservation (resv -1) appends with atomics instead.
 __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   const int4 dy = today(d);
   const int32_t t = dy.x;
@@ -3060,22 +3150,29 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
   // round 1 is the day's first reservation (base 0): the first entry is loaded with M
   const int64_t p_first = (int64_t)blockIdx.x * WG + threadIdx.x;
   const uint64_t e_first = p_first < d.sel_cap ? d.sel_list[p_first] : SEL_INVALID;
+  const uint32_t w_first = p_first < d.sel_cap ? d.sel_word[p_first] : 0u;
   for (int64_t p0 = (int64_t)blockIdx.x * WG + (threadIdx.x & ~63); p0 < M; p0 += stride) {
     const int64_t p = p0 + lane;
     int node = -1, k = -1;
     int64_t slot = 0;
     uint4 rec = make_uint4(0u, 0u, 0u, 0u);
     float r0v[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint4* at = nullptr;   // this probe's position in tomorrow's list, when reserved
     if (p < M) {
-      const uint64_t e = (p == p_first && base == 0) ? e_first : d.sel_list[base + p];
-      if (e != SEL_INVALID) {
-        slot = (int64_t)(e & ((1ull << 40) - 1));
-        const int nd = (int)(e >> 40);
-        if (nd >= d.n_nodes || slot >= d.capacity) {
-          atomicOr(d.err, ERR_SAMPLING);
-        } else {
-          // everything the exposure reads, loaded ahead of the claim
-          const NodeInfo& ni = d.ninfo[nd];
+      const bool first = p == p_first && base == 0;
+      const uint64_t e = first ? e_first : d.sel_list[base + p];
+      const uint32_t seen = first ? w_first : d.sel_word[base + p];
+      slot = (int64_t)(e & SEL_SLOT);
+      const int nd = (int)(e >> 40);
+      if (nd >= d.n_nodes || (slot != (int64_t)SEL_SLOT && slot >= d.capacity)) {
+        atomicOr(d.err, ERR_SAMPLING);
+      } else {
+        // everything the exposure reads, loaded ahead of the claim
+        const NodeInfo& ni = d.ninfo[nd];
+        const int32_t rb = d.resv[nd];
+        const int64_t j = p - d.probe_off[nd];
+        if (rb >= 0) at = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin + rb + j;
+        if (slot != (int64_t)SEL_SLOT) {             // the probe found an agent
           const int64_t lo = ni.seg_begin, hi = d.live_end[nd];
           const int64_t w = weight_q24(d.risk[slot]);
           const int64_t sp_i = ni.span_begin + (slot / LP_TILE - lo / LP_TILE) / d.tpw;
@@ -3089,7 +3186,7 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           rec = d.rec[slot];
           if (slot >= hi || slot < lo) {
             atomicOr(d.err, ERR_SAMPLING);
-          } else if (claim(d.state, slot)) {         // the first probe of an agent claims it
+          } else if (claim_from(d.state, slot, seen)) {   // the first probe of an agent claims it
             node = nd;
             unweight_in(d, sp_i, slot, w);
             k = expose_agent_thr(d, thr, ni.seed, ni.slot_base, slot, t);
@@ -3097,6 +3194,7 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
         }
       }
     }
+    if (at) *at = node >= 0 ? exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)) : make_uint4(0u, 0u, 0u, 0u);
     // new exposures by strain: one atomic per strain when the wave's entries share a node
     const int n0 = __shfl(node, __ffsll((unsigned long long)__ballot(node >= 0)) - 1, 64);
     const bool same = __ballot(node >= 0 && node != n0) == 0;
@@ -3108,14 +3206,12 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
           if (lane == 0 && c) atomicAdd(d.newexp + n0 * LP_MAX_STRAINS + s, c);
         }
         if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
-        const uint4 ent = node >= 0 ? exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)) : make_uint4(0u, 0u, 0u, 0u);
-        wave_append_entry(d, nxt, n0, ent, node >= 0, lane);
       }
     } else if (node >= 0) {
       atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
       atomicAdd(d.n_uniq + node, 1);
-      append_entry(d, nxt, node, exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)));
     }
+    if (node >= 0 && !at) append_entry(d, nxt, node, exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)));
   }
 }
 
@@ -3220,11 +3316,11 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           const uint64_t ok = __ballot(spx <= T);         // monotone over the node's spans
           if (ok) {
             const int L = 63 - __clzll((unsigned long long)ok);
-            slot = resolve_in_tiles<true, false>(d, __shfl(sp4.y, L, 64), __shfl(sp4.z, L, 64), __shfl(spx, L, 64), T,
-                                                 lane);
+            slot = resolve_in_tiles<true>(d, __shfl(sp4.y, L, 64), __shfl(sp4.z, L, 64), __shfl(spx, L, 64), T,
+                                          lane);
           }
         } else {
-          slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
+          slot = resolve_probe<true>(d, n, T, d.fin_prefix, lane);
         }
         if (lane == 0) {
           const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));
@@ -3346,8 +3442,10 @@ __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, 
   const int4 next = (advance && threadIdx.x == 0) ? day_record(d, t + 1) : make_int4(0, 0, 0, 0);
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < d.n_nodes) {
+    // the next day's inputs are loaded before today's log rows are stored
+    const DayBeginIn in = begin_next ? day_begin_load(d, n, t + 1, mask) : DayBeginIn{};
     day_end_node(d, n, t, dy.y, mask, tx_adjust);
-    if (begin_next) day_begin_node(d, n, t + 1, mask);
+    if (begin_next) day_begin_apply(d, n, t + 1, in);
   }
   if (advance) {
     // the last workgroup to get here moves the day counter: every workgroup has read it

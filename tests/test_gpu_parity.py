@@ -75,6 +75,25 @@ def test_per_component_steps_match_oracle(case):
     _compare(sim, ref)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["campaigns", "c0mini"])
+def test_one_day_calls_with_syncs_match_oracle(case):
+    """One lp_run_days call per day with a column sync after each: every sync must write
+    the records the day changed back into the columns, and the next call must continue
+    from the device-held lists and carried statistics."""
+    print(f"[parity-day-calls] case={case}", flush=True)
+    sim = make(case)
+    ref = OracleSim(sim.export_tables(), pop0=sim.pars.init_pop).run()
+    eng = sim._ensure_engine()
+    eng.run_days(0, 1, use_graph=True)
+    for t in range(1, sim.nt):
+        eng.run_days(t, t + 1, use_graph=True)
+        eng.sync()
+    sim._mark_device_newer()
+    sim.t = sim.nt
+    _compare(sim, ref)
+
+
 def _host_edit(state, inf_timer, risk, slots):
     """The edit a host would make between days: some susceptibles become infectious."""
     sus = slots[state[slots] == 0][:40]
