@@ -115,7 +115,7 @@ def main():
     ap.add_argument("--roofline-pad-mib", type=int, default=4096,
                     help="MiB copied 4 times on the library's stream ahead of the roofline window")
     ap.add_argument("--tiles-per-block", type=int, default=0)
-    for g in ("ei", "sample", "commit"):
+    for g in ("ei", "sample"):
         ap.add_argument(f"--{g}-grid", type=int, default=0, help=f"national: {g}_grid workgroups (0 = library default)")
     ap.add_argument("--init", choices=["device", "host"], default="device",
                     help="draw the initial population on the GPU (lp_init_population) or on the host (numpy)")
@@ -327,7 +327,7 @@ def national(args):
     pars.init_on_device = args.init == "device"
     if args.tiles_per_block:
         pars.tiles_per_block = args.tiles_per_block
-    for g in ("ei_grid", "sample_grid", "commit_grid"):
+    for g in ("ei_grid", "sample_grid"):
         if getattr(args, g):
             pars[g] = getattr(args, g)
     sim = configs.build(pars)

@@ -90,9 +90,9 @@ typedef struct {
   int32_t mix_mode;        /* network mix: 0 = per-node fixed-order tree reduction (the oracle's order,
                               bit-identical replays); 1 = every simulation's prevalence x network
                               contraction on MFMA (within 1e-6 of float64, not replay-identical) */
-  int32_t ei_grid;         /* workgroups of the E / I list pass, 64..65535 (0 = 2048) */
-  int32_t sample_grid;     /* workgroups of the round-1 sampler, 64..65535 (0 = 2048) */
-  int32_t commit_grid;     /* workgroups of the round-1 commit, 64..65535 (0 = 512) */
+  int32_t ei_grid;         /* workgroups of the E / I list pass, 64..65535 (0 = those resident at once, <= 2048) */
+  int32_t sample_grid;     /* workgroups of the round-1 sampler, 64..65535 (0 = those resident at once, <= 2048) */
+  int32_t commit_grid;     /* unused (the round-1 claims run inside the sampler); 0 or 64..65535 */
   int32_t reserved[4];
 } lp_config;
 

@@ -266,7 +266,7 @@ class BatchedSims:
                 for name in AGENT_COLUMNS}
         return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
 
-    GRID_CANDIDATES = {"ei_grid": (1024, 2048, 4096), "sample_grid": (1024, 2048, 4096), "commit_grid": (256, 512, 1024)}
+    GRID_CANDIDATES = {"ei_grid": (1024, 2048, 4096), "sample_grid": (1024, 2048, 4096)}
 
     def _time_subset(self, ks, days, use_graph, tpw, grids):
         import torch
@@ -289,9 +289,8 @@ class BatchedSims:
     def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True, grids=False):
         """Pick tiles_per_block by timing `days` simulated days of the first `max_sims`
         simulations with each candidate (fresh contexts, so nothing of this batch runs);
-        with `grids`, then the workgroup counts of the list pass, the round-1 sampler and
-        its commit, one after the other from GRID_CANDIDATES (the launch shapes of
-        lp_config).  Returns (tiles_per_block, times by tiles_per_block), and records the
+        with `grids`, then the workgroup counts of the list pass and the round-1 sampler,
+        one after the other from GRID_CANDIDATES (the launch shapes of lp_config).  Returns (tiles_per_block, times by tiles_per_block), and records the
         chosen grids in self.cfg and the trial times in self.tune_log."""
         ks = range(min(max_sims, len(self.packs)))
         times = {tpw: self._time_subset(ks, days, use_graph, tpw, {}) for tpw in candidates}

@@ -103,7 +103,7 @@ struct lp_ctx {
   int32_t n_spans = 0;
   int32_t tpw = 4;
   int32_t agent_grid = 1;
-  int32_t ei_grid = 2048, sample_grid = 2048, commit_grid = 512;   // launch shapes of the list and sampling kernels
+  int32_t ei_grid = 2048, sample_grid = 2048;     // launch shapes of the list pass and the sampler
   int32_t mix_tiles = 0;                           // mix_mode 1: 16-row tiles of the largest simulation
   std::vector<void*> allocs;
   // per day kind (plain, FULL_* combinations, their list forms): graph[kind][j] runs 2^j
@@ -173,11 +173,11 @@ int alloc(lp_ctx* c, T** p, size_t n) {
 
 int grid_of(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
-// Default workgroups of k_ei (4 wavefronts each; each wavefront loops over list pieces),
-// k_sample and k_commit; lp_config may override them (the batch autotuner does).
+// Most workgroups of k_ei (4 wavefronts each; each wavefront loops over list pieces) and
+// k_sample by default (fewer when fewer are resident at once); lp_config may set them
+// (the batch autotuner does).
 constexpr int kEiGrid = 2048;
 constexpr int kSampleGrid = 2048;
-constexpr int kCommitGrid = 512;
 // Workgroups per node of k_ri_list (each takes every kRiSplit-th batch of 1024 candidates).
 constexpr int kRiSplit = 8;
 
@@ -347,7 +347,6 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
     if (!off_lds) hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
     hipLaunchKernelGGL(lp::k_sample, dim3(c->sample_grid), dim3(lp::WG), off_lds, s, d);
-    hipLaunchKernelGGL(lp::k_commit, dim3(c->commit_grid), dim3(lp::WG), 0, s, d);
     hipLaunchKernelGGL(lp::k_finish, dim3(N), dim3(lp::WG), 0, s, d);
   }
   hipLaunchKernelGGL(lp::k_day_end, dim3(grid_of(N, 256)), dim3(256), 0, s, d, mask, tx_adjust, advance ? 1 : 0);
@@ -392,13 +391,12 @@ int lp_create(const lp_config* cfg, lp_ctx** out) {
       cfg->n_strains < 1 || cfg->n_strains > LP_MAX_STRAINS || cfg->nt < 1 || cfg->nt > (1 << 23) || cfg->n_sia_events < 0 ||
       cfg->n_nodes > 32767 || cfg->capacity >= (int64_t)1 << 32 || cfg->mix_mode < 0 || cfg->mix_mode > 1)
     return LP_ERR_ARG;
-  for (int32_t g : {cfg->ei_grid, cfg->sample_grid, cfg->commit_grid})
+  for (int32_t g : {cfg->ei_grid, cfg->sample_grid, cfg->commit_grid})   // commit_grid: validated, unused
     if (g != 0 && (g < 64 || g > 65535)) return LP_ERR_ARG;
   lp_ctx* c = new lp_ctx();
   c->cfg = *cfg;
   c->ei_grid = cfg->ei_grid ? cfg->ei_grid : kEiGrid;
   c->sample_grid = cfg->sample_grid ? cfg->sample_grid : kSampleGrid;
-  c->commit_grid = cfg->commit_grid ? cfg->commit_grid : kCommitGrid;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return LP_ERR_HIP; }
   c->own_stream = true;
   *out = c;
@@ -511,6 +509,16 @@ int lp_prepare(lp_ctx* c) {
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     c->agent_grid = std::max(1, std::min(c->n_spans, cus * 8));
+    // default grids of k_ei and k_sample: as many workgroups as are resident at once
+    // (occupancy x CUs, at most 2048), so none waits for another to finish before it starts
+    auto resident = [&](const void* kernel, size_t lds, int cap) -> int {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, lp::WG, lds) != hipSuccess || nb < 1) return cap;
+      return std::max(64, std::min(cap, nb * cus));
+    };
+    const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
+    if (!c->cfg.ei_grid) c->ei_grid = resident(reinterpret_cast<const void*>(lp::k_ei), 0, kEiGrid);
+    if (!c->cfg.sample_grid) c->sample_grid = resident(reinterpret_cast<const void*>(lp::k_sample), off_lds, kSampleGrid);
   }
   int4* spans_d; int32_t* nspan_d;
   int rc;

@@ -20,14 +20,14 @@
 //   k_mix_foi      per node: network mix (fixed-order tree), seasonality, r0 scalars,
 //                  rate -> probability, expected exposures, Poisson / zero-inflated NB
 //                  count draw (chunks in parallel across the wave)  model.py:1327-1407
-//   k_plan, k_sample, k_commit, k_finish
+//   k_plan, k_sample, k_finish
 //                  weighted sampling without replacement by rounds of with-
 //                  replacement probes + unique (model.py:1089-1122): every probe
 //                  searches its node's span prefix, then a span's chunks, then one
 //                  chunk, one wavefront per probe; round 1 grid-wide (planned
-//                  inside k_sample, by k_plan above 8191 nodes; k_commit
-//                  claims and exposes its agents), later rounds and the take-all
-//                  nodes per node in k_finish; each exposed agent's strain is drawn
+//                  inside k_sample, by k_plan above 8191 nodes; k_sample claims
+//                  its agents, k_finish makes them E), later rounds and the
+//                  take-all nodes per node in k_finish; each exposed agent's strain is drawn
 //                  as it is exposed (S -> E) and it joins the next day's E / I list
 //                                                               model.py:1125-1147
 //   k_day_end      per node: log rows + transmission incidence; its last
@@ -131,7 +131,7 @@ struct Dev {
   int32_t* size; int32_t* take_all; int32_t* n_uniq; int32_t* m_round; int64_t* W_round;
   uint64_t* thr_strain; int32_t* newexp;
   int64_t* probe_off; int64_t* round_base; uint64_t* sel_list; int64_t* sel_count; int64_t sel_cap;
-  uint32_t* sel_word;      // [sel_cap] round 1: the hit agent's state word as the sampler read it
+  uint32_t* sel_word;      // [sel_cap] round 1: a claimed agent's selection weight (all ones: recompute)
   int64_t* fin_prefix; int64_t* span_prefix;
   SpanLane* span_lane;     // [nodes][64]
   int32_t* resv;           // [nodes] first list position of today's round-1 exposures (-1: appended)
@@ -2816,7 +2816,7 @@ __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t,
   d.take_all[i] = ta;
   const int32_t m = ta ? 0 : sz;
   d.m_round[i] = m;
-  // positions for the round's probes in tomorrow's list (k_commit), when the segment
+  // positions for the round's probes in tomorrow's list (k_sample), when the segment
   // also holds every later exposure of the node (at most sz more, appended)
   if ((int64_t)lbase + m + sz <= room) {
     *lcnt = lbase + m;
@@ -2895,9 +2895,8 @@ __global__ void __launch_bounds__(WG) k_plan(Dev d, int round) { plan_scan(d, ro
 // Infection sampling (model.py:1089-1122): rounds of `need` with-replacement
 // probes proportional to the remaining selection weights; the distinct agents
 // hit are exposed.  Round 1 runs grid-wide (k_sample, with k_plan first above
-// LDS_OFF_MAX - 1 nodes, then k_commit);
-// k_finish then takes each node on its own for its later rounds and handles the
-// take-all nodes.
+// LDS_OFF_MAX - 1 nodes); k_finish then takes each node on its own: round 1's
+// agents become E, then its later rounds, and the take-all nodes.
 
 __device__ __forceinline__ int64_t find_node(const int64_t* off, int N, int64_t p) {
   int a = 0, b = N;   // largest n in [0, N) with off[n] <= p < off[n+1]
@@ -2912,12 +2911,13 @@ __device__ __forceinline__ int64_t ld64(const int64_t* p) {
 
 // The agent hit by target T inside the span of `ntiles` tiles from tile `tile0`,
 // whose prefix (weight before it) is sp_base: the chunk by a wave scan of the span's
-// chunk weights, the agent by a wave scan of the chunk's agent weights (only S
-// agents, state 0, weigh).  ATOMIC_IO: the weights and states are read with atomic
-// loads (other workgroups may change them during the kernel).  Returns the slot, or -1
-// if the weights were inconsistent; *word (when given) receives the hit agent's state
-// word as read here.
-template <bool ATOMIC_IO>
+// chunk weights, the agent by a wave scan of the chunk's agent weights (S agents,
+// state 0, weigh).  ATOMIC_IO: the weights and states are read with atomic loads
+// (other wavefronts change them during the kernel).  MARKED_AS_S: an agent claimed
+// earlier in the round (SEL_MARK) keeps its weight, so that every probe of a round
+// sees the weights of the round's start.  Returns the slot, or -1 if the weights were
+// inconsistent; *word (when given) receives the hit agent's state word as read here.
+template <bool ATOMIC_IO, bool MARKED_AS_S>
 __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int64_t sp_base, int64_t T, int lane,
                                     uint32_t* word = nullptr) {
   const int64_t c0 = tile0 * (LP_TILE / LP_CHUNK);
@@ -2931,7 +2931,7 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
   const int64_t c = c0 + Lc;
   const int64_t i0 = c * LP_CHUNK + (int64_t)lane * 4;
   uint32_t st4;
-  if (ATOMIC_IO)
+  if (ATOMIC_IO || MARKED_AS_S)
     st4 = __hip_atomic_load(reinterpret_cast<const uint32_t*>(d.state) + (i0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   else
     st4 = reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2];
@@ -2940,7 +2940,8 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const uint32_t b = (st4 >> (8 * a)) & 0xffu;
-    w[a] = b == 0 ? weight_q24(lane_of(rk, a)) : 0;
+    const bool s_like = b == 0 || (MARKED_AS_S && b == (uint32_t)(uint8_t)SEL_MARK);
+    w[a] = s_like ? weight_q24(lane_of(rk, a)) : 0;
   }
   const int64_t lsum = w[0] + w[1] + w[2] + w[3];
   const int64_t incl = wave_incl_scan_i64(lsum, lane);
@@ -2956,19 +2957,20 @@ __device__ int64_t resolve_in_tiles(const Dev& d, int64_t tile0, int ntiles, int
 }
 
 // resolve_in_tiles over span s.
-template <bool ATOMIC_IO>
+template <bool ATOMIC_IO, bool MARKED_AS_S>
 __device__ __forceinline__ int64_t resolve_in_span(const Dev& d, int64_t s, int64_t sp_base, int64_t T, int lane,
                                                    uint32_t* word = nullptr) {
   const int4 span = d.spans[s];
-  return resolve_in_tiles<ATOMIC_IO>(d, span.y, span.z, sp_base, T, lane, word);
+  return resolve_in_tiles<ATOMIC_IO, MARKED_AS_S>(d, span.y, span.z, sp_base, T, lane, word);
 }
 
 
 // One wavefront per probe: target T in [0, W); the span with prefix <= T by a
 // 64-ary search over the node's span prefix, the chunk inside the span by a wave
 // scan of its chunk weights, the agent inside the chunk by a wave scan of the
-// agents' weights.  Returns the slot, or -1 if the weights were inconsistent.
-template <bool ATOMIC_IO>
+// agents' weights (ATOMIC_IO, MARKED_AS_S as for resolve_in_tiles).  Returns the
+// slot, or -1 if the weights were inconsistent.
+template <bool ATOMIC_IO, bool MARKED_AS_S>
 __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_t* prefix, int lane,
                                  uint32_t* word = nullptr) {
   int64_t lo = d.node_span_begin[node];
@@ -2985,7 +2987,7 @@ __device__ int64_t resolve_probe(const Dev& d, int node, int64_t T, const int64_
     hi = nhi < hi ? nhi : hi;
   }
   const int64_t s = lo;
-  return resolve_in_span<ATOMIC_IO>(d, s, ld64<ATOMIC_IO>(prefix + s), T, lane, word);
+  return resolve_in_span<ATOMIC_IO, MARKED_AS_S>(d, s, ld64<ATOMIC_IO>(prefix + s), T, lane, word);
 }
 
 // Claim a hit: state byte 0 -> SEL_MARK with a word CAS; false if the agent was
@@ -3024,27 +3026,43 @@ __device__ __forceinline__ void unweight(const Dev& d, int node, int64_t slot) {
   unweight_in(d, span_of(d, node, slot), slot, weight_q24(d.risk[slot]));
 }
 
-// S -> E of a selected agent with its strain drawn by cumulative strain probability
-// (model.py:1125-1147), against the node's thresholds thr[0..K) (already loaded);
-// returns the strain (the caller counts it).
-__device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* thr, uint64_t seed, int64_t slot_base,
-                                                int64_t slot, int32_t t) {
+// The strain of a selected agent, drawn by cumulative strain probability
+// (model.py:1125-1147) against the node's thresholds thr[0..K) (already loaded).
+__device__ __forceinline__ int strain_draw(const Dev& d, const uint64_t* thr, uint64_t seed, int64_t slot_base,
+                                           int64_t slot, int32_t t) {
   const uint4 o = draw(seed, (uint64_t)(slot - slot_base), t, ST_STRAIN);
   int k = 0;
 #pragma unroll
   for (int s = LP_MAX_STRAINS - 1; s >= 0; --s)
     if (s < d.K && (uint64_t)o.x < thr[s]) k = s;   // the first strain whose threshold exceeds the draw
+  return k;
+}
+
+// S -> E of a selected agent with its drawn strain; returns the strain (the caller
+// counts it).
+__device__ __forceinline__ int expose_agent_thr(const Dev& d, const uint64_t* thr, uint64_t seed, int64_t slot_base,
+                                                int64_t slot, int32_t t) {
+  const int k = strain_draw(d, thr, seed, slot_base, slot, t);
   d.state[slot] = 1;
   *rec_byte(d, slot, RB_STRAIN) = (int8_t)k;
   return k;
 }
 
+constexpr uint64_t SEL_WON = 1ull << 63;   // round-1 entry whose probe claimed its agent
 
-// Sampling round 1, grid-wide: one wavefront per probe resolves it to an agent
-// (k_commit claims the hits).  When the probe offsets fit in LDS (dynamic shared
-// memory of (n_nodes + 1) * 8 bytes) every workgroup scans the nodes' round-1 probe
-// counts itself and workgroup 0 publishes the plan (round 1 is the day's first
-// reservation in the selected list: base 0); otherwise k_plan planned the round.
+// Sampling round 1, grid-wide: one wavefront per probe resolves it to an agent, and
+// the first probe of each agent hit claims it (the round's distinct agents, counted
+// per node in n_uniq).  Every probe of the round must see the weights of the round's
+// start, so a claimed agent keeps its weight and reads as S (state SEL_MARK) until
+// k_finish makes it E and removes its weight; its strain, record byte and entry for
+// tomorrow's list are written here.  Probe j of node n owns position resv[n] + j of
+// the node's list for tomorrow (reserved by k_mix_foi): the agent's entry, or a hole
+// (state 0, skipped by every reader of the lists) when the probe lost the claim; a
+// node without a reservation (resv -1) appends with atomics instead.  When the probe
+// offsets fit in LDS (dynamic shared memory of (n_nodes + 1) * 8 bytes) every
+// workgroup scans the nodes' round-1 probe counts itself and workgroup 0 publishes the
+// plan (round 1 is the day's first reservation in the selected list: base 0);
+// otherwise k_plan planned the round.
 constexpr int LDS_OFF_MAX = 8192;
 __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   extern __shared__ int64_t s_off[];
@@ -3077,6 +3095,7 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int64_t* off = lds ? s_off : d.probe_off;
   const int4 dy = today(d);
   const int32_t t = dy.x;
+  const int nxt = (t + 1) & 1;
   const int lane = threadIdx.x & 63;
   const int64_t base = lds ? 0 : d.round_base[1];
   // each wave takes a contiguous range of probes (probes are ordered by node); the
@@ -3089,117 +3108,90 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   int64_t node_lo = 0, node_hi = 0, W = 0, s0 = 0, spx = INT64_MAX;
   uint64_t seed = 0;
   uint32_t lnode = 0;
-  for (int64_t p = p0; p < p1; ++p) {
-    if (node < 0 || p >= node_hi) {
-      node = __builtin_amdgcn_readfirstlane((int)find_node(off, N, p));
-      node_lo = off[node];
-      node_hi = off[node + 1];
-      const NodeInfo& ni = d.ninfo[node];
-      const SpanLane sl = d.span_lane[(size_t)node * 64 + lane];   // beside the NodeInfo loads
-      seed = ni.seed;
-      lnode = (uint32_t)ni.lnode;
-      W = d.W_round[node];
-      s0 = ni.span_begin;
-      ns = ni.n_spans;
-      spx = ns <= 64 ? sl.prefix : INT64_MAX;
-      stile = sl.tile0;
-      sntile = sl.ntiles;
-    }
-    const int64_t j = p - node_lo;
-    const uint4 o = philox((uint32_t)j, lnode, (uint32_t)t, ST_PROBE | (1u << 8), seed);
-    const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
-    const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
-    // no agent is claimed (SEL_MARK) before k_commit, so the state words are read plainly
-    int64_t slot = -1;
-    uint32_t word = 0;
-    if (W > 0) {
-      if (ns <= 64) {
-        const uint64_t ok = __ballot(spx <= T);      // monotone over the node's spans
-        if (ok) {
-          const int L = 63 - __clzll((unsigned long long)ok);
-          slot = resolve_in_tiles<false>(d, __shfl(stile, L, 64), __shfl(sntile, L, 64), __shfl(spx, L, 64), T,
-                                         lane, &word);
-        }
-      } else {
-        slot = resolve_probe<false>(d, node, T, d.span_prefix, lane, &word);
+  // groups of up to 64 probes: resolved one at a time by the whole wave, lane q keeping
+  // probe g0 + q's agent; then every lane claims its own at once
+  for (int64_t g0 = p0; g0 < p1; g0 += 64) {
+    const int64_t g1 = g0 + 64 < p1 ? g0 + 64 : p1;
+    int64_t my_slot = -1, my_j = 0;
+    uint32_t my_word = 0;
+    int my_node = -1;
+    for (int64_t p = g0; p < g1; ++p) {
+      if (node < 0 || p >= node_hi) {
+        node = __builtin_amdgcn_readfirstlane((int)find_node(off, N, p));
+        node_lo = off[node];
+        node_hi = off[node + 1];
+        const NodeInfo& ni = d.ninfo[node];
+        const SpanLane sl = d.span_lane[(size_t)node * 64 + lane];   // beside the NodeInfo loads
+        seed = ni.seed;
+        lnode = (uint32_t)ni.lnode;
+        W = d.W_round[node];
+        s0 = ni.span_begin;
+        ns = ni.n_spans;
+        spx = ns <= 64 ? sl.prefix : INT64_MAX;
+        stile = sl.tile0;
+        sntile = sl.ntiles;
       }
+      const int64_t j = p - node_lo;
+      const uint4 o = philox((uint32_t)j, lnode, (uint32_t)t, ST_PROBE | (1u << 8), seed);
+      const uint64_t u = (uint64_t)o.x | ((uint64_t)o.y << 32);
+      const int64_t T = (int64_t)mulhi64(u, (uint64_t)(W > 0 ? W : 0));
+      int64_t slot = -1;
+      uint32_t word = 0;
+      if (W > 0) {
+        if (ns <= 64) {
+          const uint64_t ok = __ballot(spx <= T);      // monotone over the node's spans
+          if (ok) {
+            const int L = 63 - __clzll((unsigned long long)ok);
+            slot = resolve_in_tiles<false, true>(d, __shfl(stile, L, 64), __shfl(sntile, L, 64), __shfl(spx, L, 64),
+                                                 T, lane, &word);
+          }
+        } else {
+          slot = resolve_probe<false, true>(d, node, T, d.span_prefix, lane, &word);
+        }
+      }
+      if (lane == (int)(p - g0)) { my_slot = slot; my_word = word; my_node = node; my_j = j; }
     }
-    if (lane == 0) {
-      if (slot < 0) atomicOr(d.err, ERR_SAMPLING);
-      // a probe that found no agent keeps its node (k_commit writes its list position)
-      d.sel_list[base + p] = (slot < 0 ? SEL_SLOT : (uint64_t)slot) | ((uint64_t)node << 40);
-      d.sel_word[base + p] = word;
-    }
-  }
-}
-
-// Grid-wide, after round 1: the first probe of each agent hit in round 1 claims
-// it (the round's distinct agents, counted per node in n_uniq); a claimed agent
-// loses its weight and is exposed.  Probe j of node n owns position resv[n] + j of
-// the node's list for tomorrow (reserved by k_mix_foi): it writes its agent's entry
-// there, or a hole (state 0, skipped by every reader of the lists) when it lost the
-// claim; a node without a reservation (resv -1) appends with atomics instead.
-__global__ void __launch_bounds__(WG) k_commit(Dev d) {
-  const int4 dy = today(d);
-  const int32_t t = dy.x;
-  const int nxt = (t + 1) & 1;
-  const int64_t M = d.probe_off[d.n_nodes];
-  const int64_t base = d.round_base[1];
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * WG;
-  // round 1 is the day's first reservation (base 0): the first entry is loaded with M
-  const int64_t p_first = (int64_t)blockIdx.x * WG + threadIdx.x;
-  const uint64_t e_first = p_first < d.sel_cap ? d.sel_list[p_first] : SEL_INVALID;
-  const uint32_t w_first = p_first < d.sel_cap ? d.sel_word[p_first] : 0u;
-  for (int64_t p0 = (int64_t)blockIdx.x * WG + (threadIdx.x & ~63); p0 < M; p0 += stride) {
-    const int64_t p = p0 + lane;
-    int node = -1, k = -1;
-    int64_t slot = 0;
+    // the group's claims, lane q for probe g0 + q
+    const int64_t p = g0 + lane;
+    int k = -1;
+    bool won = false;
     uint4 rec = make_uint4(0u, 0u, 0u, 0u);
     float r0v[LP_MAX_STRAINS] = {0.0f, 0.0f, 0.0f, 0.0f};
     uint4* at = nullptr;   // this probe's position in tomorrow's list, when reserved
-    if (p < M) {
-      const bool first = p == p_first && base == 0;
-      const uint64_t e = first ? e_first : d.sel_list[base + p];
-      const uint32_t seen = first ? w_first : d.sel_word[base + p];
-      slot = (int64_t)(e & SEL_SLOT);
-      const int nd = (int)(e >> 40);
-      if (nd >= d.n_nodes || (slot != (int64_t)SEL_SLOT && slot >= d.capacity)) {
-        atomicOr(d.err, ERR_SAMPLING);
-      } else {
-        // everything the exposure reads, loaded ahead of the claim
-        const NodeInfo& ni = d.ninfo[nd];
-        const int32_t rb = d.resv[nd];
-        const int64_t j = p - d.probe_off[nd];
-        if (rb >= 0) at = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin + rb + j;
-        if (slot != (int64_t)SEL_SLOT) {             // the probe found an agent
-          const int64_t lo = ni.seg_begin, hi = d.live_end[nd];
-          const int64_t w = weight_q24(d.risk[slot]);
-          const int64_t sp_i = ni.span_begin + (slot / LP_TILE - lo / LP_TILE) / d.tpw;
-          uint64_t thr[LP_MAX_STRAINS];
+    if (p < g1) {
+      const NodeInfo& ni = d.ninfo[my_node];
+      const int32_t rb = d.resv[my_node];
+      if (rb >= 0) at = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin + rb + my_j;
+      int64_t w = 0;
+      if (my_slot >= 0) {
+        // everything the exposure reads, loaded beside the claim
+        uint64_t thr[LP_MAX_STRAINS];
 #pragma unroll
-          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
-            thr[q] = d.thr_strain[(size_t)nd * LP_MAX_STRAINS + q];
-            r0v[q] = ni.r0k[q];
-          }
-          // the record fields of the list entry do not change when an agent is exposed
-          rec = d.rec[slot];
-          if (slot >= hi || slot < lo) {
-            atomicOr(d.err, ERR_SAMPLING);
-          } else if (claim_from(d.state, slot, seen)) {   // the first probe of an agent claims it
-            node = nd;
-            unweight_in(d, sp_i, slot, w);
-            k = expose_agent_thr(d, thr, ni.seed, ni.slot_base, slot, t);
-          }
+        for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+          thr[q] = d.thr_strain[(size_t)my_node * LP_MAX_STRAINS + q];
+          r0v[q] = ni.r0k[q];
         }
+        rec = d.rec[my_slot];   // the entry's record fields do not change when an agent is exposed
+        w = weight_q24(d.risk[my_slot]);
+        if (claim_from(d.state, my_slot, my_word)) {   // the first probe of an agent claims it
+          won = true;
+          k = strain_draw(d, thr, ni.seed, ni.slot_base, my_slot, t);
+          *rec_byte(d, my_slot, RB_STRAIN) = (int8_t)k;
+        }
+      } else {
+        atomicOr(d.err, ERR_SAMPLING);
       }
+      d.sel_list[base + p] = (my_slot < 0 ? SEL_SLOT : (uint64_t)my_slot) | ((uint64_t)my_node << 40) | (won ? SEL_WON : 0);
+      if (won) d.sel_word[base + p] = w < (int64_t)0xFFFFFFFF ? (uint32_t)w : 0xFFFFFFFFu;   // k_finish removes it
     }
-    if (at) *at = node >= 0 ? exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)) : make_uint4(0u, 0u, 0u, 0u);
-    // new exposures by strain: one atomic per strain when the wave's entries share a node
-    const int n0 = __shfl(node, __ffsll((unsigned long long)__ballot(node >= 0)) - 1, 64);
-    const bool same = __ballot(node >= 0 && node != n0) == 0;
+    const int wn = won ? my_node : -1;
+    const uint4 ent = won ? exposed_entry_rec(my_slot, rec, k, t, r0_sel(r0v, k)) : make_uint4(0u, 0u, 0u, 0u);
+    if (at) *at = ent;
+    // new exposures by strain: one atomic per strain when the group's winners share a node
+    const int n0 = __shfl(wn, __ffsll((unsigned long long)__ballot(wn >= 0)) - 1, 64);
+    const bool same = __ballot(wn >= 0 && wn != n0) == 0;
     if (same) {
-      const uint64_t any = __ballot(node >= 0);
+      const uint64_t any = __ballot(wn >= 0);
       if (any) {
         for (int s = 0; s < d.K; ++s) {
           const int c = __popcll(__ballot(k == s));
@@ -3207,11 +3199,11 @@ __global__ void __launch_bounds__(WG) k_commit(Dev d) {
         }
         if (lane == 0) atomicAdd(d.n_uniq + n0, __popcll(any));
       }
-    } else if (node >= 0) {
-      atomicAdd(d.newexp + node * LP_MAX_STRAINS + k, 1);
-      atomicAdd(d.n_uniq + node, 1);
+    } else if (wn >= 0) {
+      atomicAdd(d.newexp + wn * LP_MAX_STRAINS + k, 1);
+      atomicAdd(d.n_uniq + wn, 1);
     }
-    if (node >= 0 && !at) append_entry(d, nxt, node, exposed_entry_rec(slot, rec, k, t, r0_sel(r0v, k)));
+    if (won && !at) append_entry(d, nxt, wn, ent);
   }
 }
 
@@ -3234,9 +3226,29 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   __shared__ int32_t s_ne[LP_MAX_STRAINS];   // the node's new exposures by strain (one global add each)
   __shared__ int32_t s_claimed;
   // most nodes have nothing left after round 1: one round of loads decides
-  const int32_t size = d.size[n], take_all = d.take_all[n], uniq0 = d.n_uniq[n];
-  if (size == 0 || (!take_all && size - uniq0 <= 0)) return;
+  const int32_t size = d.size[n], take_all = d.take_all[n], uniq0 = d.n_uniq[n], m1 = d.m_round[n];
+  const bool later = size > 0 && (take_all || size - uniq0 > 0);
   const NodeInfo& ni = d.ninfo[n];
+  // (a round 1 that overflowed the selected list ran no probes: probe_off[N] = 0)
+  if (m1 > 0 && d.probe_off[d.n_nodes] > 0) {
+    // round 1's winners (claimed by k_sample, state SEL_MARK): E now, and their weights
+    // leave the chunk and span sums before any later round reads them
+    const int64_t b1 = d.round_base[1] + d.probe_off[n];
+    for (int j = tid; j < m1; j += WG) {
+      const uint64_t e = d.sel_list[b1 + j];
+      const uint32_t wq = d.sel_word[b1 + j];
+      if (e & SEL_WON) {
+        const int64_t slot = (int64_t)(e & SEL_SLOT);
+        const int64_t w = wq != 0xFFFFFFFFu ? (int64_t)wq : weight_q24(d.risk[slot]);
+        d.state[slot] = 1;
+        unweight_in(d, ni.span_begin + (slot / LP_TILE - ni.seg_begin / LP_TILE) / d.tpw, slot, w);
+      }
+    }
+    if (!later) return;
+    __syncthreads();   // the weights and states before the later rounds
+  } else if (!later) {
+    return;
+  }
   if (tid < LP_MAX_STRAINS) s_ne[tid] = 0;
   if (tid == 0) s_claimed = 0;
   uint64_t thr[LP_MAX_STRAINS];   // the node's cumulative strain thresholds, loaded once
@@ -3316,11 +3328,11 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
           const uint64_t ok = __ballot(spx <= T);         // monotone over the node's spans
           if (ok) {
             const int L = 63 - __clzll((unsigned long long)ok);
-            slot = resolve_in_tiles<true>(d, __shfl(sp4.y, L, 64), __shfl(sp4.z, L, 64), __shfl(spx, L, 64), T,
+            slot = resolve_in_tiles<true, false>(d, __shfl(sp4.y, L, 64), __shfl(sp4.z, L, 64), __shfl(spx, L, 64), T,
                                           lane);
           }
         } else {
-          slot = resolve_probe<true>(d, n, T, d.fin_prefix, lane);
+          slot = resolve_probe<true, false>(d, n, T, d.fin_prefix, lane);
         }
         if (lane == 0) {
           const uint64_t e = slot < 0 ? SEL_INVALID : ((uint64_t)slot | ((uint64_t)n << 40));

@@ -629,7 +629,6 @@ class SEIR_ABM:
             mix_mode=int(self.pars.mix_mode or 0),
             ei_grid=int(self.pars.get("ei_grid") or 0),
             sample_grid=int(self.pars.get("sample_grid") or 0),
-            commit_grid=int(self.pars.get("commit_grid") or 0),
         )
         device_only = set(AGENT_COLUMNS) if (self.device_init and not self._device_initialized) else set()
         self._engine = engmod.Engine(cfg, arrays, timing=self._timing, device_only=device_only)

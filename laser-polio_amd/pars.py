@@ -16,8 +16,8 @@ synthetic workloads (all default to the reference behaviour):
   mix_mode                0: network mix in the oracle's fixed order (bit-identical replays);
                           1: the prevalence x network contraction on MFMA (float32 within 1e-6
                           of float64, not replay-identical)
-  ei_grid, sample_grid,   workgroups of the E / I list pass, the round-1 sampler and the round-1
-  commit_grid             commit (0 = the library's defaults; each 64..65535)
+  ei_grid, sample_grid    workgroups of the E / I list pass and the round-1 sampler (0 = the
+                          library's defaults; each 64..65535)
 """
 
 import datetime
@@ -106,7 +106,6 @@ default_pars = PropertySet(
         "mix_mode": 0,
         "ei_grid": 0,
         "sample_grid": 0,
-        "commit_grid": 0,
     }
 )
 
