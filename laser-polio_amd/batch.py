@@ -266,7 +266,12 @@ class BatchedSims:
                 for name in AGENT_COLUMNS}
         return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
 
-    GRID_CANDIDATES = {"ei_grid": (1024, 2048, 4096), "sample_grid": (1024, 2048, 4096)}
+    # 0: the library's default (the workgroups resident at once)
+    GRID_CANDIDATES = {"ei_grid": (0, 1024, 2048, 4096), "sample_grid": (0, 1024, 2048, 4096)}
+    TPW_DEFAULT = 16
+    # a trial of a few early days of a few simulations cannot separate candidates closer
+    # than this: the default stays unless another candidate beats it by more
+    TUNE_MARGIN = 0.03
 
     def _time_subset(self, ks, days, use_graph, tpw, grids):
         import torch
@@ -286,15 +291,25 @@ class BatchedSims:
         sub.close()
         return el
 
+    @classmethod
+    def _pick(cls, times, default):
+        """The fastest candidate if it beats `default` by more than TUNE_MARGIN, else `default`."""
+        best = min(times, key=times.get)
+        if default in times and times[best] >= times[default] * (1.0 - cls.TUNE_MARGIN):
+            return default
+        return best
+
     def autotune(self, candidates=(4, 8, 16), days=20, max_sims=32, use_graph=True, grids=False):
         """Pick tiles_per_block by timing `days` simulated days of the first `max_sims`
         simulations with each candidate (fresh contexts, so nothing of this batch runs);
         with `grids`, then the workgroup counts of the list pass and the round-1 sampler,
-        one after the other from GRID_CANDIDATES (the launch shapes of lp_config).  Returns (tiles_per_block, times by tiles_per_block), and records the
-        chosen grids in self.cfg and the trial times in self.tune_log."""
+        one after the other from GRID_CANDIDATES (the launch shapes of lp_config).  A
+        candidate replaces the default (16 tiles, the library's grids) only when it is
+        more than TUNE_MARGIN faster.  Returns (tiles_per_block, times by tiles_per_block),
+        and records the chosen grids in self.cfg and the trial times in self.tune_log."""
         ks = range(min(max_sims, len(self.packs)))
         times = {tpw: self._time_subset(ks, days, use_graph, tpw, {}) for tpw in candidates}
-        best = min(times, key=times.get)
+        best = self._pick(times, self.TPW_DEFAULT)
         self.tiles_per_block = best
         self.cfg.tiles_per_block = best
         self.tune_log = {"tiles_per_block": dict(times)}
@@ -302,7 +317,7 @@ class BatchedSims:
             chosen = {}
             for name, cands in self.GRID_CANDIDATES.items():
                 tried = {g: self._time_subset(ks, days, use_graph, best, {**chosen, name: g}) for g in cands}
-                chosen[name] = min(tried, key=tried.get)
+                chosen[name] = self._pick(tried, 0)
                 self.tune_log[name] = tried
             for name, g in chosen.items():
                 setattr(self.cfg, name, int(g))

@@ -3104,17 +3104,19 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int64_t per = (M + nw - 1) / nw;
   const int64_t p0 = (first + (threadIdx.x >> 6)) * per;
   const int64_t p1 = p0 + per < M ? p0 + per : M;
-  int node = -1, ns = 0, stile = 0, sntile = 0;   // lane l: span l of the node (first tile, tiles)
-  int64_t node_lo = 0, node_hi = 0, W = 0, s0 = 0, spx = INT64_MAX;
-  uint64_t seed = 0;
-  uint32_t lnode = 0;
-  // groups of up to 64 probes: resolved one at a time by the whole wave, lane q keeping
-  // probe g0 + q's agent; then every lane claims its own at once
+  // groups of up to 64 probes: resolved one at a time by the whole wave into the wave's
+  // LDS row (entry q: probe g0 + q's agent and node, its state word and index in the
+  // node), then every lane claims its own at once.  The node's data is reloaded per
+  // group and the row is in LDS, so neither is held in registers through the other
+  // phase (more wavefronts resident)
+  __shared__ uint4 s_res[WG / 64][64];
+  uint4* res = s_res[threadIdx.x >> 6];
   for (int64_t g0 = p0; g0 < p1; g0 += 64) {
     const int64_t g1 = g0 + 64 < p1 ? g0 + 64 : p1;
-    int64_t my_slot = -1, my_j = 0;
-    uint32_t my_word = 0;
-    int my_node = -1;
+    int node = -1, ns = 0, stile = 0, sntile = 0;   // lane l: span l of the node (first tile, tiles)
+    int64_t node_lo = 0, node_hi = 0, W = 0, spx = INT64_MAX;
+    uint64_t seed = 0;
+    uint32_t lnode = 0;
     for (int64_t p = g0; p < g1; ++p) {
       if (node < 0 || p >= node_hi) {
         node = __builtin_amdgcn_readfirstlane((int)find_node(off, N, p));
@@ -3125,7 +3127,6 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
         seed = ni.seed;
         lnode = (uint32_t)ni.lnode;
         W = d.W_round[node];
-        s0 = ni.span_begin;
         ns = ni.n_spans;
         spx = ns <= 64 ? sl.prefix : INT64_MAX;
         stile = sl.tile0;
@@ -3149,7 +3150,24 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
           slot = resolve_probe<false, true>(d, node, T, d.span_prefix, lane, &word);
         }
       }
-      if (lane == (int)(p - g0)) { my_slot = slot; my_word = word; my_node = node; my_j = j; }
+      if (lane == 0) {
+        const uint64_t e = (slot < 0 ? SEL_SLOT : (uint64_t)slot) | ((uint64_t)node << 40);
+        res[p - g0] = make_uint4((uint32_t)e, (uint32_t)(e >> 32), word, (uint32_t)j);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int64_t my_slot = -1, my_j = 0;
+    uint32_t my_word = 0;
+    int my_node = -1;
+    if (g0 + lane < g1) {
+      const uint4 r = res[lane];
+      const uint64_t e = (uint64_t)r.x | ((uint64_t)r.y << 32);
+      my_slot = (e & SEL_SLOT) == SEL_SLOT ? -1 : (int64_t)(e & SEL_SLOT);
+      my_node = (int)(e >> 40);
+      my_word = r.z;
+      my_j = r.w;
     }
     // the group's claims, lane q for probe g0 + q
     const int64_t p = g0 + lane;
@@ -3164,17 +3182,18 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
       if (rb >= 0) at = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin + rb + my_j;
       int64_t w = 0;
       if (my_slot >= 0) {
-        // everything the exposure reads, loaded beside the claim
-        uint64_t thr[LP_MAX_STRAINS];
-#pragma unroll
-        for (int q = 0; q < LP_MAX_STRAINS; ++q) {
-          thr[q] = d.thr_strain[(size_t)my_node * LP_MAX_STRAINS + q];
-          r0v[q] = ni.r0k[q];
-        }
         rec = d.rec[my_slot];   // the entry's record fields do not change when an agent is exposed
         w = weight_q24(d.risk[my_slot]);
         if (claim_from(d.state, my_slot, my_word)) {   // the first probe of an agent claims it
           won = true;
+          // the node's strain thresholds and r0 scalars only for a won claim (fewer
+          // registers held through the compare-and-swap: more wavefronts resident)
+          uint64_t thr[LP_MAX_STRAINS];
+#pragma unroll
+          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+            thr[q] = d.thr_strain[(size_t)my_node * LP_MAX_STRAINS + q];
+            r0v[q] = ni.r0k[q];
+          }
           k = strain_draw(d, thr, ni.seed, ni.slot_base, my_slot, t);
           *rec_byte(d, my_slot, RB_STRAIN) = (int8_t)k;
         }

@@ -122,9 +122,9 @@ def test_device_initialised_replicates_full_size():
 
 @pytest.mark.gpu
 def test_launch_shapes_do_not_change_results():
-    """The workgroup counts of the list pass, the sampler and its commit (lp_config) leave
-    every draw where it was: a batch with other grids still equals the oracle, and the
-    autotuner picks each from its candidates."""
+    """The workgroup counts of the list pass and the sampler (lp_config) leave every draw
+    where it was: a batch with other grids still equals the oracle, and the autotuner
+    picks each from its candidates."""
     print("[batch-grids] autotune and non-default grids", flush=True)
     from test_gpu_parity import _compare
 

@@ -173,3 +173,15 @@ def test_thresholds_round_trip():
     # r32 < thr(p)  <=>  r32 * 2^-32 < p
     for pi, ti in zip(p, thr):
         assert np.array_equal(r < ti, r.astype(np.float64) * 2.0**-32 < pi)
+
+
+def test_autotune_keeps_the_default_within_the_margin():
+    """A launch-shape candidate replaces the default only when its trial is more than
+    TUNE_MARGIN faster: short trials cannot separate closer candidates."""
+    pick = lp.BatchedSims._pick
+    m = lp.BatchedSims.TUNE_MARGIN
+    assert pick({0: 1.00, 1024: 0.99, 2048: 1.02}, 0) == 0
+    assert pick({0: 1.00, 1024: 1.0 - 2 * m, 2048: 1.02}, 0) == 1024
+    assert pick({4: 3.1, 8: 3.0, 16: 3.05}, 16) == 16
+    assert pick({4: 3.1, 8: 2.8, 16: 3.05}, 16) == 8
+    assert pick({4: 3.1, 8: 3.0}, 16) == 8          # no default among the candidates
