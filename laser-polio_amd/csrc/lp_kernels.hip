@@ -3182,18 +3182,17 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
       if (rb >= 0) at = d.ei_list + (size_t)nxt * (size_t)d.capacity + ni.seg_begin + rb + my_j;
       int64_t w = 0;
       if (my_slot >= 0) {
+        // everything the exposure reads, loaded beside the claim
+        uint64_t thr[LP_MAX_STRAINS];
+#pragma unroll
+        for (int q = 0; q < LP_MAX_STRAINS; ++q) {
+          thr[q] = d.thr_strain[(size_t)my_node * LP_MAX_STRAINS + q];
+          r0v[q] = ni.r0k[q];
+        }
         rec = d.rec[my_slot];   // the entry's record fields do not change when an agent is exposed
         w = weight_q24(d.risk[my_slot]);
         if (claim_from(d.state, my_slot, my_word)) {   // the first probe of an agent claims it
           won = true;
-          // the node's strain thresholds and r0 scalars only for a won claim (fewer
-          // registers held through the compare-and-swap: more wavefronts resident)
-          uint64_t thr[LP_MAX_STRAINS];
-#pragma unroll
-          for (int q = 0; q < LP_MAX_STRAINS; ++q) {
-            thr[q] = d.thr_strain[(size_t)my_node * LP_MAX_STRAINS + q];
-            r0v[q] = ni.r0k[q];
-          }
           k = strain_draw(d, thr, ni.seed, ni.slot_base, my_slot, t);
           *rec_byte(d, my_slot, RB_STRAIN) = (int8_t)k;
         }
