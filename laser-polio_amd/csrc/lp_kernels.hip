@@ -1730,7 +1730,16 @@ __device__ __forceinline__ void vd_chunk(const Dev& d, int64_t c, int node, int6
   const bool live_q = i0 < live_hi;
   const uint32_t w0 = live_q ? reinterpret_cast<const uint32_t*>(d.state)[i0 >> 2] : 0xFFFFFFFFu;
   int4 ddu = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
-  if (cm <= t && live_q) ddu = *reinterpret_cast<const int4*>(d.dod + i0);
+  const bool die_q = cm <= t && live_q;                       // a death may be due in the quad
+  if (die_q) ddu = *reinterpret_cast<const int4*>(d.dod + i0);
+  // the quad's other inputs load with its state and death dates: the stores below go
+  // through other pointers, which the compiler cannot move these loads past
+  const bool nb_q = BIRTHS && live_q && i0 + 3 >= nb_lo;     // the quad holds a newborn
+  const float4 rq = (nb_q || die_q) ? *reinterpret_cast<const float4*>(d.risk + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t mq = die_q ? reinterpret_cast<const uint32_t*>(d.missed)[i0 >> 2] : 0u;
+  const uint2 tq = (die_q || (nb_q && sp.newborn_ri_timer < 0)) ? *reinterpret_cast<const uint2*>(d.ri_timer + i0)
+                                                                : make_uint2(0u, 0u);
+  auto timer_of = [&](int a) { return (int16_t)(((a < 2 ? tq.x : tq.y) >> (16 * (a & 1))) & 0xffffu); };
   uint32_t w1 = w0;
   int32_t dmin = INT_MAX;
   int64_t cw = 0;
@@ -1746,10 +1755,10 @@ __device__ __forceinline__ void vd_chunk(const Dev& d, int64_t c, int node, int6
       d.dod[i] = dod;
       dmin = dod < dmin ? dod : dmin;
       d.node_id[i] = (int16_t)(node - sp.node_begin);   // sim-local node id
-      const int16_t base = sp.newborn_ri_timer >= 0 ? (int16_t)sp.newborn_ri_timer : d.ri_timer[i];
+      const int16_t base = sp.newborn_ri_timer >= 0 ? (int16_t)sp.newborn_ri_timer : timer_of(a);
       const int16_t col = (int16_t)(base + sh_prev);     // as of the reference tick
       if (sp.newborn_ri_timer >= 0 || col != base) d.ri_timer[i] = col;
-      const int64_t w = weight_q24(d.risk[i]);           // a newborn enters S
+      const int64_t w = weight_q24(lane_of(rq, a));      // a newborn enters S
       acc[VA_S] += 1; acc[VA_SPOS] += (w > 0); cw += w;
       s = 0;
     } else if (s >= 0 && lane_of(ddu, a) <= t) {         // get_deaths (model.py:1775-1777)
@@ -1758,9 +1767,9 @@ __device__ __forceinline__ void vd_chunk(const Dev& d, int64_t c, int node, int6
       if ((int8_t)(rd.y >> 24) == 1) acc[VA_DPP] += 1;
       if ((int8_t)(rd.w >> 8) == 1) acc[VA_DP] += 1;
       // its ri_timer stops here: the actual value (chronically missed agents are never decremented)
-      if (sh_prev != 0 && (int8_t)d.missed[i] != 1) d.ri_timer[i] = (int16_t)(d.ri_timer[i] - sh_prev);
+      if (sh_prev != 0 && byte_of(mq, a) != 1) d.ri_timer[i] = (int16_t)(timer_of(a) - sh_prev);
       if (s == 0) {
-        const int64_t w = weight_q24(d.risk[i]);
+        const int64_t w = weight_q24(lane_of(rq, a));
         acc[VA_S] -= 1; acc[VA_SPOS] -= (w > 0); cw -= w;
       } else if (s == 3) {
         acc[VA_R] -= 1;
@@ -1779,8 +1788,8 @@ __device__ __forceinline__ void vd_chunk(const Dev& d, int64_t c, int node, int6
   const int64_t cs = wave_sum_i64(cw);
   if (lane == 0) {
     if (nm != cm) d.chunk_min_dod[c] = nm;
-    if (cs != 0) {
-      d.chunk_sum[c] += cs;
+    if (cs != 0) {   // (no value returned: nothing waits for these)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + c), (unsigned long long)cs);
       atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)span_of(d, node, c * LP_CHUNK) * NPART + P_SW),
                 (unsigned long long)cs);
     }
