@@ -3109,7 +3109,8 @@ __global__ void __launch_bounds__(WG) k_sample(Dev d) {
   const int64_t base = lds ? 0 : d.round_base[1];
   // each wave takes a contiguous range of probes (probes are ordered by node); the
   // node's seed, weight and span prefix (one span per lane, up to 64 spans) are
-  // loaded once per node, so a probe costs the chunk-weight and the agent loads
+  // loaded once per node within a group of 64 probes, so a probe costs the
+  // chunk-weight and the agent loads
   const int64_t per = (M + nw - 1) / nw;
   const int64_t p0 = (first + (threadIdx.x >> 6)) * per;
   const int64_t p1 = p0 + per < M ? p0 + per : M;
