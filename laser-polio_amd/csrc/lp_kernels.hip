@@ -3261,17 +3261,41 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   if (m1 > 0 && d.probe_off[d.n_nodes] > 0) {
     // round 1's winners (claimed by k_sample, state SEL_MARK): E now, and their weights
     // leave the chunk and span sums before any later round reads them
+    // eight entries per thread with their loads in flight together; up to 64 spans the
+    // span weights are summed in LDS and leave with one atomic per span
+    __shared__ int64_t s_spw[64];
+    const bool lds_spans = ni.n_spans <= 64;
+    if (tid < 64) s_spw[tid] = 0;
+    __syncthreads();
     const int64_t b1 = d.round_base[1] + d.probe_off[n];
-    for (int j = tid; j < m1; j += WG) {
-      const uint64_t e = d.sel_list[b1 + j];
-      const uint32_t wq = d.sel_word[b1 + j];
-      if (e & SEL_WON) {
-        const int64_t slot = (int64_t)(e & SEL_SLOT);
-        const int64_t w = wq != 0xFFFFFFFFu ? (int64_t)wq : weight_q24(d.risk[slot]);
+    const int64_t tile0 = ni.seg_begin / LP_TILE;
+    constexpr int U = 8;
+    for (int j0 = 0; j0 < m1; j0 += U * WG) {
+      uint64_t e[U];
+      uint32_t wq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * WG + tid;
+        e[u] = j < m1 ? d.sel_list[b1 + j] : 0ull;
+        wq[u] = j < m1 ? d.sel_word[b1 + j] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!(e[u] & SEL_WON)) continue;
+        const int64_t slot = (int64_t)(e[u] & SEL_SLOT);
+        const int64_t w = wq[u] != 0xFFFFFFFFu ? (int64_t)wq[u] : weight_q24(d.risk[slot]);
         d.state[slot] = 1;
-        unweight_in(d, ni.span_begin + (slot / LP_TILE - ni.seg_begin / LP_TILE) / d.tpw, slot, w);
+        const unsigned long long dw = (unsigned long long)(-w);
+        atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
+        const int ls = (int)((slot / LP_TILE - tile0) / d.tpw);   // the node's span
+        if (lds_spans) atomicAdd(reinterpret_cast<unsigned long long*>(&s_spw[ls]), dw);
+        else atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)(ni.span_begin + ls) * NPART + P_SW), dw);
       }
     }
+    __syncthreads();
+    if (lds_spans && tid < ni.n_spans && s_spw[tid] != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)(ni.span_begin + tid) * NPART + P_SW),
+                (unsigned long long)s_spw[tid]);
     if (!later) return;
     __syncthreads();   // the weights and states before the later rounds
   } else if (!later) {
