@@ -3433,52 +3433,74 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
   if (tid == 0 && s_claimed) atomicAdd(d.n_uniq + n, s_claimed);
 }
 
-__device__ void day_end_node(const Dev& d, int n, int32_t t, int32_t day_flags_t, int mask, int tx_adjust) {
-  const int sim = d.node_sim[n];
-  const size_t N = d.n_nodes, row = (size_t)t * N + n;
-  const int K = d.K;
-  const int64_t* tl = d.node_tally + (size_t)n * NPART;
-  // every input up front (unconditional, in-bounds loads in flight together)
-  const int32_t stop = d.stop_day[sim];
-  const int32_t flags = stage_day_flags(day_flags_t, mask);
+// Per node: transmission incidence and the log rows of day t, in two parts as
+// day_begin: day_end_load (every input, the values the rows add to included, in flight
+// together) and day_end_apply (stores only).
+struct DayEndIn {
+  int32_t sim, stop, flags, s_t, r_t, potp, pr;
   int32_t ne[LP_MAX_STRAINS], ek[LP_MAX_STRAINS], ik[LP_MAX_STRAINS];
+  int32_t r_newexp, r_newexp_bs[LP_MAX_STRAINS], r_R;
+  int64_t carry_s, carry_spos;
+};
+__device__ __forceinline__ DayEndIn day_end_load(const Dev& d, int n, int32_t t, int32_t day_flags_t, int mask,
+                                                 int tx_adjust) {
+  DayEndIn in;
+  const int K = d.K;
+  const size_t row = (size_t)t * d.n_nodes + n;
+  const int64_t* tl = d.node_tally + (size_t)n * NPART;
+  in.sim = d.node_sim[n];
+  in.stop = d.stop_day[in.sim];
+  in.flags = stage_day_flags(day_flags_t, mask);
 #pragma unroll
   for (int k = 0; k < LP_MAX_STRAINS; ++k) {
-    ne[k] = (tx_adjust && k < K) ? d.newexp[n * LP_MAX_STRAINS + k] : 0;
-    ek[k] = k < K ? (int32_t)tl[P_E0 + k] : 0;
-    ik[k] = k < K ? (int32_t)tl[P_I0 + k] : 0;
+    in.ne[k] = (tx_adjust && k < K) ? d.newexp[n * LP_MAX_STRAINS + k] : 0;
+    in.ek[k] = k < K ? (int32_t)tl[P_E0 + k] : 0;
+    in.ik[k] = k < K ? (int32_t)tl[P_I0 + k] : 0;
+    in.r_newexp_bs[k] = k < K ? d.r_newexp_bs[row * K + k] : 0;
   }
-  const int32_t s_t = (int32_t)tl[P_S], r_t = (int32_t)tl[P_R];
-  const int32_t potp = d.potp_run[n], pr = d.p_run[n];
-  if (stop < t || !(flags & LP_DAY_TX)) return;
+  in.s_t = (int32_t)tl[P_S];
+  in.r_t = (int32_t)tl[P_R];
+  in.potp = d.potp_run[n];
+  in.pr = d.p_run[n];
+  in.r_newexp = d.r_newexp[row];
+  in.r_R = d.r_R[row];
+  in.carry_s = d.node_carry[(size_t)n * 3 + 0];
+  in.carry_spos = d.node_carry[(size_t)n * 3 + 1];
+  return in;
+}
+__device__ __forceinline__ void day_end_apply(const Dev& d, int n, int32_t t, int mask, int tx_adjust,
+                                              const DayEndIn& in) {
+  const size_t N = d.n_nodes, row = (size_t)t * N + n;
+  const int K = d.K;
+  if (in.stop < t || !(in.flags & LP_DAY_TX)) return;
   int32_t ne_tot = 0;
 #pragma unroll
-  for (int k = 0; k < LP_MAX_STRAINS; ++k) ne_tot += ne[k];
+  for (int k = 0; k < LP_MAX_STRAINS; ++k) ne_tot += in.ne[k];
   if ((mask & LP_STAGE_TRANSMISSION) && tx_adjust) {
-    d.r_newexp[row] += ne_tot;
+    d.r_newexp[row] = in.r_newexp + ne_tot;
 #pragma unroll
     for (int k = 0; k < LP_MAX_STRAINS; ++k)
-      if (k < K) d.r_newexp_bs[row * K + k] += ne[k];
+      if (k < K) d.r_newexp_bs[row * K + k] = in.r_newexp_bs[k] + in.ne[k];
     // every agent exposed by transmission left S and had a positive weight
-    d.node_carry[(size_t)n * 3 + 0] -= ne_tot;
-    d.node_carry[(size_t)n * 3 + 1] -= ne_tot;
+    d.node_carry[(size_t)n * 3 + 0] = in.carry_s - ne_tot;
+    d.node_carry[(size_t)n * 3 + 1] = in.carry_spos - ne_tot;
   }
   if (mask & LP_STAGE_LOG) {
-    int32_t e = 0, in = 0;
+    int32_t e = 0, i = 0;
 #pragma unroll
     for (int k = 0; k < LP_MAX_STRAINS; ++k) {
       if (k >= K) break;
-      d.r_Ebs[row * K + k] = ek[k] + ne[k];
-      d.r_Ibs[row * K + k] = ik[k];
-      e += ek[k] + ne[k]; in += ik[k];
+      d.r_Ebs[row * K + k] = in.ek[k] + in.ne[k];
+      d.r_Ibs[row * K + k] = in.ik[k];
+      e += in.ek[k] + in.ne[k]; i += in.ik[k];
     }
-    d.r_S[row] = s_t - ne_tot;
+    d.r_S[row] = in.s_t - ne_tot;
     d.r_E[row] = e;
-    d.r_I[row] = in;
-    if (e + in > 0) d.sim_cases[sim] = 1;   // for the next day's early-stop rule
-    d.r_R[row] += r_t;
-    d.r_potp[row] = potp;
-    d.r_p[row] = pr;
+    d.r_I[row] = i;
+    if (e + i > 0) d.sim_cases[in.sim] = 1;   // for the next day's early-stop rule
+    d.r_R[row] = in.r_R + in.r_t;
+    d.r_potp[row] = in.potp;
+    d.r_p[row] = in.pr;
   }
 }
 
@@ -3507,8 +3529,10 @@ __global__ void __launch_bounds__(WG) k_day_end(Dev d, int mask, int tx_adjust, 
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < d.n_nodes) {
     // the next day's inputs are loaded before today's log rows are stored
+    // every input of the day's end and of the next day's beginning loads before any store
+    const DayEndIn ein = day_end_load(d, n, t, dy.y, mask, tx_adjust);
     const DayBeginIn in = begin_next ? day_begin_load(d, n, t + 1, mask) : DayBeginIn{};
-    day_end_node(d, n, t, dy.y, mask, tx_adjust);
+    day_end_apply(d, n, t, mask, tx_adjust, ein);
     if (begin_next) day_begin_apply(d, n, t + 1, in);
   }
   if (advance) {
