@@ -2665,6 +2665,19 @@ __device__ void plan_scan(const Dev& d, int round) {
 // them in-lane for the first two tree levels and across lanes for the rest.
 // `off`: no transmission today (the node's sampling sizes are zeroed instead); a
 // stopped simulation is tested after the mix's loads are issued.
+// t / s, correctly rounded, for a step number s <= 64 that is the same on every lane:
+// t * y with y = RN(1/s) (lane s - 1 of ystep), refined by its exact residual
+// r = t - s * (t * y) (Markstein's correction).  Over every significand of t and
+// s = 1 .. 64 this equals the correctly rounded t / s (tests/test_step_quotient.py); it
+// holds at any exponent where the residual stays normal, which t >= 2^-90 ensures.
+__device__ __forceinline__ float step_quot(float t, int s, float ystep) {
+  const float y = __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ystep), s - 1));
+  const float q0 = t * y;
+  const float r = __builtin_fmaf(-(float)s, q0, t);
+  return __builtin_fmaf(r, y, q0);
+}
+
 __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t, int lane, bool off) {
   const int K = d.K;
   const int sim = ni.sim;
@@ -2768,23 +2781,50 @@ __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t,
     }
   }
   int64_t mine = 0;
+  // Each pass inverts one chunk per lane.  Lanes step together, so step s divides by the
+  // same s on every lane still stepping: steps 1 .. 64 take RN(1/s) from lane s - 1 of
+  // ystep (step_quot), later steps divide.  A stepping lane's numerator stays >= 2^-90:
+  // the last term added was >= ulp(F) / 2 >= 2^-48 (F >= e^-16), times lc >= 2^-25 (a
+  // smaller chunk has F = e^-lc = 1 and never steps; the quick path is kept off for a
+  // node whose last chunk is that small) or times (k + rc) omp >= omp >= 2^-40.
+  const float ystep = mode != 0 ? 1.0f / (float)(lane + 1) : 0.0f;
   if (mode == 1) {
     const float nf = floorf(ex / 16.0f);
     const int64_t n_full = (int64_t)nf;
     const float last = ex - 16.0f * nf;
     const int64_t nch = n_full + (last > 0.0f ? 1 : 0);
-    for (int64_t j = lane; j < nch; j += 64) {
-      const float lc = j < n_full ? 16.0f : last;
-      const uint4 o = philox((uint32_t)j, (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
-      const float u = u24(o.x);
-      float p = expf_lp(-lc), F = p;
+    const bool quick = !(last > 0.0f && last < 0x1p-25f);
+    for (int64_t j0 = 0; j0 < nch; j0 += 64) {
+      const int64_t j = j0 + lane;
+      float lc = 0.0f, u = 0.0f, p = 0.0f, F = 0.0f;
+      bool go = false;
+      if (j < nch) {
+        lc = j < n_full ? 16.0f : last;
+        const uint4 o = philox((uint32_t)j, (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
+        u = u24(o.x);
+        p = expf_lp(-lc);
+        F = p;
+        go = u >= F;
+      }
       int k = 0;
-      while (u >= F) {
+      if (quick) {
+        for (int st = 1; st <= 64 && __builtin_amdgcn_ballot_w64(go) != 0; ++st) {
+          const float q = step_quot(p * lc, st, ystep);
+          const float Fn = F + q;
+          k = go ? st : k;
+          p = q;
+          const bool more = go && Fn != F;
+          F = more ? Fn : F;
+          go = more && u >= Fn;
+        }
+      }
+      while (go) {
         k += 1;
         p = (p * lc) / (float)k;
         const float Fn = F + p;
         if (Fn == F) break;
         F = Fn;
+        go = u >= F;
       }
       mine += k;
     }
@@ -2798,18 +2838,40 @@ __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t,
       if (rc_max < 1) rc_max = 1;
       const int32_t n_full = ni.disp_r / rc_max, rest = ni.disp_r % rc_max;
       const int32_t nch = n_full + (rest > 0 ? 1 : 0);
-      for (int32_t j = lane; j < nch; j += 64) {
-        const int32_t rc = j < n_full ? rc_max : rest;
-        const uint4 o = philox((uint32_t)(j + 1), (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
-        const float u = u24(o.x);
-        float P = expf_lp(-((float)rc * lq)), F = P;
+      // (float)(k + rc) in 32 bits equals the 64-bit conversion while k + rc < 2^31
+      const bool quick = omp >= 0x1p-40f && rc_max <= (1 << 24);
+      for (int32_t j0 = 0; j0 < nch; j0 += 64) {
+        const int32_t j = j0 + lane;
+        int32_t rc = 0;
+        float u = 0.0f, P = 0.0f, F = 0.0f;
+        bool go = false;
+        if (j < nch) {
+          rc = j < n_full ? rc_max : rest;
+          const uint4 o = philox((uint32_t)(j + 1), (uint32_t)li, (uint32_t)t, ST_COUNT, ni.seed);
+          u = u24(o.x);
+          P = expf_lp(-((float)rc * lq));
+          F = P;
+          go = u >= F;
+        }
         int64_t k = 0;
-        while (u >= F) {
+        if (quick) {
+          for (int st = 1; st <= 64 && __builtin_amdgcn_ballot_w64(go) != 0; ++st) {
+            const float q1 = step_quot(P * ((float)(st - 1 + rc) * omp), st, ystep);
+            const float Fn = F + q1;
+            k = go ? st : k;
+            P = q1;
+            const bool more = go && Fn != F;
+            F = more ? Fn : F;
+            go = more && u >= Fn;
+          }
+        }
+        while (go) {
           P = (P * ((float)(k + rc) * omp)) / (float)(k + 1);
           k += 1;
           const float Fn = F + P;
           if (Fn == F) break;
           F = Fn;
+          go = u >= F;
         }
         mine += k;
       }
