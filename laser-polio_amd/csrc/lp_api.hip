@@ -104,6 +104,7 @@ struct lp_ctx {
   int32_t tpw = 4;
   int32_t agent_grid = 1;
   int32_t ei_grid = 2048, sample_grid = 2048;     // launch shapes of the list pass and the sampler
+  bool mix_wide = false;                          // k_mix_foi<4> (more nodes than resident waves)
   int32_t mix_tiles = 0;                           // mix_mode 1: 16-row tiles of the largest simulation
   std::vector<void*> allocs;
   // per day kind (plain, FULL_* combinations, their list forms): graph[kind][j] runs 2^j
@@ -342,7 +343,10 @@ int enqueue_day(lp_ctx* c, int32_t t, int kind, int mask, int tx_adjust, bool ad
   if (mask & LP_STAGE_TRANSMISSION) {
     if (c->cfg.mix_mode == 1)
       hipLaunchKernelGGL(lp::k_mix_mfma, dim3(c->cfg.n_sims, c->mix_tiles), dim3(64), 0, s, d);
-    hipLaunchKernelGGL(lp::k_mix_foi, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
+    if (c->mix_wide)
+      hipLaunchKernelGGL(lp::k_mix_foi<4>, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
+    else
+      hipLaunchKernelGGL(lp::k_mix_foi<1>, dim3(grid_of(N, lp::WG / 64)), dim3(lp::WG), 0, s, d, mask);
     // with the probe offsets in LDS k_sample plans round 1 itself
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
     if (!off_lds) hipLaunchKernelGGL(lp::k_plan, dim3(1), dim3(lp::WG), 0, s, d, 1);
@@ -504,6 +508,7 @@ int lp_prepare(lp_ctx* c) {
   Dev& d = c->d;
   d.capacity = g.capacity; d.n_nodes = N; d.n_sims = S; d.K = g.n_strains; d.nt = g.nt;
   d.n_spans = c->n_spans; d.tpw = tpw;
+  d.tpw_sh = __builtin_ctz((unsigned)tpw);
   {
     int dev = 0, cus = 0;
     HIPCHK(c, hipGetDevice(&dev));
@@ -519,6 +524,10 @@ int lp_prepare(lp_ctx* c) {
     const size_t off_lds = (N + 1 <= lp::LDS_OFF_MAX) ? (size_t)(N + 1) * sizeof(int64_t) : 0;
     if (!c->cfg.ei_grid) c->ei_grid = resident(reinterpret_cast<const void*>(lp::k_ei), 0, kEiGrid);
     if (!c->cfg.sample_grid) c->sample_grid = resident(reinterpret_cast<const void*>(lp::k_sample), off_lds, kSampleGrid);
+    // k_mix_foi: the variant held to 4 waves per SIMD once its workgroups outnumber those
+    // of the unrestricted variant resident at once
+    const int mix_blocks = (int)grid_of(N, lp::WG / 64);
+    c->mix_wide = mix_blocks > resident(reinterpret_cast<const void*>(lp::k_mix_foi<1>), 0, 1 << 30);
   }
   int4* spans_d; int32_t* nspan_d;
   int rc;

@@ -137,6 +137,7 @@ struct Dev {
   int32_t* resv;           // [nodes] first list position of today's round-1 exposures (-1: appended)
   uint32_t* ticket;   // [4] last-block-done counters
   int64_t capacity; int32_t n_nodes, n_sims, K, nt, n_spans, tpw;
+  int32_t tpw_sh;                  // tpw = 1 << tpw_sh (1, 2, 4, 8 or 16 tiles per span)
 };
 
 __device__ __forceinline__ bool sim_stopped(const Dev& d, int sim, int t) { return d.stop_day[sim] < t; }
@@ -700,7 +701,7 @@ __device__ __forceinline__ void unweight_in(const Dev& d, int64_t s, int64_t slo
 // The span of node `node` holding `slot`.
 __device__ __forceinline__ int64_t span_of(const Dev& d, int node, int64_t slot) {
   const int64_t tile = slot / LP_TILE, first = d.seg_begin[node] / LP_TILE;
-  return d.node_span_begin[node] + (tile - first) / d.tpw;
+  return d.node_span_begin[node] + ((tile - first) >> d.tpw_sh);   // tile >= first
 }
 
 // RI candidate lists: ri_list[b] holds, per node, the slots of live agents that are
@@ -2787,8 +2788,8 @@ __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t,
   // the last term added was >= ulp(F) / 2 >= 2^-48 (F >= e^-16), times lc >= 2^-25 (a
   // smaller chunk has F = e^-lc = 1 and never steps; the quick path is kept off for a
   // node whose last chunk is that small) or times (k + rc) omp >= omp >= 2^-40.
-  const float ystep = mode != 0 ? 1.0f / (float)(lane + 1) : 0.0f;
   if (mode == 1) {
+    const float ystep = 1.0f / (float)(lane + 1);
     const float nf = floorf(ex / 16.0f);
     const int64_t n_full = (int64_t)nf;
     const float last = ex - 16.0f * nf;
@@ -2840,6 +2841,7 @@ __device__ void mix_foi_node(const Dev& d, int i, const NodeInfo& ni, int32_t t,
       const int32_t nch = n_full + (rest > 0 ? 1 : 0);
       // (float)(k + rc) in 32 bits equals the 64-bit conversion while k + rc < 2^31
       const bool quick = omp >= 0x1p-40f && rc_max <= (1 << 24);
+      const float ystep = 1.0f / (float)(lane + 1);
       for (int32_t j0 = 0; j0 < nch; j0 += 64) {
         const int32_t j = j0 + lane;
         int32_t rc = 0;
@@ -2946,8 +2948,12 @@ __global__ void __launch_bounds__(64) k_mix_mfma(Dev d) {
   }
 }
 
-// Per node: network mix, FOI, count draw.  One wavefront per node.
-__global__ void __launch_bounds__(WG) k_mix_foi(Dev d, int mask) {
+// Per node: network mix, FOI, count draw.  One wavefront per node.  MINW = 4 keeps at
+// least 4 waves per SIMD (<= 128 VGPRs; one 8-byte value spills, stored at entry and
+// reloaded once), for more nodes than are resident at once, where the network reads need
+// the occupancy; MINW = 1 leaves the kernel unspilled for fewer nodes.
+template <int MINW>
+__global__ void __launch_bounds__(WG, MINW) k_mix_foi(Dev d, int mask) {
   const int4 dy = today(d);
   const int32_t t = dy.x;
   const int i = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
@@ -3349,7 +3355,7 @@ __global__ void __launch_bounds__(WG) k_finish(Dev d) {
         d.state[slot] = 1;
         const unsigned long long dw = (unsigned long long)(-w);
         atomicAdd(reinterpret_cast<unsigned long long*>(d.chunk_sum + (slot >> 8)), dw);
-        const int ls = (int)((slot / LP_TILE - tile0) / d.tpw);   // the node's span
+        const int ls = (int)((slot / LP_TILE - tile0) >> d.tpw_sh);   // the node's span
         if (lds_spans) atomicAdd(reinterpret_cast<unsigned long long*>(&s_spw[ls]), dw);
         else atomicAdd(reinterpret_cast<unsigned long long*>(d.partials + (size_t)(ni.span_begin + ls) * NPART + P_SW), dw);
       }
