@@ -774,8 +774,9 @@ def continental_single(scale):
 
 def continental_batched(seeds, scale):
     """`seeds` replicate seeds of the continental geography at population `scale`,
-    batched in one library context (every kernel launch of a day covers all of them);
-    each seed's results checked by result_properties."""
+    batched in one library context (every kernel launch of a day covers all of them;
+    their identical networks stored once, share_networks); each seed's results checked by
+    result_properties."""
     import torch
 
     __graft_entry__.load_package()
@@ -784,11 +785,13 @@ def continental_batched(seeds, scale):
 
     plist = [configs.continental_pars(scale=scale, run_seed=42 + k) for k in range(seeds)]
     tb0 = time.perf_counter()
-    batch = build_replicates_device(plist, configs.build)
+    batch = build_replicates_device(plist, configs.build, share_networks=True)
+    n_blocks = len(set(batch.net_off.tolist()))
     try:
         beng = batch.prepare()
         torch.cuda.synchronize()
         log(f"[bench] {seeds} seeds at scale {scale}: {batch.cfg.n_nodes} nodes, capacity {batch.cfg.capacity}, "
+            f"{n_blocks} network block(s), "
             f"built and drawn in {time.perf_counter() - tb0:.1f}s")
         t0 = time.perf_counter()
         beng.run_days(0, batch.cfg.nt, use_graph=True)
@@ -807,7 +810,7 @@ def continental_batched(seeds, scale):
         batch.close()
         torch.cuda.empty_cache()
     out = {"seeds": seeds, "scale": scale, "agents": int(sum(int(q.init_pop.sum()) for q in plist)),
-           "capacity": int(batch.cfg.capacity), "seconds": round(el, 4), "agent_days_per_s": round(ad / el, 1),
+           "capacity": int(batch.cfg.capacity), "network_blocks": n_blocks, "seconds": round(el, 4), "agent_days_per_s": round(ad / el, 1),
            "properties_ok": all(q["ok"] for q in props), "properties": props}
     log(f"[bench] batched seeds: {out['agents']} agents, 365 days in {el:.3f}s -> {ad / el:.4e} agent-days/s; "
         f"properties ok: {out['properties_ok']}")

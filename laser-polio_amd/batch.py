@@ -24,6 +24,11 @@ objects.  ``build_replicates_device(pars_list)`` draws every simulation's agent
 columns on the GPU instead (lp_init_population, one spec per simulation, draws
 keyed by sim-local slot, so a simulation in a batch equals the same simulation
 initialised alone); the host only builds node tables and schedules.
+
+With ``share_networks=True`` simulations whose transposed networks are identical
+(replicate seeds of one geography) read one copy of it: the network mix streams each
+row once for all of them.  ``set_gravity`` then refuses, since it builds one network
+per simulation.
 """
 
 import multiprocessing as mp
@@ -58,20 +63,21 @@ class SimPack:
 
 
 class BatchedSims:
-    def __init__(self, sims, tiles_per_block=0, timing=False):
+    def __init__(self, sims, tiles_per_block=0, timing=False, share_networks=False):
         if not sims:
             raise ValueError("BatchedSims needs at least one simulation")
-        self._init([SimPack.of(s) for s in sims], None, tiles_per_block, timing)
+        self._init([SimPack.of(s) for s in sims], None, tiles_per_block, timing, share_networks)
 
     @classmethod
-    def from_packs(cls, packs, cols=None, tiles_per_block=0, timing=False, owner=None):
+    def from_packs(cls, packs, cols=None, tiles_per_block=0, timing=False, owner=None, share_networks=False):
         """Batch of SimPacks; `cols` = the already concatenated agent columns, if any."""
         self = cls.__new__(cls)
-        self._init(list(packs), cols, tiles_per_block, timing)
+        self._init(list(packs), cols, tiles_per_block, timing, share_networks)
         self._owner = owner
         return self
 
-    def _init(self, packs, cols, tiles_per_block, timing):
+    def _init(self, packs, cols, tiles_per_block, timing, share_networks=False):
+        self.share_networks = bool(share_networks)
         self.packs = packs
         self.sims = [p.sim for p in packs]
         self.tables = [p.tables for p in packs]
@@ -102,8 +108,21 @@ class BatchedSims:
         nns = np.array([tb.n_nodes for tb in tbs], np.int64)
         self.slot_off = np.concatenate([[0], np.cumsum(caps)])
         self.node_off = np.concatenate([[0], np.cumsum(nns)])
-        net_off = np.concatenate([[0], np.cumsum(nns * nns)])
-        self.net_off = net_off
+        # network blocks: one per simulation, or with share_networks one per distinct network
+        net_off = np.zeros(S, np.int64)
+        blocks, first_of, pos = [], {}, 0
+        for i, tb in enumerate(tbs):
+            if self.share_networks:
+                key = (int(nns[i]), hash(np.ascontiguousarray(tb.network_t).tobytes()))
+                j = first_of.get(key)
+                if j is not None and np.array_equal(tbs[j].network_t, tb.network_t):
+                    net_off[i] = net_off[j]
+                    continue
+                first_of[key] = i
+            net_off[i] = pos
+            blocks.append(tb.network_t)
+            pos += int(nns[i] * nns[i])
+        self.net_off = net_off          # each simulation's first network_t entry
         N = int(self.node_off[-1])
         if cols is None:
             cols = {name: np.concatenate([tb.cols[name] for tb in tbs]) for name in AGENT_COLUMNS}
@@ -117,10 +136,10 @@ class BatchedSims:
             arrays[key] = np.concatenate([getattr(tb, attr) for tb in tbs])
         sp = np.concatenate([p.sim_params for p in self.packs])
         sp["slot_base"] = self.slot_off[:-1]
-        sp["net_off"] = net_off[:-1]
+        sp["net_off"] = net_off
         sp["node_begin"] = self.node_off[:-1]
         arrays["sim_params"] = sp
-        arrays["network_t"] = np.concatenate([tb.network_t for tb in tbs])
+        arrays["network_t"] = np.concatenate(blocks)
         arrays["km_cd"] = np.concatenate([tb.km_cd for tb in tbs])
         arrays["day_flags"] = tbs[0].day_flags
         arrays["day_sia_begin"] = tbs[0].day_sia_begin
@@ -161,6 +180,9 @@ class BatchedSims:
         import torch
 
         S = len(self.tables)
+        if len(set(self.net_off.tolist())) < S:
+            raise ValueError("set_gravity builds one network per simulation; this batch shares networks "
+                             "(share_networks=True)")
         if any(getattr(tb, "gravity_spec", None) is None for tb in self.tables):
             raise ValueError("set_gravity needs gravity networks (migration_method 'gravity') in every simulation")
         cols = []
@@ -180,7 +202,8 @@ class BatchedSims:
             eng.pull("network_t", self.arrays["network_t"])
             eng.pull("keep", self.arrays["keep"])
             for i, tb in enumerate(self.tables):
-                tb.network_t = self.arrays["network_t"][self.net_off[i]:self.net_off[i + 1]].copy()
+                n = tb.n_nodes
+                tb.network_t = self.arrays["network_t"][self.net_off[i]:self.net_off[i] + n * n].copy()
                 tb.keep = self.arrays["keep"][self.node_off[i]:self.node_off[i + 1]].copy()
         return specs
 
@@ -259,12 +282,14 @@ class BatchedSims:
         if self.device_init is not None:
             total = int(sum(self.slot_off[k + 1] - self.slot_off[k] for k in ks))
             cols = {name: np.empty(total, dtype=dt) for name, (dt, _) in AGENT_COLUMNS.items()}
-            sub = BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
+            sub = BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing,
+                                         share_networks=self.share_networks)
             sub.device_init = ([self.device_init[0][k] for k in ks], [self.device_init[1][k] for k in ks])
             return sub
         cols = {name: np.concatenate([self.arrays[name][self.slot_off[k]:self.slot_off[k + 1]] for k in ks])
                 for name in AGENT_COLUMNS}
-        return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing)
+        return BatchedSims.from_packs(packs, cols, tiles_per_block=tpw, timing=self.timing,
+                                      share_networks=self.share_networks)
 
     # 0: the library's default (the workgroups resident at once)
     GRID_CANDIDATES = {"ei_grid": (0, 1024, 2048, 4096), "sample_grid": (0, 1024, 2048, 4096)}
@@ -409,7 +434,7 @@ def build_replicates(pars_list, build, workers=8, tiles_per_block=0, timing=Fals
         raise
 
 
-def build_replicates_device(pars_list, build, tiles_per_block=0, timing=False, log=None):
+def build_replicates_device(pars_list, build, tiles_per_block=0, timing=False, log=None, share_networks=False):
     """BatchedSims of `build(pars)` for every pars in `pars_list`, their agent columns drawn
     on the GPU when the batch is prepared (the host builds only node tables, schedules and
     the exact-count initial infections)."""
@@ -426,6 +451,7 @@ def build_replicates_device(pars_list, build, tiles_per_block=0, timing=False, l
             log(f"[batch] built {k + 1}/{len(pars_list)} simulations (population drawn on the GPU)")
     total = int(sum(pk.tables.capacity for pk in packs))
     cols = {name: np.empty(total, dtype=dt) for name, (dt, _) in AGENT_COLUMNS.items()}
-    batch = BatchedSims.from_packs(packs, cols, tiles_per_block=tiles_per_block, timing=timing)
+    batch = BatchedSims.from_packs(packs, cols, tiles_per_block=tiles_per_block, timing=timing,
+                                   share_networks=share_networks)
     batch.device_init = (specs, infected)
     return batch

@@ -48,6 +48,57 @@ def test_batch_packing():
         assert np.array_equal(a["pop"][0, na:nb], sims[k].pars.init_pop)
 
 
+def _same_geography(n, i=0, scale=0.004, dur=40):
+    """`n` seeds of replicate `i`'s geography (identical networks)."""
+    sims = []
+    for k in range(n):
+        p = configs.replicate_pars(i, scale=scale, dur=dur)
+        p.seed = 7000 + k
+        sims.append(configs.build(p))
+    return sims
+
+
+def test_shared_networks_are_stored_once():
+    sims = _same_geography(3) + [configs.build(configs.replicate_pars(1, scale=0.004, dur=40))]
+    tb = [s.export_tables() for s in sims]
+    n0, n1 = tb[0].n_nodes, tb[3].n_nodes
+    assert len({t.seed for t in tb}) == 4
+    assert all(np.array_equal(t.network_t, tb[0].network_t) for t in tb[:3])
+    b = lp.BatchedSims(sims, share_networks=True)
+    assert list(b.net_off) == [0, 0, 0, n0 * n0]
+    assert list(b.arrays["sim_params"]["net_off"]) == list(b.net_off)
+    assert b.arrays["network_t"].size == n0 * n0 + n1 * n1
+    for k, t in enumerate(tb):
+        n = t.n_nodes
+        assert np.array_equal(b.arrays["network_t"][b.net_off[k]:b.net_off[k] + n * n], t.network_t)
+    with pytest.raises(ValueError, match="shares networks"):
+        b.set_gravity(k=0.5)
+    u = lp.BatchedSims(_same_geography(3))
+    assert list(u.net_off) == [0, n0 * n0, 2 * n0 * n0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("other", [False, True])
+def test_shared_networks_match_each_sim_alone(other):
+    """Seeds of one geography reading one network copy: each ends bit-identical to the
+    oracle run of that simulation alone (which holds its own copy), with and without
+    another geography in the batch."""
+    print(f"[batch-shared-network] 3 seeds, other geography {other}", flush=True)
+    from test_gpu_parity import _compare
+
+    sims = _same_geography(3)
+    if other:
+        sims.append(configs.build(configs.replicate_pars(1, scale=0.004, dur=40)))
+    refs = [OracleSim(s.export_tables(), pop0=s.pars.init_pop).run() for s in sims]
+    b = lp.BatchedSims(sims, share_networks=True)
+    assert len(set(b.net_off.tolist())) == (2 if other else 1)
+    b.run(use_graph=True)
+    for s, ref in zip(sims, refs):
+        assert s.t == ref.ticks_run
+        _compare(s, ref)
+    b.close()
+
+
 def test_batch_rejects_mismatched_schedules():
     a = configs.build(configs.replicate_pars(0, scale=0.004, dur=40))
     b = configs.build(configs.replicate_pars(1, scale=0.004, dur=50))
