@@ -159,7 +159,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the library runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the nodes over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
